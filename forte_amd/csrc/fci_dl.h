@@ -1,0 +1,91 @@
+// fci_dl.h -- Davidson-Liu eigensolver with device-resident vectors
+// (forte/helpers/davidson_liu_solver.h:52-240, davidson_liu_solver.cc:45-731).
+#pragma once
+
+#include "fci_internal.h"
+
+#include <functional>
+
+namespace fci {
+
+using SparseVec = std::vector<std::pair<int64_t, double>>;
+
+// Symmetric eigensolver (cyclic Jacobi), ascending; V column-major: eigenvector k at V[k*n..].
+void jacobi_eigh(int n, const std::vector<double>& A, std::vector<double>& evals, std::vector<double>& V);
+
+class DavidsonLiu {
+  public:
+    // sigma(b, sigma, nvec): nvec consecutive vectors of length size (device pointers)
+    using SigmaFn = std::function<void(const double* b, double* sigma)>;
+    // optional replacement of form_correction_vectors: (residuals r [nroot x size], lambda) -> r
+    using PrecondFn = std::function<bool(double* r, const double* lambda, int nroot, int iter)>;
+
+    DavidsonLiu(int64_t size, int nroot, int collapse_per_root, int subspace_per_root);
+
+    void add_h_diag_host(const double* h, int64_t n);
+    void add_h_diag_device(const double* d_h, int64_t n);
+    void add_guesses(const std::vector<SparseVec>& g) { guesses_ = g; }
+    void add_project_out_vectors(const std::vector<SparseVec>& v);
+    void add_sigma_builder(SigmaFn f) { sigma_ = std::move(f); }
+    void set_preconditioner(PrecondFn f) { precond_ = std::move(f); }
+    // called after the residual norms of an iteration are known; returning true ends the solve
+    // as converged (used to end a reduced-precision phase once it stalls)
+    using IterFn = std::function<bool(int iter, const std::vector<double>& resid, const std::vector<double>& lambda)>;
+    void set_iteration_observer(IterFn f) { observe_ = std::move(f); }
+    void set_e_convergence(double v) { e_conv_ = v; }
+    void set_r_convergence(double v) { r_conv_ = v; }
+    void set_maxiter(int v) { maxiter_ = v; }
+    void set_print_level(int v) { print_ = v; }
+    void reset();
+    bool solve();
+
+    int64_t size() const { return size_; }
+    int nroot() const { return nroot_; }
+    const std::vector<double>& eigenvalues() const { return lambda_; }
+    const double* eigenvector_device(int k) const { return b_.p + size_t(k) * size_; }
+    void eigenvector_host(int k, double* out) const;
+    const double* hdiag_device() const { return hdiag_.p; }
+    int iterations() const { return iters_; }
+    int sigma_builds() const { return nsigma_; }
+    const std::vector<double>& residual_norms() const { return resid_; }
+    int basis_size() const { return basis_size_; }
+
+  private:
+    int64_t size_;
+    int nroot_, collapse_size_, subspace_size_;
+    int basis_size_ = 0, sigma_size_ = 0;
+    int maxiter_ = 50, print_ = 0;
+    double e_conv_ = 1.0e-12, r_conv_ = 1.0e-6;
+    double discard_ = 1.0e-7, orth_ = 1.0e-12;
+    SigmaFn sigma_;
+    PrecondFn precond_;
+    IterFn observe_;
+    bool have_hdiag_ = false;
+    DBuf<double> hdiag_, b_, sigma_v_, r_, temp_;
+    DBuf<double> part_;  // reduction partials
+    DBuf<double> small_; // coefficients uploaded per call
+    DBuf<int64_t> pidx_;
+    DBuf<double> pval_;
+    std::vector<int64_t> poff_;
+    std::vector<SparseVec> guesses_;
+    std::vector<double> lambda_, lambda_old_, resid_, alpha_;
+    std::vector<double> G_, S_;  // cached b.sigma and b.b (subspace x subspace)
+    int iters_ = 0, nsigma_ = 0;
+    int gvalid_ = 0;  // G_ entries [i][j] with i,j < gvalid_ are current
+    uint64_t rng_ = 0x5EEDULL;
+
+    // device building blocks
+    void dots(const double* x, const double* Y, int m, double* out);  // out[i] = x . Y_i
+    void lincomb(const double* X, int m, const double* coef /*m x k, col-major per output*/, int k,
+                 double* out);
+    double add_row_and_orthonormalize(double* A, int rowsA, const double* v_src, bool* ok);
+    int add_rows_and_orthonormalize(double* A, int rowsA, const double* B, int rowsB);
+    void project_out(double* v, int nvec);
+    void fill_random(double* v, int nvec);
+    void rotate(const double* X, double* out, int kout);  // out_k = sum_j alpha[j][k] X_j
+    void form_G();
+    void check_orthonormality();
+    void get_results();
+};
+
+}  // namespace fci

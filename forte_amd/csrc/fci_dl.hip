@@ -1,0 +1,634 @@
+// fci_dl.hip -- Davidson-Liu on the device, following forte/helpers/davidson_liu_solver.cc.
+// The subspace problem (at most a few dozen vectors) is diagonalized on the host; every
+// operation on vectors of the full space runs in FP64 on the GPU.
+#include "fci_dl.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace fci {
+
+static inline unsigned nblk(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+void jacobi_eigh(int n, const std::vector<double>& A0, std::vector<double>& evals, std::vector<double>& Vout) {
+    std::vector<double> A(A0), V(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double a = A[size_t(i) * n + j];
+                tot += a * a;
+                if (i != j) off += a * a;
+            }
+        if (off == 0.0 || off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = A[size_t(p) * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double app = A[size_t(p) * n + p], aqq = A[size_t(q) * n + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A[size_t(k) * n + p], akq = A[size_t(k) * n + q];
+                    A[size_t(k) * n + p] = c * akp - s * akq;
+                    A[size_t(k) * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A[size_t(p) * n + k], aqk = A[size_t(q) * n + k];
+                    A[size_t(p) * n + k] = c * apk - s * aqk;
+                    A[size_t(q) * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = V[size_t(k) * n + p], vkq = V[size_t(k) * n + q];
+                    V[size_t(k) * n + p] = c * vkp - s * vkq;
+                    V[size_t(k) * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return A[size_t(a) * n + a] < A[size_t(b) * n + b]; });
+    evals.resize(n);
+    Vout.assign(size_t(n) * n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        evals[k] = A[size_t(idx[k]) * n + idx[k]];
+        for (int i = 0; i < n; ++i) Vout[size_t(k) * n + i] = V[size_t(i) * n + idx[k]];
+    }
+}
+
+// ---- kernels -----------------------------------------------------------------------------------
+constexpr int DL_BS = 256;
+constexpr int DL_MAXM = 64;
+constexpr int DL_NBLK = 1024;  // blocks of the reduction kernels
+
+// out_part[blk][i] = sum over this block's elements of x[e] * Y_i[e]  (i < m)
+__global__ __launch_bounds__(DL_BS) void k_dots(const double* __restrict__ x, const double* __restrict__ Y,
+                                                int m, int64_t n, double* __restrict__ part) {
+    __shared__ double red[DL_MAXM][DL_BS / 64];
+    double acc[DL_MAXM];
+#pragma unroll
+    for (int i = 0; i < DL_MAXM; ++i) acc[i] = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double xv = x[e];
+#pragma unroll
+        for (int i = 0; i < DL_MAXM; ++i)
+            if (i < m) acc[i] += xv * Y[size_t(i) * n + e];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < DL_MAXM; ++i) {
+        if (i >= m) break;
+        double v = acc[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[i][wave] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += DL_BS) {
+        double s = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) s += red[i][w];
+        part[size_t(blockIdx.x) * m + i] = s;
+    }
+}
+
+__global__ void k_reduce_parts(const double* __restrict__ part, int nparts, int m, double* __restrict__ out) {
+    int i = blockIdx.x;
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += part[size_t(p) * m + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
+}
+
+// out_k[e] = sum_j coef[k*m + j] X_j[e]   (k < kout <= 8)
+__global__ __launch_bounds__(DL_BS) void k_lincomb(const double* __restrict__ X, int m, const double* __restrict__ coef,
+                                                   int kout, int64_t n, double* __restrict__ out) {
+    __shared__ double cf[DL_MAXM * 8];
+    for (int i = threadIdx.x; i < m * kout; i += DL_BS) cf[i] = coef[i];
+    __syncthreads();
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < m; ++j) {
+            double xv = X[size_t(j) * n + e];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (k < kout) acc[k] += cf[k * m + j] * xv;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < kout) out[size_t(k) * n + e] = acc[k];
+    }
+}
+
+// v -= sum_i d[i] A_i ; part[blk] = partial ||v||^2
+__global__ __launch_bounds__(DL_BS) void k_gs_update(double* __restrict__ v, const double* __restrict__ A, int m,
+                                                     const double* __restrict__ d, int64_t n, double* __restrict__ part) {
+    __shared__ double dd[DL_MAXM];
+    __shared__ double red[DL_BS / 64];
+    for (int i = threadIdx.x; i < m; i += DL_BS) dd[i] = d[i];
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double x = v[e];
+        for (int i = 0; i < m; ++i) x -= dd[i] * A[size_t(i) * n + e];
+        v[e] = x;
+        s2 += x * x;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    if (lane == 0) red[wave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_scale(double* __restrict__ v, int64_t n, double a) {
+    int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n) v[e] *= a;
+}
+
+// residual r = sum_j a_j sigma_j - lambda sum_j a_j b_j; part = ||r||^2 partial; then the
+// diagonal preconditioner r /= (lambda - h) with |denom| <= 1e-6 -> 0 (davidson_liu_solver.cc:412-448)
+__global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ Sg, const double* __restrict__ B, int m,
+                                                    const double* __restrict__ a, double lambda,
+                                                    const double* __restrict__ h, int precond, int64_t n,
+                                                    double* __restrict__ r, double* __restrict__ part) {
+    __shared__ double ca[DL_MAXM];
+    __shared__ double red[DL_BS / 64];
+    for (int i = threadIdx.x; i < m; i += DL_BS) ca[i] = a[i];
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double rs = 0.0, tb = 0.0;
+        for (int j = 0; j < m; ++j) {
+            rs += ca[j] * Sg[size_t(j) * n + e];
+            tb += ca[j] * B[size_t(j) * n + e];
+        }
+        double x = rs - lambda * tb;
+        s2 += x * x;
+        if (precond) {
+            double den = lambda - h[e];
+            x = (fabs(den) > 1.0e-6) ? x / den : 0.0;
+        }
+        r[e] = x;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    if (lane == 0) red[wave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// project_out_roots (davidson_liu_solver.cc:590-603): sequential over bad roots, one block
+__global__ void k_project_out(double* __restrict__ v, const int64_t* __restrict__ off, int nbad,
+                              const int64_t* __restrict__ idx, const double* __restrict__ val) {
+    __shared__ double red[256];
+    for (int b = 0; b < nbad; ++b) {
+        double s = 0.0;
+        for (int64_t e = off[b] + threadIdx.x; e < off[b + 1]; e += blockDim.x) s += v[idx[e]] * val[e];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        double ov = red[0];
+        __syncthreads();
+        // entries of one sparse vector have distinct indices
+        for (int64_t e = off[b] + threadIdx.x; e < off[b + 1]; e += blockDim.x) v[idx[e]] -= ov * val[e];
+        __syncthreads();
+    }
+}
+
+__global__ void k_random(double* __restrict__ v, int64_t n, uint64_t seed) {
+    int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ULL * uint64_t(e + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    v[e] = 2.0 * ((double(z >> 11) + 0.5) * 0x1.0p-53) - 1.0;
+}
+
+// ---- DavidsonLiu -------------------------------------------------------------------------------
+DavidsonLiu::DavidsonLiu(int64_t size, int nroot, int cpr, int spr) : size_(size), nroot_(nroot) {
+    // startup(), davidson_liu_solver.cc:60-103
+    if (size == 0) throw Error(-1, "DavidsonLiuSolver called with space of dimension zero.");
+    if (nroot == 0) throw Error(-1, "DavidsonLiuSolver called with zero roots.");
+    if (cpr < 1)
+        throw Error(-1, "DavidsonLiuSolver: collapse_per_root_ (" + std::to_string(cpr) +
+                            ") must be greater or equal to the number of roots (" + std::to_string(nroot) + ")");
+    if (spr < cpr + 1)
+        throw Error(-1, "DavidsonLiuSolver: subspace_per_root_ (" + std::to_string(spr) +
+                            ") must be greater or equal to collapse_per_root_ + 1 (" + std::to_string(cpr + 1) + ")");
+    collapse_size_ = int(std::min<int64_t>(int64_t(cpr) * nroot, size));
+    subspace_size_ = int(std::min<int64_t>(int64_t(spr) * nroot, size));
+    if (subspace_size_ > DL_MAXM) throw Error(-1, "DavidsonLiuSolver: subspace larger than 64 vectors");
+    if (nroot_ > 8 && nroot_ > 0) {
+        // rotations write at most 8 vectors per pass; larger counts are split in rotate()
+    }
+    size_t m = size_t(subspace_size_);
+    b_.alloc(m * size_);
+    sigma_v_.alloc(m * size_);
+    temp_.alloc(m * size_);
+    r_.alloc(m * size_);
+    part_.alloc(size_t(DL_NBLK) * DL_MAXM + DL_MAXM);
+    small_.alloc(size_t(DL_MAXM) * 8);
+    HIPCHK(hipMemset(b_.p, 0, m * size_ * sizeof(double)));
+    HIPCHK(hipMemset(sigma_v_.p, 0, m * size_ * sizeof(double)));
+    lambda_.assign(m, 0.0);
+    lambda_old_.assign(m, 0.0);
+    resid_.assign(nroot_, 0.0);
+    alpha_.assign(m * m, 0.0);
+    G_.assign(m * m, 0.0);
+    S_.assign(m * m, 0.0);
+}
+
+void DavidsonLiu::add_h_diag_host(const double* h, int64_t n) {
+    if (n != size_)
+        throw Error(-1, "DavidsonLiuSolver: h_diag vector size (" + std::to_string(n) +
+                            ") must be equal to space size (" + std::to_string(size_) + ")");
+    if (!hdiag_.p) hdiag_.alloc(size_);
+    hdiag_.upload(h, n);
+    have_hdiag_ = true;
+}
+void DavidsonLiu::add_h_diag_device(const double* d, int64_t n) {
+    if (n != size_)
+        throw Error(-1, "DavidsonLiuSolver: h_diag vector size (" + std::to_string(n) +
+                            ") must be equal to space size (" + std::to_string(size_) + ")");
+    if (!hdiag_.p) hdiag_.alloc(size_);
+    HIPCHK(hipMemcpy(hdiag_.p, d, n * sizeof(double), hipMemcpyDeviceToDevice));
+    have_hdiag_ = true;
+}
+
+void DavidsonLiu::add_project_out_vectors(const std::vector<SparseVec>& v) {
+    poff_.assign(1, 0);
+    std::vector<int64_t> idx;
+    std::vector<double> val;
+    for (const auto& x : v) {
+        for (const auto& [i, c] : x) {
+            if (i < 0 || i >= size_) throw Error(-1, "DavidsonLiuSolver: project-out index out of range");
+            idx.push_back(i);
+            val.push_back(c);
+        }
+        poff_.push_back(int64_t(idx.size()));
+    }
+    if (!idx.empty()) {
+        pidx_.alloc(idx.size());
+        pidx_.upload(idx.data(), idx.size());
+        pval_.alloc(val.size());
+        pval_.upload(val.data(), val.size());
+    }
+}
+
+void DavidsonLiu::reset() {
+    // davidson_liu_solver.cc:145-150
+    basis_size_ = 0;
+    sigma_size_ = 0;
+    HIPCHK(hipMemset(b_.p, 0, b_.n * sizeof(double)));
+    HIPCHK(hipMemset(sigma_v_.p, 0, sigma_v_.n * sizeof(double)));
+}
+
+void DavidsonLiu::eigenvector_host(int k, double* out) const {
+    HIPCHK(hipMemcpy(out, b_.p + size_t(k) * size_, size_ * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void DavidsonLiu::dots(const double* x, const double* Y, int m, double* out) {
+    if (m <= 0) return;
+    int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    double* part = part_.p;
+    double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
+    k_dots<<<nb, DL_BS>>>(x, Y, m, size_, part);
+    k_reduce_parts<<<m, 256>>>(part, nb, m, res);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, res, m * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+void DavidsonLiu::lincomb(const double* X, int m, const double* coef, int k, double* out) {
+    // coef: k outputs x m inputs (coef[kk*m + j]); split outputs in groups of 8
+    int nb = int(std::min<int64_t>(4 * DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    for (int k0 = 0; k0 < k; k0 += 8) {
+        int kk = std::min(8, k - k0);
+        small_.upload(coef + size_t(k0) * m, size_t(kk) * m);  // synchronous copy
+        k_lincomb<<<nb, DL_BS>>>(X, m, small_.p, kk, size_, out + size_t(k0) * size_);
+        HIPCHK(hipGetLastError());
+    }
+}
+
+// out_k = sum_j alpha[j][k] X_j for k < kout over the current basis (temp_->gemm(true,false,...))
+void DavidsonLiu::rotate(const double* X, double* out, int kout) {
+    int m = subspace_size_;
+    std::vector<double> coef(size_t(kout) * basis_size_);
+    for (int k = 0; k < kout; ++k)
+        for (int j = 0; j < basis_size_; ++j) coef[size_t(k) * basis_size_ + j] = alpha_[size_t(j) * m + k];
+    lincomb(X, basis_size_, coef.data(), kout, out);
+}
+
+// add_row_and_orthonormalize (davidson_liu_solver.cc:648-687): returns norm check pass via ok.
+// Gram-Schmidt against the rows of A is applied to all rows at once per cycle (classical GS);
+// the cycle structure and thresholds are the reference's.
+double DavidsonLiu::add_row_and_orthonormalize(double* A, int rowsA, const double* vsrc, bool* ok) {
+    double* v = A + size_t(rowsA) * size_;
+    if (v != vsrc) HIPCHK(hipMemcpy(v, vsrc, size_ * sizeof(double), hipMemcpyDeviceToDevice));
+    int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    std::vector<double> d(std::max(rowsA, 1)), parts(nb);
+    *ok = false;
+    for (int cycle = 0; cycle < 10; ++cycle) {
+        if (rowsA > 0) dots(v, A, rowsA, d.data());
+        small_.upload(d.data(), std::max(rowsA, 1));
+        k_gs_update<<<nb, DL_BS>>>(v, A, rowsA, small_.p, size_, part_.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(parts.data(), part_.p, nb * sizeof(double), hipMemcpyDeviceToHost));
+        double s2 = 0.0;
+        for (double x : parts) s2 += x;
+        double normval = std::sqrt(s2);
+        if (normval < discard_) return 0.0;
+        k_scale<<<nblk(size_, 256), 256>>>(v, size_, 1. / normval);
+        HIPCHK(hipGetLastError());
+        std::vector<double> ov(rowsA + 1);
+        // overlaps with the previous rows and the norm (A_rowsA is v itself)
+        dots(v, A, rowsA + 1, ov.data());
+        double mo = 0.0;
+        for (int i = 0; i < rowsA; ++i) mo = std::max(mo, std::fabs(ov[i]));
+        double nn = ov[rowsA];
+        if (mo < orth_ && std::fabs(nn - 1.0) < orth_) {
+            *ok = true;
+            // cache the overlaps for check_orthonormality
+            int m = subspace_size_;
+            if (A == b_.p) {
+                for (int i = 0; i < rowsA; ++i) S_[size_t(i) * m + rowsA] = S_[size_t(rowsA) * m + i] = ov[i];
+                S_[size_t(rowsA) * m + rowsA] = nn;
+            }
+            return 1.0;
+        }
+    }
+    return 0.0;
+}
+
+int DavidsonLiu::add_rows_and_orthonormalize(double* A, int rowsA, const double* B, int rowsB) {
+    if (rowsA + rowsB > subspace_size_)
+        throw Error(-5, "DavidsonLiuSolver: rowsA + rowsB (" + std::to_string(rowsA + rowsB) +
+                            ") must be less or equal to matrix size (" + std::to_string(subspace_size_) + ")");
+    int added = 0;
+    for (int j = 0; j < rowsB; ++j) {
+        bool ok = false;
+        add_row_and_orthonormalize(A, rowsA + added, B + size_t(j) * size_, &ok);
+        if (ok) ++added;
+    }
+    return added;
+}
+
+void DavidsonLiu::project_out(double* v, int nvec) {
+    int nbad = int(poff_.size()) - 1;
+    if (nbad <= 0) return;
+    DBuf<int64_t> off(poff_.size());
+    off.upload(poff_.data(), poff_.size());
+    for (int k = 0; k < nvec; ++k) k_project_out<<<1, 256>>>(v + size_t(k) * size_, off.p, nbad, pidx_.p, pval_.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+}
+
+void DavidsonLiu::fill_random(double* v, int nvec) {
+    for (int k = 0; k < nvec; ++k) {
+        rng_ += 0x9E3779B97F4A7C15ULL;
+        k_random<<<nblk(size_, 256), 256>>>(v + size_t(k) * size_, size_, rng_);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void DavidsonLiu::form_G() {
+    // G = b sigma^T (davidson_liu_solver.cc:385-386). Entries between vectors that did not
+    // change since the last call are kept; new columns and rows are computed.
+    int m = subspace_size_;
+    std::vector<double> tmp(m);
+    int gv = std::min(gvalid_, basis_size_);
+    for (int j = gv; j < basis_size_; ++j) {
+        dots(sigma_v_.p + size_t(j) * size_, b_.p, basis_size_, tmp.data());
+        for (int i = 0; i < basis_size_; ++i) G_[size_t(i) * m + j] = tmp[i];
+    }
+    for (int i = gv; i < basis_size_; ++i) {
+        if (gv == 0) break;
+        dots(b_.p + size_t(i) * size_, sigma_v_.p, gv, tmp.data());
+        for (int j = 0; j < gv; ++j) G_[size_t(i) * m + j] = tmp[j];
+    }
+    gvalid_ = basis_size_;
+}
+
+void DavidsonLiu::check_orthonormality() {
+    // davidson_liu_solver.cc:689-730 (overlaps cached from the Gram-Schmidt passes)
+    int m = subspace_size_;
+    double thr = orth_ * 3.0, md = 0.0, mo = 0.0;
+    for (int i = 0; i < basis_size_; ++i)
+        for (int j = 0; j < basis_size_; ++j) {
+            double s = S_[size_t(i) * m + j];
+            if (i == j)
+                md = std::max(md, std::fabs(s - 1.0));
+            else
+                mo = std::max(mo, std::fabs(s));
+        }
+    if (md > thr) throw Error(-5, "DavidsonLiuSolver::check_orthogonality(): eigenvectors are not normalized!");
+    if (mo > thr) throw Error(-5, "DavidsonLiuSolver::check_orthogonality(): eigenvectors are not orthogonal!");
+}
+
+void DavidsonLiu::get_results() {
+    // davidson_liu_solver.cc:488-501
+    lambda_old_ = lambda_;
+    rotate(b_.p, temp_.p, nroot_);
+    HIPCHK(hipMemset(b_.p, 0, b_.n * sizeof(double)));
+    std::fill(S_.begin(), S_.end(), 0.0);
+    basis_size_ = 0;
+    int added = add_rows_and_orthonormalize(b_.p, 0, temp_.p, nroot_);
+    if (added != nroot_)
+        throw Error(-5, "DavidsonLiuSolver: get_results generated less vectors (" + std::to_string(added) +
+                            ") than expected (" + std::to_string(nroot_) + ")");
+    basis_size_ = nroot_;
+    sigma_size_ = 0;  // sigma rows no longer match the rotated basis
+    gvalid_ = 0;
+}
+
+bool DavidsonLiu::solve() {
+    const int m = subspace_size_;
+    iters_ = 0;
+    nsigma_ = 0;
+    // setup_guesses, davidson_liu_solver.cc:276-330
+    if (basis_size_ == 0) {
+        int ng = int(guesses_.size());
+        if (ng >= nroot_ && ng <= m) {
+            HIPCHK(hipMemset(temp_.p, 0, temp_.n * sizeof(double)));
+            for (int g = 0; g < ng; ++g) {
+                std::vector<int64_t> idx;
+                std::vector<double> val;
+                for (const auto& [i, c] : guesses_[g]) {
+                    if (i < 0 || i >= size_) throw Error(-1, "DavidsonLiuSolver: guess index out of range");
+                    idx.push_back(i);
+                    val.push_back(c);
+                }
+                // set_vector: M->set(k, I, CI) (later duplicates overwrite)
+                std::vector<double> host_row;
+                for (size_t t = 0; t < idx.size(); ++t)
+                    HIPCHK(hipMemcpy(temp_.p + size_t(g) * size_ + idx[t], &val[t], sizeof(double), hipMemcpyHostToDevice));
+            }
+        } else if (ng == 0) {
+            HIPCHK(hipMemset(temp_.p, 0, temp_.n * sizeof(double)));
+            fill_random(temp_.p, nroot_);
+        } else {
+            throw Error(-1, "DavidsonLiuSolver: number of guess vectors (" + std::to_string(ng) +
+                                ") must be between 0 and the subspace size (" + std::to_string(m) + ")");
+        }
+        int should = std::max(nroot_, ng);
+        project_out(temp_.p, nroot_);
+        std::fill(S_.begin(), S_.end(), 0.0);
+        int added = add_rows_and_orthonormalize(b_.p, 0, temp_.p, should);
+        if (added != should) throw Error(-5, "DavidsonLiuSolver: guess vectors are zero or linearly dependent");
+        basis_size_ = added;
+        sigma_size_ = 0;
+    } else if (basis_size_ >= nroot_) {
+        basis_size_ = nroot_;
+        sigma_size_ = 0;
+    } else {
+        throw Error(-1, "DavidsonLiuSolver: number of guess vectors must be between 0 and the subspace size");
+    }
+    // preiteration_sanity_checks, :332-345
+    if (!sigma_) throw Error(-1, "DavidsonLiuSolver: sigma builder has not been set");
+    if (!have_hdiag_) throw Error(-1, "DavidsonLiuSolver: h_diag has not been set");
+
+    std::fill(lambda_old_.begin(), lambda_old_.end(), 0.0);
+    gvalid_ = 0;
+    const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    std::vector<double> parts(nb);
+    for (int iter = 0; iter < maxiter_; ++iter) {
+        iters_ = iter + 1;
+        check_orthonormality();
+        // compute_sigma, :374-383
+        for (int j = sigma_size_; j < basis_size_; ++j) {
+            sigma_(b_.p + size_t(j) * size_, sigma_v_.p + size_t(j) * size_);
+            ++nsigma_;
+        }
+        sigma_size_ = basis_size_;
+        // form_and_diagonalize_effective_hamiltonian, :385-410
+        form_G();
+        int bs = basis_size_;
+        std::vector<double> Gb(size_t(bs) * bs), ev, V;
+        for (int i = 0; i < bs; ++i)
+            for (int j = 0; j < bs; ++j) Gb[size_t(i) * bs + j] = 0.5 * (G_[size_t(i) * m + j] + G_[size_t(j) * m + i]);
+        jacobi_eigh(bs, Gb, ev, V);
+        std::fill(alpha_.begin(), alpha_.end(), 0.0);
+        std::fill(lambda_.begin(), lambda_.end(), 0.0);
+        for (int k = 0; k < bs; ++k) {
+            lambda_[k] = ev[k];
+            for (int i = 0; i < bs; ++i) alpha_[size_t(i) * m + k] = V[size_t(k) * bs + i];
+        }
+        // form_residual_vectors + compute_residual_norm + form_correction_vectors, :412-455
+        DBuf<double> ak(bs);
+        for (int k = 0; k < nroot_; ++k) {
+            std::vector<double> a(bs);
+            for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];
+            ak.upload(a.data(), bs);
+            k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak.p, lambda_[k], hdiag_.p, precond_ ? 0 : 1, size_,
+                                      r_.p + size_t(k) * size_, part_.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpy(parts.data(), part_.p, nb * sizeof(double), hipMemcpyDeviceToHost));
+            double s2 = 0.0;
+            for (double x : parts) s2 += x;
+            resid_[k] = std::sqrt(s2);
+        }
+        if (precond_) {
+            if (!precond_(r_.p, lambda_.data(), nroot_, iter)) {
+                // the custom preconditioner declined: apply the diagonal one to the residuals
+                for (int k = 0; k < nroot_; ++k) {
+                    std::vector<double> a(bs);
+                    for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];
+                    ak.upload(a.data(), bs);
+                    k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak.p, lambda_[k], hdiag_.p, 1, size_,
+                                              r_.p + size_t(k) * size_, part_.p);
+                }
+                HIPCHK(hipGetLastError());
+            }
+        }
+        // project_out_roots(r_), normalize_vectors(r_, nroot_), :457-465
+        project_out(r_.p, nroot_);
+        for (int k = 0; k < nroot_; ++k) {
+            double nr2 = 0.0;
+            dots(r_.p + size_t(k) * size_, r_.p + size_t(k) * size_, 1, &nr2);
+            k_scale<<<nblk(size_, 256), 256>>>(r_.p + size_t(k) * size_, size_, 1.0 / std::sqrt(nr2));
+        }
+        HIPCHK(hipGetLastError());
+        if (print_ >= 2) {
+            double avg = 0.0, mx_de = 0.0, mx_r = 0.0;
+            for (int k = 0; k < nroot_; ++k) {
+                avg += lambda_[k] / nroot_;
+                mx_de = std::max(mx_de, std::fabs(lambda_[k] - lambda_old_[k]));
+                mx_r = std::max(mx_r, resid_[k]);
+            }
+            printf("  %6d  %20.12f  %20.12f  %20.12f %6d\n", iter, avg, mx_de, mx_r, basis_size_);
+            fflush(stdout);
+        }
+        // check_convergence, :467-486
+        int ne = 0, nr = 0;
+        for (int k = 0; k < nroot_; ++k) {
+            ne += std::fabs(lambda_[k] - lambda_old_[k]) < e_conv_;
+            nr += resid_[k] < r_conv_;
+        }
+        bool e_ok = ne == nroot_, r_ok = nr == nroot_;
+        bool stop = observe_ && observe_(iter, resid_, lambda_);
+        if ((e_ok && r_ok) || int64_t(basis_size_) == size_ || stop) {
+            get_results();
+            return true;
+        }
+        // subspace_collapse, :538-588
+        if (basis_size_ + nroot_ > m) {
+            int cs = std::min(std::max(collapse_size_, nroot_), basis_size_);
+            rotate(b_.p, temp_.p, cs);
+            HIPCHK(hipMemset(b_.p, 0, b_.n * sizeof(double)));
+            std::fill(S_.begin(), S_.end(), 0.0);
+            int keep_bs = basis_size_;
+            basis_size_ = 0;
+            int added = add_rows_and_orthonormalize(b_.p, 0, temp_.p, cs);
+            basis_size_ = keep_bs;
+            rotate(sigma_v_.p, temp_.p, cs);
+            HIPCHK(hipMemcpy(sigma_v_.p, temp_.p, size_t(cs) * size_ * sizeof(double), hipMemcpyDeviceToDevice));
+            if (added != cs)
+                throw Error(-5, "DavidsonLiuSolver: collapse_vectors generated less vectors (" + std::to_string(added) +
+                                    ") than expected (" + std::to_string(cs) + ")");
+            basis_size_ = cs;
+            sigma_size_ = cs;
+            gvalid_ = 0;
+        }
+        // add the correction vectors, :221-251
+        int num_to_add = std::min(nroot_, m - basis_size_);
+        int added = add_rows_and_orthonormalize(b_.p, basis_size_, r_.p, num_to_add);
+        basis_size_ += added;
+        int missing = num_to_add - added;
+        if (missing > 0) {
+            HIPCHK(hipMemset(temp_.p, 0, size_t(missing) * size_ * sizeof(double)));
+            fill_random(temp_.p, missing);
+            project_out(temp_.p, std::min(missing, nroot_));
+            int ra = add_rows_and_orthonormalize(b_.p, basis_size_, temp_.p, missing);
+            basis_size_ += ra;
+            added += ra;
+        }
+        if (added == 0) {
+            get_results();
+            return r_ok;
+        }
+        lambda_old_ = lambda_;
+    }
+    get_results();
+    return false;
+}
+
+}  // namespace fci

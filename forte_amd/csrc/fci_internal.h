@@ -1,0 +1,173 @@
+// fci_internal.h -- shared declarations of the gfx950 FCI library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fci {
+
+// ---- errors ---------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int status;
+    Error(int st, const std::string& m) : std::runtime_error(m), status(st) {}
+};
+[[noreturn]] void hip_fail(hipError_t e, const char* what, const char* file, int line);
+#define HIPCHK(x)                                                                                \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) ::fci::hip_fail(e_, #x, __FILE__, __LINE__);                       \
+    } while (0)
+
+// ---- device buffer ---------------------------------------------------------------------------
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count) { HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice)); }
+    void download(T* h, size_t count) const { HIPCHK(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost)); }
+    std::vector<T> to_host() const {
+        std::vector<T> v(n);
+        if (n) download(v.data(), n);
+        return v;
+    }
+    T* get() const { return p; }
+};
+
+// ---- combinatorics ---------------------------------------------------------------------------
+constexpr int kMaxOrb = 22;  // strings fit in 32 bits; addresses in 21 bits
+inline uint64_t binom_host(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * uint64_t(n - k + i) / uint64_t(i);
+    return r;
+}
+// Binomial table C(a,b), a,b in [0, 32], row stride 33; device copy passed to kernels.
+std::vector<uint32_t> binom_table();
+
+// Packed table entries (u32): address in bits 0..20, a small index in bits 21..30, sign bit 31.
+constexpr uint32_t kAddrMask = (1u << 21) - 1u;
+constexpr uint32_t kInvalid = 0x7FFFFFFFu;
+
+// ---- one spin: strings and the device tables the kernels read --------------------------------
+struct SpinTables {
+    int n = 0, k = 0;
+    int64_t nstr = 0;
+    DBuf<uint32_t> str;   // [nstr] occupation bits in Forte address order
+    // singles into J: ns = k*(n-k) entries per J, J = sign * a+_p a_q I (p in J, q not in J);
+    // packed (I | P<<21 | sign<<31), P = packed unordered pair index max(p,q)*(max+1)/2+min.
+    int ns = 0;
+    DBuf<uint32_t> into;  // [nstr * ns]
+    // out of I along packed pair P (np = n(n+1)/2 entries per I): target J and sign, kInvalid
+    // when the excitation does not exist. P = (r,r): J = I when r occupied.
+    int np = 0;
+    DBuf<uint32_t> out;   // [np * nstr], pair-major
+    // pair cliques for the same-spin operator: hole strings K with k-d electrons (d = 2, or 1
+    // when k == 1); members = K plus d electrons in vir(K); packed (I | m<<21 | sign<<31) where m
+    // is the pair index a(a-1)/2+b (d = 2) or the orbital (d = 1).
+    int d = 0;
+    int64_t nK = 0;
+    int M = 0;
+    DBuf<uint32_t> clq;   // [nK * M]
+};
+
+void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* d_binom, hipStream_t st);
+
+// Forte-order lists built on the device (parity checks): returns count, fills host arrays.
+int64_t forte_vo_list(const SpinTables& t, const uint32_t* d_binom, int p, int q,
+                      std::vector<double>* sign, std::vector<uint32_t>* I, std::vector<uint32_t>* J);
+int64_t forte_oo_list(const SpinTables& t, const uint32_t* d_binom, int p, int q,
+                      std::vector<double>* sign, std::vector<uint32_t>* I, std::vector<uint32_t>* J);
+int64_t forte_vvoo_list(const SpinTables& t, const uint32_t* d_binom, int p, int q, int r, int s,
+                        std::vector<double>* sign, std::vector<uint32_t>* I, std::vector<uint32_t>* J);
+
+// H diagonal (form_Hdiag_det), identical summation order to ActiveSpaceIntegrals::energy.
+void hdiag_device(const SpinTables& a, const SpinTables& b, const double* d_oei_a,
+                  const double* d_oei_b, const double* d_tei_aa, const double* d_tei_ab,
+                  const double* d_tei_bb, double e_core, double* d_out, hipStream_t st);
+
+// k smallest (value, index) pairs of a device array, ties by index; result sorted ascending.
+void smallest_k(const double* d_v, int64_t n, int k, std::vector<double>& vals,
+                std::vector<int64_t>& idx, hipStream_t st);
+
+// ---- integrals on the device ------------------------------------------------------------------
+struct DevInts {
+    int n = 0;
+    double e_core = 0.0;
+    DBuf<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;  // Forte layout, FP64 (Hdiag, RDM checks)
+    // packed chemist ERIs V[P][Q] = (pq|rs), P,Q packed unordered pairs (np x np)
+    DBuf<double> vpk;
+    DBuf<float> vpk32;
+    // same-spin clique operator G'[m][m'] over members (pairs or orbitals), per spin
+    DBuf<double> g2[2];
+    DBuf<float> g2f[2];
+    int gdim[2] = {0, 0};
+};
+
+// ---- sigma engine -----------------------------------------------------------------------------
+enum class Prec { FP64 = 0, MIXED = 1 };
+
+struct SigmaWork;  // scratch buffers
+
+class SigmaEngine {
+  public:
+    SigmaEngine(const SpinTables* a, const SpinTables* b, const DevInts* ints);
+    ~SigmaEngine();
+    // sigma = H C on device buffers, FP64 or FP32 storage
+    void apply(const double* C, double* S, hipStream_t st);
+    void apply(const float* C, float* S, hipStream_t st);
+    void timings(double* ms3) const;
+    void set_timing(bool on) { timing_ = on; }
+
+  private:
+    template <class T> void apply_t(const T* C, T* S, hipStream_t st);
+    const SpinTables* a_;
+    const SpinTables* b_;
+    const DevInts* ints_;
+    std::unique_ptr<SigmaWork> w_;
+    bool timing_ = false;
+    double ms_[3] = {0, 0, 0};
+};
+
+// transposes: out[c*R + r] = in[r*Cc + c] (+ optional accumulate of beta * base)
+template <class T>
+void transpose_device(const T* in, T* out, int64_t rows, int64_t cols, hipStream_t st);
+template <class T>
+void transpose_add_device(const T* in, T* out, int64_t rows, int64_t cols, const T* base,
+                          T base_scale, hipStream_t st);
+
+// ---- RDMs ---------------------------------------------------------------------------------------
+void rdm1_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,
+                 double* g1a, double* g1b, hipStream_t st);
+void rdm2_same_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,
+                      double* g2aa, double* g2bb, hipStream_t st);
+void rdm2_ab_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,
+                    double* g2ab, hipStream_t st);
+double spin2_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,
+                    hipStream_t st);
+
+}  // namespace fci

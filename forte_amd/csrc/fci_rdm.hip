@@ -1,0 +1,363 @@
+// fci_rdm.hip -- 1-/2-RDMs and <S^2> on gfx950 (FCIVector::compute_rdms, fci_vector_rdm.cc:43-393;
+// FCIVector::compute_spin2, fci_vector_spin2.cc:38-75). L and R are N_a x N_b FP64 vectors.
+//
+// g1[p][q]       = <L| a+p a_q |R>                     one spin, rows of L/R (beta: transposes)
+// g2ss[p][q][r][s] = <L| a+p a+q a_s a_r |R>            same spin, through the pair cliques:
+//                   sum_K <L|a+p a+q|K><K|a_s a_r|R>, a dense M x M product per hole string K
+// g2ab[p][r][q][s] = <L| E^a_pq E^b_rs |R>              per alpha row, singles x beta excitations
+// <S^2> = -sum_rs <E^a_sr E^b_rs> + (na-nb)^2/4 + (na+nb)/2
+#include "fci_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fci {
+
+static inline unsigned nblk(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+__device__ __forceinline__ void unpack_pair(int P, int& hi, int& lo) {
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= P) ++r;
+    hi = r;
+    lo = P - r * (r + 1) / 2;
+}
+
+constexpr int RD_BS = 256;
+constexpr int RD_MAXE = 192;
+
+// 1-RDM of one spin over rows: block per row J. Partial sums of each single excitation into J
+// and of the diagonal, reduced over the block, then added to g[p][q].
+__global__ __launch_bounds__(RD_BS) void k_rdm1_rows(const double* __restrict__ L, const double* __restrict__ R,
+                                                     int64_t nrow, int64_t ncol, const uint32_t* __restrict__ into,
+                                                     int ns, const uint32_t* __restrict__ str, int n,
+                                                     double* __restrict__ g) {
+    __shared__ uint32_t ent[RD_MAXE];
+    __shared__ double red[RD_MAXE + 1][RD_BS / 64];
+    const int64_t J = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < ns; i += RD_BS) ent[i] = into[J * ns + i];
+    __syncthreads();
+    const double* lrow = L + J * ncol;
+    for (int e0 = 0; e0 <= ns; e0 += 32) {
+        double acc[32];
+#pragma unroll
+        for (int x = 0; x < 32; ++x) acc[x] = 0.0;
+        for (int64_t c = tid; c < ncol; c += RD_BS) {
+            double lv = lrow[c];
+#pragma unroll
+            for (int x = 0; x < 32; ++x) {
+                int e = e0 + x;
+                if (e < ns) acc[x] += lv * R[int64_t(ent[e] & kAddrMask) * ncol + c];
+                else if (e == ns) acc[x] += lv * R[J * ncol + c];
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 32; ++x) {
+            double v = acc[x];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0 && e0 + x <= ns) red[e0 + x][wave] = v;
+        }
+    }
+    __syncthreads();
+    const uint32_t occ = str[J];
+    for (int e = tid; e <= ns; e += RD_BS) {
+        double s = 0.0;
+        for (int w = 0; w < RD_BS / 64; ++w) s += red[e][w];
+        if (e < ns) {
+            uint32_t t = ent[e];
+            int hi, lo;
+            unpack_pair(int((t >> 21) & 0x3FFu), hi, lo);
+            int p = ((occ >> hi) & 1u) ? hi : lo;  // created orbital is occupied in J
+            int q = p == hi ? lo : hi;
+            atomicAdd(&g[p * n + q], (t >> 31) ? -s : s);
+        } else {
+            uint32_t o = occ;
+            while (o) {
+                int p = __ffs(o) - 1;
+                o &= o - 1u;
+                atomicAdd(&g[p * n + p], s);
+            }
+        }
+    }
+}
+
+// same-spin 2-RDM through pair cliques: acc[m_pair][m'_pair] += s_m s_m' sum_c L[I_m,c] R[I_m',c]
+constexpr int RC_TW = 32;
+constexpr int RC_MAXM = 96;
+__global__ __launch_bounds__(RD_BS) void k_rdm2_clique(const double* __restrict__ L, const double* __restrict__ R,
+                                                       int64_t ncol, const uint32_t* __restrict__ clq, int M,
+                                                       double* __restrict__ acc_out, int gdim) {
+    __shared__ uint32_t mem[RC_MAXM];
+    __shared__ double DL_[RC_MAXM][RC_TW + 1], DR_[RC_MAXM][RC_TW + 1];
+    const int64_t K = blockIdx.x;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < M; i += RD_BS) mem[i] = clq[K * M + i];
+    __syncthreads();
+    constexpr int PER = (RC_MAXM * RC_MAXM + RD_BS - 1) / RD_BS;
+    double acc[PER];
+#pragma unroll
+    for (int x = 0; x < PER; ++x) acc[x] = 0.0;
+    for (int64_t c0 = 0; c0 < ncol; c0 += RC_TW) {
+        for (int i = tid; i < M * RC_TW; i += RD_BS) {
+            int m = i / RC_TW, c = i % RC_TW;
+            double l = 0.0, r = 0.0;
+            if (c0 + c < ncol) {
+                int64_t row = int64_t(mem[m] & kAddrMask);
+                l = L[row * ncol + c0 + c];
+                r = R[row * ncol + c0 + c];
+                if (mem[m] >> 31) {
+                    l = -l;
+                    r = -r;
+                }
+            }
+            DL_[m][c] = l;
+            DR_[m][c] = r;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < PER; ++x) {
+            int idx = tid + x * RD_BS;
+            if (idx < M * M) {
+                int m = idx / M, mp = idx % M;
+                double s = 0.0;
+                for (int c = 0; c < RC_TW; ++c) s += DL_[m][c] * DR_[mp][c];
+                acc[x] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < PER; ++x) {
+        int idx = tid + x * RD_BS;
+        if (idx < M * M) {
+            int m = idx / M, mp = idx % M;
+            int pm = int((mem[m] >> 21) & 0x3FFu), pmp = int((mem[mp] >> 21) & 0x3FFu);
+            atomicAdd(&acc_out[size_t(pm) * gdim + pmp], acc[x]);
+        }
+    }
+}
+
+// alpha-beta 2-RDM: block per (Ja, chunk of 16 packed beta pairs).
+constexpr int RA_PC = 16;
+constexpr int RA_TW = 32;
+__global__ __launch_bounds__(RD_BS) void k_rdm2_ab(const double* __restrict__ L, const double* __restrict__ R,
+                                                   int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a,
+                                                   int ns, const uint32_t* __restrict__ str_a,
+                                                   const uint32_t* __restrict__ str_b,
+                                                   const uint32_t* __restrict__ out_b, int np, int n,
+                                                   double* __restrict__ g2ab) {
+    extern __shared__ double sm[];
+    __shared__ uint32_t ent[RD_MAXE];
+    __shared__ int ep[RD_MAXE], eq[RD_MAXE];
+    const int npc = (np + RA_PC - 1) / RA_PC;
+    const int64_t Ja = blockIdx.x / npc;
+    const int P0 = int(blockIdx.x % npc) * RA_PC;
+    const int tid = threadIdx.x;
+    const uint32_t occ = str_a[Ja];
+    const int ka = __popc(occ);
+    const int EF = ns + ka;  // singles, then one diagonal entry per occupied orbital
+    double* X = sm;                    // [EF][RA_TW]
+    double* Z = sm + EF * RA_TW;       // [2*RA_PC][RA_TW]
+    for (int i = tid; i < EF; i += RD_BS) {
+        if (i < ns) {
+            uint32_t t = into_a[Ja * ns + i];
+            ent[i] = t;
+            int hi, lo;
+            unpack_pair(int((t >> 21) & 0x3FFu), hi, lo);
+            int p = ((occ >> hi) & 1u) ? hi : lo;
+            ep[i] = p;
+            eq[i] = p == hi ? lo : hi;
+        } else {
+            int j = i - ns, cnt = 0, q = 0;
+            for (q = 0; q < n; ++q)
+                if ((occ >> q) & 1u) {
+                    if (cnt == j) break;
+                    ++cnt;
+                }
+            ent[i] = uint32_t(Ja);
+            ep[i] = q;
+            eq[i] = q;
+        }
+    }
+    __syncthreads();
+    const int NO = EF * 2 * RA_PC;
+    constexpr int PER = (RD_MAXE * 2 * RA_PC + RD_BS - 1) / RD_BS;
+    double acc[PER];
+#pragma unroll
+    for (int x = 0; x < PER; ++x) acc[x] = 0.0;
+    for (int64_t c0 = 0; c0 < Nb; c0 += RA_TW) {
+        for (int i = tid; i < EF * RA_TW; i += RD_BS) {
+            int e = i / RA_TW, c = i % RA_TW;
+            double v = 0.0;
+            if (c0 + c < Nb) {
+                uint32_t t = ent[e];
+                v = R[int64_t(t & kAddrMask) * Nb + c0 + c];
+                if (e < ns && (t >> 31)) v = -v;
+            }
+            X[e * RA_TW + c] = v;
+        }
+        for (int i = tid; i < 2 * RA_PC * RA_TW; i += RD_BS) {
+            int zr = i / RA_TW, c = i % RA_TW;
+            int P = P0 + zr / 2, dir = zr % 2;
+            double v = 0.0;
+            int64_t Ib = c0 + c;
+            if (Ib < Nb && P < np) {
+                uint32_t t = out_b[int64_t(P) * Nb + Ib];
+                if (t != kInvalid) {
+                    int hi, lo;
+                    unpack_pair(P, hi, lo);
+                    // valid direction annihilates the occupied orbital of the pair
+                    int d = (hi == lo) ? 0 : (((str_b[Ib] >> lo) & 1u) ? 0 : 1);
+                    if (d == dir) {
+                        v = L[Ja * Nb + (t & kAddrMask)];
+                        if (t >> 31) v = -v;
+                    }
+                }
+            }
+            Z[zr * RA_TW + c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < PER; ++x) {
+            int idx = tid + x * RD_BS;
+            if (idx < NO) {
+                int e = idx / (2 * RA_PC), zr = idx % (2 * RA_PC);
+                double s = 0.0;
+                for (int c = 0; c < RA_TW; ++c) s += X[e * RA_TW + c] * Z[zr * RA_TW + c];
+                acc[x] += s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < PER; ++x) {
+        int idx = tid + x * RD_BS;
+        if (idx < NO) {
+            int e = idx / (2 * RA_PC), zr = idx % (2 * RA_PC);
+            int P = P0 + zr / 2, dir = zr % 2;
+            if (P >= np) continue;
+            int hi, lo;
+            unpack_pair(P, hi, lo);
+            if (hi == lo && dir == 1) continue;
+            int r = dir == 0 ? hi : lo, s = dir == 0 ? lo : hi;  // create r, annihilate s
+            int p = ep[e], q = eq[e];
+            atomicAdd(&g2ab[((size_t(p) * n + r) * n + q) * n + s], acc[x]);
+        }
+    }
+}
+
+// <S^2> partial sums, block per alpha row Ja
+__global__ __launch_bounds__(RD_BS) void k_spin2(const double* __restrict__ L, const double* __restrict__ R,
+                                                 int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
+                                                 const uint32_t* __restrict__ str_a, const uint32_t* __restrict__ str_b,
+                                                 const uint32_t* __restrict__ out_b, double* __restrict__ part) {
+    __shared__ uint32_t ent[RD_MAXE];
+    __shared__ int ep[RD_MAXE];
+    __shared__ double red[RD_BS / 64];
+    const int64_t Ja = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t occ = str_a[Ja];
+    for (int i = tid; i < ns; i += RD_BS) {
+        uint32_t t = into_a[Ja * ns + i];
+        ent[i] = t;
+        int hi, lo;
+        unpack_pair(int((t >> 21) & 0x3FFu), hi, lo);
+        ep[i] = ((occ >> hi) & 1u) ? hi : lo;
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t Ib = tid; Ib < Nb; Ib += RD_BS) {
+        const uint32_t ob = str_b[Ib];
+        // diagonal: n^a_r n^b_r
+        double d = L[Ja * Nb + Ib] * R[Ja * Nb + Ib];
+        s += d * double(__popc(occ & ob));
+        for (int e = 0; e < ns; ++e) {
+            uint32_t t = ent[e];
+            int p = ep[e];
+            if (!((ob >> p) & 1u)) continue;  // beta must annihilate p
+            uint32_t o = out_b[int64_t((t >> 21) & 0x3FFu) * Nb + Ib];
+            if (o == kInvalid) continue;
+            double v = L[Ja * Nb + (o & kAddrMask)] * R[int64_t(t & kAddrMask) * Nb + Ib];
+            if ((t >> 31) ^ (o >> 31)) v = -v;
+            s += v;
+        }
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < RD_BS / 64; ++w) t += red[w];
+        part[Ja] = t;
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------------------
+void rdm1_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R, double* g1a,
+                 double* g1b, hipStream_t st) {
+    const int n = a.n;
+    const int64_t Na = a.nstr, Nb = b.nstr;
+    HIPCHK(hipMemsetAsync(g1a, 0, size_t(n) * n * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(g1b, 0, size_t(n) * n * sizeof(double), st));
+    if (a.ns + 1 > RD_MAXE || b.ns + 1 > RD_MAXE) throw Error(-1, "rdm1: too many excitations per string");
+    if (a.k > 0) k_rdm1_rows<<<unsigned(Na), RD_BS, 0, st>>>(L, R, Na, Nb, a.into.p, a.ns, a.str.p, n, g1a);
+    if (b.k > 0) {
+        DBuf<double> Lt(size_t(Na) * Nb), Rt(size_t(Na) * Nb);
+        transpose_device<double>(L, Lt.p, Na, Nb, st);
+        transpose_device<double>(R, Rt.p, Na, Nb, st);
+        k_rdm1_rows<<<unsigned(Nb), RD_BS, 0, st>>>(Lt.p, Rt.p, Nb, Na, b.into.p, b.ns, b.str.p, n, g1b);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void rdm2_same_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R, double* acc_a,
+                      double* acc_b, hipStream_t st) {
+    // acc_a / acc_b: npair x npair accumulators (pair index p(p-1)/2+q for p>q)
+    const int64_t Na = a.nstr, Nb = b.nstr;
+    const int npair = a.n * (a.n - 1) / 2;
+    HIPCHK(hipMemsetAsync(acc_a, 0, size_t(npair) * npair * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(acc_b, 0, size_t(npair) * npair * sizeof(double), st));
+    if (a.d == 2) {
+        if (a.M > RC_MAXM) throw Error(-1, "rdm2: clique too large");
+        k_rdm2_clique<<<unsigned(a.nK), RD_BS, 0, st>>>(L, R, Nb, a.clq.p, a.M, acc_a, npair);
+    }
+    if (b.d == 2) {
+        if (b.M > RC_MAXM) throw Error(-1, "rdm2: clique too large");
+        DBuf<double> Lt(size_t(Na) * Nb), Rt(size_t(Na) * Nb);
+        transpose_device<double>(L, Lt.p, Na, Nb, st);
+        transpose_device<double>(R, Rt.p, Na, Nb, st);
+        k_rdm2_clique<<<unsigned(b.nK), RD_BS, 0, st>>>(Lt.p, Rt.p, Na, b.clq.p, b.M, acc_b, npair);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void rdm2_ab_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R, double* g2ab,
+                    hipStream_t st) {
+    const int n = a.n;
+    HIPCHK(hipMemsetAsync(g2ab, 0, size_t(n) * n * n * n * sizeof(double), st));
+    if (a.k == 0 || b.k == 0) return;
+    if (a.ns + a.k > RD_MAXE) throw Error(-1, "rdm2ab: too many excitations per string");
+    const int npc = (b.np + RA_PC - 1) / RA_PC;
+    size_t sh = size_t(a.ns + a.k + 2 * RA_PC) * RA_TW * sizeof(double);
+    k_rdm2_ab<<<unsigned(a.nstr * npc), RD_BS, sh, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p,
+                                                        b.out.p, b.np, n, g2ab);
+    HIPCHK(hipGetLastError());
+}
+
+double spin2_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R, hipStream_t st) {
+    DBuf<double> part(a.nstr);
+    if (a.ns + 1 > RD_MAXE) throw Error(-1, "spin2: too many excitations per string");
+    k_spin2<<<unsigned(a.nstr), RD_BS, 0, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p, b.out.p,
+                                               part.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h = part.to_host();
+    double s = 0.0;
+    for (double x : h) s += x;
+    double na = a.k, nb = b.k;
+    return -s + 0.25 * (na - nb) * (na - nb) + 0.5 * (na + nb);
+}
+
+}  // namespace fci

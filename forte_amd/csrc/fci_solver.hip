@@ -1,0 +1,1062 @@
+// fci_solver.hip -- FCISolver on gfx950 and the C ABI of include/forte_fci.h.
+//
+// Host orchestration follows forte/fci/fci_solver.cc:177-314: H diagonal, determinant guess
+// (fci_solver_initial_guess.cc:44-101 -> sparse_ci/sparse_initial_guess.cc:71-264), Davidson-Liu
+// with the device sigma builder, <S^2> per root, RDMs on request. The small dense guess problem
+// (a few hundred determinants at most) is solved on the host, as in the reference.
+#include "../../include/forte_fci.h"
+#include "fci_dl.h"
+#include "fci_internal.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_set>
+
+namespace fci {
+
+// ---- host integrals and determinant algebra (the guess) ---------------------------------------
+struct HostInts {
+    int n = 0;
+    std::vector<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;
+    double e_core = 0.0;
+    size_t i4(int p, int q, int r, int s) const { return ((size_t(p) * n + q) * n + r) * n + s; }
+};
+
+static inline int popc32(uint32_t x) { return __builtin_popcount(x); }
+static inline double sign_between(uint32_t x, int m, int n) {
+    if (m > n) std::swap(m, n);
+    if (n - m < 2) return 1.0;
+    uint32_t mask = ((1u << n) - 1u) & ~((2u << m) - 1u);
+    return (popc32(x & mask) & 1) ? -1.0 : 1.0;
+}
+static inline double sign_below(uint32_t x, int n) { return (popc32(x & ((1u << n) - 1u)) & 1) ? -1.0 : 1.0; }
+
+static uint32_t unrank_host(uint64_t a, int n, int k) {
+    uint32_t s = 0;
+    int c = n;
+    for (int t = k; t >= 1; --t) {
+        int cc = c - 1;
+        while (binom_host(cc, t) > a) --cc;
+        a -= binom_host(cc, t);
+        s |= 1u << (n - 1 - cc);
+        c = cc;
+    }
+    return s;
+}
+static int64_t rank_host(uint32_t s, int n) {
+    int64_t r = 0;
+    int t = 0;
+    for (int i = n - 1; i >= 0; --i)
+        if ((s >> i) & 1u) {
+            ++t;
+            r += int64_t(binom_host(n - 1 - i, t));
+        }
+    return r;
+}
+
+// DeterminantImpl::slater_sign_aaaa (sparse_ci/determinant.hpp:390-409)
+static double slater_sign_aaaa(uint32_t w, int i, int j, int a, int b) {
+    if (((i < a) && (j < a) && (i < b) && (j < b)) || !((i < a) || (j < a) || (i < b) || (j < b))) {
+        if ((i < j) ^ (a < b)) return -1.0 * sign_between(w, i, j) * sign_between(w, a, b);
+        return sign_between(w, i, j) * sign_between(w, a, b);
+    }
+    if ((i < j) ^ (a < b)) return -1.0 * sign_between(w, i, b) * sign_between(w, j, a);
+    return sign_between(w, i, a) * sign_between(w, j, b);
+}
+
+// ActiveSpaceIntegrals::slater_rules (integrals/active_space_integrals.cc:247-411)
+static double slater_rules(const HostInts& I, uint32_t la, uint32_t lb, uint32_t ra, uint32_t rb) {
+    const int n = I.n;
+    if (popc32(la) != popc32(ra) || popc32(lb) != popc32(rb)) return 0.0;
+    int nad = popc32(la ^ ra), nbd = popc32(lb ^ rb);
+    if (nad + nbd > 4) return 0.0;
+    nad /= 2;
+    nbd /= 2;
+    double me = 0.0;
+    if (nad == 0 && nbd == 0) {
+        for (int p = 0; p < n; ++p) {
+            bool ap = (la >> p) & 1, bp = (lb >> p) & 1;
+            if (ap) me += I.oei_a[p * n + p];
+            if (bp) me += I.oei_b[p * n + p];
+            for (int q = 0; q < n; ++q) {
+                bool aq = (la >> q) & 1, bq = (lb >> q) & 1;
+                if (ap && aq) me += 0.5 * I.tei_aa[I.i4(p, q, p, q)];
+                if (bp && bq) me += 0.5 * I.tei_bb[I.i4(p, q, p, q)];
+                if (ap && bq) me += I.tei_ab[I.i4(p, q, p, q)];
+            }
+        }
+    }
+    if (nad == 1 && nbd == 0) {
+        int i = 0, j = 0;
+        for (int p = 0; p < n; ++p) {
+            if ((((la ^ ra) >> p) & 1) && ((la >> p) & 1)) i = p;
+            if ((((la ^ ra) >> p) & 1) && ((ra >> p) & 1)) j = p;
+        }
+        double sign = sign_between(la, i, j);
+        me = sign * I.oei_a[i * n + j];
+        for (int p = 0; p < n; ++p) {
+            if (((la >> p) & 1) && ((ra >> p) & 1)) me += sign * I.tei_aa[I.i4(i, p, j, p)];
+            if (((lb >> p) & 1) && ((rb >> p) & 1)) me += sign * I.tei_ab[I.i4(i, p, j, p)];
+        }
+    }
+    if (nad == 0 && nbd == 1) {
+        int i = 0, j = 0;
+        for (int p = 0; p < n; ++p) {
+            if ((((lb ^ rb) >> p) & 1) && ((lb >> p) & 1)) i = p;
+            if ((((lb ^ rb) >> p) & 1) && ((rb >> p) & 1)) j = p;
+        }
+        double sign = sign_between(lb, i, j);
+        me = sign * I.oei_b[i * n + j];
+        for (int p = 0; p < n; ++p) {
+            if (((la >> p) & 1) && ((ra >> p) & 1)) me += sign * I.tei_ab[I.i4(p, i, p, j)];
+            if (((lb >> p) & 1) && ((rb >> p) & 1)) me += sign * I.tei_bb[I.i4(i, p, j, p)];
+        }
+    }
+    if (nad == 2 && nbd == 0) {
+        int i = -1, j = 0, k = -1, l = 0;
+        for (int p = 0; p < n; ++p) {
+            if ((((la ^ ra) >> p) & 1) && ((la >> p) & 1)) {
+                if (i == -1) i = p; else j = p;
+            }
+            if ((((la ^ ra) >> p) & 1) && ((ra >> p) & 1)) {
+                if (k == -1) k = p; else l = p;
+            }
+        }
+        me = slater_sign_aaaa(la, i, j, k, l) * I.tei_aa[I.i4(i, j, k, l)];
+    }
+    if (nad == 0 && nbd == 2) {
+        int i = -1, j = -1, k = -1, l = -1;
+        for (int p = 0; p < n; ++p) {
+            if ((((lb ^ rb) >> p) & 1) && ((lb >> p) & 1)) {
+                if (i == -1) i = p; else j = p;
+            }
+            if ((((lb ^ rb) >> p) & 1) && ((rb >> p) & 1)) {
+                if (k == -1) k = p; else l = p;
+            }
+        }
+        me = slater_sign_aaaa(lb, i, j, k, l) * I.tei_bb[I.i4(i, j, k, l)];
+    }
+    if (nad == 1 && nbd == 1) {
+        int i = -1, j = -1, k = -1, l = -1;
+        for (int p = 0; p < n; ++p) {
+            if ((((la ^ ra) >> p) & 1) && ((la >> p) & 1)) i = p;
+            if ((((lb ^ rb) >> p) & 1) && ((lb >> p) & 1)) j = p;
+            if ((((la ^ ra) >> p) & 1) && ((ra >> p) & 1)) k = p;
+            if ((((lb ^ rb) >> p) & 1) && ((rb >> p) & 1)) l = p;
+        }
+        me = sign_between(la, i, k) * sign_between(lb, j, l) * I.tei_ab[I.i4(i, j, k, l)];
+    }
+    return me;
+}
+
+struct Det {
+    uint32_t a, b;
+    bool operator==(const Det& o) const { return a == o.a && b == o.b; }
+};
+struct DetHash {
+    size_t operator()(const Det& d) const { return (size_t(d.a) << 32) ^ d.b; }
+};
+
+// spin2(lhs, rhs) matrix element, sparse_ci/determinant.hpp:910-955
+static double spin2_el(int n, Det I, Det J) {
+    if (popc32(I.a) != popc32(J.a) || popc32(I.b) != popc32(J.b)) return 0.0;
+    int nad = popc32(I.a ^ J.a) / 2, nbd = popc32(I.b ^ J.b) / 2;
+    int na = popc32(I.a), nb = popc32(I.b), npair = popc32(I.a & I.b);
+    double Ms = 0.5 * double(na - nb), me = 0.0;
+    if (nad == 0 && nbd == 0) me += Ms * (Ms + 1.0) + double(nb) - double(npair);
+    if (nad == 1 && nbd == 1) {
+        int i = -1, j = -1;
+        for (int p = 0; p < n; ++p) {
+            if (((J.a >> p) & 1) && ((I.b >> p) & 1) && !((J.b >> p) & 1) && !((I.a >> p) & 1)) i = p;
+            if (((J.b >> p) & 1) && ((I.a >> p) & 1) && !((J.a >> p) & 1) && !((I.b >> p) & 1)) j = p;
+        }
+        if (i != j && i >= 0 && j >= 0)
+            me -= sign_below(J.a, i) * sign_below(J.b, j) * sign_below(I.a, j) * sign_below(I.b, i);
+    }
+    return me;
+}
+
+// enforce_spin_completeness, sparse_ci/determinant_functions.cc:133-194
+static void spin_complete(std::vector<Det>& dets, int nmo) {
+    std::unordered_set<Det, DetHash> seen(dets.begin(), dets.end());
+    size_t det_size = dets.size();
+    for (size_t I = 0; I < det_size; ++I) {
+        Det det = dets[I];
+        std::vector<int> open, closed;
+        int naopen = 0, nbopen = 0;
+        for (int i = 0; i < nmo; ++i) {
+            bool a = (det.a >> i) & 1, b = (det.b >> i) & 1;
+            if (a && !b) { open.push_back(i); naopen++; }
+            else if (!a && b) { open.push_back(i); nbopen++; }
+            else if (a && b) closed.push_back(i);
+        }
+        if (naopen + nbopen == 0) continue;
+        std::vector<int> bits(naopen + nbopen, 0);
+        for (int i = nbopen; i < naopen + nbopen; ++i) bits[i] = 1;
+        do {
+            Det nd{0, 0};
+            for (int c : closed) { nd.a |= 1u << c; nd.b |= 1u << c; }
+            for (size_t o = 0; o < open.size(); ++o) {
+                if (bits[o]) nd.a |= 1u << open[o]; else nd.b |= 1u << open[o];
+            }
+            if (!seen.count(nd)) {
+                dets.push_back(nd);
+                seen.insert(nd);
+            }
+        } while (std::next_permutation(bits.begin(), bits.end()));
+    }
+}
+
+// find_integer_groups, helpers/helpers.cc:233-262
+struct Group { int m; int start, end; };
+static std::vector<Group> integer_groups(const std::vector<double>& v, double toll) {
+    auto near_int = [toll](double x) { return std::fabs(x - std::round(x)) <= toll; };
+    std::vector<Group> g;
+    int n = int(v.size()), it = 0;
+    while (it != n) {
+        while (it < n && !near_int(v[it])) ++it;
+        if (it != n) {
+            int start = it, j = it;
+            while (j + 1 < n && !(near_int(v[j]) && std::fabs(v[j] - v[j + 1]) > 2 * toll)) ++j;
+            g.push_back({int(std::llround(v[start])), start, j + 1});
+            it = j + 1;
+        }
+    }
+    return g;
+}
+
+// ---- the solver ----------------------------------------------------------------------------
+static std::once_flag g_dev_once;
+static int g_dev_status = 0;
+static std::string g_dev_msg;
+
+static void check_device() {
+    std::call_once(g_dev_once, [] {
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || ndev == 0) {
+            g_dev_status = -3;
+            g_dev_msg = "no HIP device available (the gfx950 FCI library needs an MI355X)";
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            g_dev_status = -3;
+            g_dev_msg = "hipGetDeviceProperties failed";
+            return;
+        }
+        if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+            g_dev_status = -3;
+            g_dev_msg = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950 only";
+        }
+    });
+    if (g_dev_status) throw Error(g_dev_status, g_dev_msg);
+}
+
+class Solver {
+  public:
+    Solver(int n, int na, int nb) : n_(n), na_(na), nb_(nb) {
+        check_device();
+        if (n < 1 || n > kMaxOrb) throw Error(-1, "number of orbitals must be in [1, 22]");
+        if (na < 0 || nb < 0 || na > n || nb > n) throw Error(-1, "invalid number of electrons");
+        auto bt = binom_table();
+        binom_.alloc(bt.size());
+        binom_.upload(bt.data(), bt.size());
+        build_spin_tables(sa_, n, na, binom_.p, 0);
+        build_spin_tables(sb_, n, nb, binom_.p, 0);
+        HIPCHK(hipDeviceSynchronize());
+        ndet_ = sa_.nstr * sb_.nstr;
+        eng_.reset(new SigmaEngine(&sa_, &sb_, &di_));
+        fci_default_options(&opt_);
+    }
+
+    void set_integrals(const double* oa, const double* ob, const double* taa, const double* tab,
+                       const double* tbb, double e_core) {
+        const int n = n_;
+        size_t n2 = size_t(n) * n, n4 = n2 * n2;
+        hi_.n = n;
+        hi_.oei_a.assign(oa, oa + n2);
+        hi_.oei_b.assign(ob, ob + n2);
+        hi_.tei_aa.assign(taa, taa + n4);
+        hi_.tei_ab.assign(tab, tab + n4);
+        hi_.tei_bb.assign(tbb, tbb + n4);
+        hi_.e_core = e_core;
+        di_.n = n;
+        di_.e_core = e_core;
+        di_.oei_a.alloc(n2); di_.oei_a.upload(oa, n2);
+        di_.oei_b.alloc(n2); di_.oei_b.upload(ob, n2);
+        di_.tei_aa.alloc(n4); di_.tei_aa.upload(taa, n4);
+        di_.tei_ab.alloc(n4); di_.tei_ab.upload(tab, n4);
+        di_.tei_bb.alloc(n4); di_.tei_bb.upload(tbb, n4);
+        // packed chemist ERIs (pq|rs) = tei_ab[p,r,q,s]; the packed form needs (pq|rs) = (qp|rs)
+        const int np = n * (n + 1) / 2;
+        std::vector<double> v(size_t(np) * np);
+        std::vector<float> v32(size_t(np) * np);
+        double scale = 0.0;
+        for (size_t i = 0; i < n4; ++i) scale = std::max(scale, std::fabs(tab[i]));
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q <= p; ++q)
+                for (int r = 0; r < n; ++r)
+                    for (int s = 0; s <= r; ++s) {
+                        double x = tab[hi_.i4(p, r, q, s)];
+                        double y = tab[hi_.i4(q, r, p, s)], z = tab[hi_.i4(p, s, q, r)];
+                        if (std::fabs(x - y) > 1e-10 * (1.0 + scale) || std::fabs(x - z) > 1e-10 * (1.0 + scale))
+                            throw Error(-1, "tei_ab must have the permutational symmetry of real orbitals");
+                        size_t P = size_t(p) * (p + 1) / 2 + q, Q = size_t(r) * (r + 1) / 2 + s;
+                        v[P * np + Q] = x;
+                        v32[P * np + Q] = float(x);
+                    }
+        di_.vpk.alloc(v.size()); di_.vpk.upload(v.data(), v.size());
+        di_.vpk32.alloc(v32.size()); di_.vpk32.upload(v32.data(), v32.size());
+        // same-spin clique operator per spin
+        for (int spin = 0; spin < 2; ++spin) {
+            const SpinTables& sp = spin ? sb_ : sa_;
+            const std::vector<double>& h = spin ? hi_.oei_b : hi_.oei_a;
+            const std::vector<double>& t = spin ? hi_.tei_bb : hi_.tei_aa;
+            std::vector<double> g;
+            int gd = 0;
+            if (sp.d == 2) {
+                gd = n * (n - 1) / 2;
+                g.assign(size_t(gd) * gd, 0.0);
+                auto pidx = [](int p, int q) { return p > q ? p * (p - 1) / 2 + q : q * (q - 1) / 2 + p; };
+                // two-body: H2_aaaa2 uses tei(p,q,r,s) with pair(pq) >= pair(rs) for both orders
+                for (int p = 0; p < n; ++p)
+                    for (int q = 0; q < p; ++q)
+                        for (int r = 0; r < n; ++r)
+                            for (int s = 0; s < r; ++s) {
+                                int P = pidx(p, q), R = pidx(r, s);
+                                g[size_t(P) * gd + R] = (P >= R) ? t[hi_.i4(p, q, r, s)] : t[hi_.i4(r, s, p, q)];
+                            }
+                // one-body fold: h_pq a+p a_q = 1/(k-1) sum_t h_pq a+p a+t a_t a_q
+                const double f = 1.0 / double(sp.k - 1);
+                for (int p = 0; p < n; ++p)
+                    for (int q = 0; q < n; ++q)
+                        for (int tt = 0; tt < n; ++tt) {
+                            if (tt == p || tt == q) continue;
+                            double sg = (p > tt ? 1.0 : -1.0) * (q > tt ? 1.0 : -1.0);
+                            g[size_t(pidx(p, tt)) * gd + pidx(q, tt)] += f * sg * h[p * n + q];
+                        }
+            } else if (sp.d == 1) {
+                gd = n;
+                g.assign(h.begin(), h.end());
+            }
+            di_.gdim[spin] = gd;
+            if (gd) {
+                std::vector<float> g32(g.begin(), g.end());
+                di_.g2[spin].alloc(g.size());
+                di_.g2[spin].upload(g.data(), g.size());
+                di_.g2f[spin].alloc(g32.size());
+                di_.g2f[spin].upload(g32.data(), g32.size());
+            }
+        }
+        have_ints_ = true;
+    }
+
+    fci_options opt_;
+    int n_, na_, nb_;
+    int64_t ndet_ = 0;
+    DBuf<uint32_t> binom_;
+    SpinTables sa_, sb_;
+    DevInts di_;
+    HostInts hi_;
+    bool have_ints_ = false;
+    std::unique_ptr<SigmaEngine> eng_;
+    std::unique_ptr<DavidsonLiu> dl_;
+    DBuf<double> hdiag_;
+    DBuf<float> c32_, s32_;
+    std::vector<double> energies_, spin2_;
+    bool solved_ = false;
+    int stat_iter_ = 0, stat_nsigma_ = 0, stat_nsigma64_ = 0, stat_conv_ = 0, stat_fellback_ = 0;
+    double stat_sec_ = 0.0;
+    std::string err_;
+
+    void sigma64(const double* C, double* S) { eng_->apply(C, S, 0); }
+    void sigma_mixed(const double* C, double* S);
+    void compute_hdiag() {
+        if (!hdiag_.p) hdiag_.alloc(ndet_);
+        hdiag_device(sa_, sb_, di_.oei_a.p, di_.oei_b.p, di_.tei_aa.p, di_.tei_ab.p, di_.tei_bb.p, di_.e_core,
+                     hdiag_.p, 0);
+        HIPCHK(hipDeviceSynchronize());
+    }
+    std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess(int num_guess_states);
+    double compute_energy();
+};
+
+__global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = float(x[i]);
+}
+__global__ void k_f32_to_f64(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = double(x[i]);
+}
+
+void Solver::sigma_mixed(const double* C, double* S) {
+    if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
+    if (s32_.n < size_t(ndet_)) s32_.alloc(ndet_);
+    unsigned g = unsigned((ndet_ + 255) / 256);
+    k_f64_to_f32<<<g, 256>>>(C, c32_.p, ndet_);
+    eng_->apply(c32_.p, s32_.p, 0);
+    k_f32_to_f64<<<g, 256>>>(s32_.p, S, ndet_);
+    HIPCHK(hipGetLastError());
+}
+
+// FCISolver::initial_guess_det (fci_solver_initial_guess.cc:88-101) and find_initial_guess_det
+std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(int num_guess_states) {
+    const int64_t nbs = sb_.nstr;
+    int64_t ngd = std::min<int64_t>(int64_t(num_guess_states) * opt_.ndets_per_guess, ndet_);
+    std::vector<double> vals;
+    std::vector<int64_t> idx;
+    smallest_k(hdiag_.p, ndet_, int(ngd), vals, idx, 0);
+    std::vector<Det> dets;
+    for (int64_t I : idx) dets.push_back({unrank_host(uint64_t(I / nbs), n_, na_), unrank_host(uint64_t(I % nbs), n_, nb_)});
+    spin_complete(dets, n_);
+    const int G = int(dets.size());
+    std::vector<int64_t> pos(G);
+    for (int i = 0; i < G; ++i) pos[i] = rank_host(dets[i].a, n_) * nbs + rank_host(dets[i].b, n_);
+    std::vector<double> S2(size_t(G) * G), H(size_t(G) * G);
+    for (int I = 0; I < G; ++I)
+        for (int J = I; J < G; ++J) {
+            S2[size_t(I) * G + J] = S2[size_t(J) * G + I] = spin2_el(n_, dets[I], dets[J]);
+            H[size_t(I) * G + J] = H[size_t(J) * G + I] = slater_rules(hi_, dets[I].a, dets[I].b, dets[J].a, dets[J].b);
+        }
+    std::vector<double> s2val, U;  // U column-major: eigenvector k at U[k*G..]
+    jacobi_eigh(G, S2, s2val, U);
+    std::vector<double> HU(size_t(G) * G, 0.0), HS2(size_t(G) * G, 0.0);
+    for (int i = 0; i < G; ++i)
+        for (int k = 0; k < G; ++k) {
+            double s = 0.0;
+            for (int j = 0; j < G; ++j) s += H[size_t(i) * G + j] * U[size_t(k) * G + j];
+            HU[size_t(i) * G + k] = s;
+        }
+    for (int a = 0; a < G; ++a)
+        for (int k = 0; k < G; ++k) {
+            double s = 0.0;
+            for (int i = 0; i < G; ++i) s += U[size_t(a) * G + i] * HU[size_t(i) * G + k];
+            HS2[size_t(a) * G + k] = s;
+        }
+    std::vector<double> mult(G);
+    for (int i = 0; i < G; ++i) mult[i] = std::sqrt(1.0 + 4.0 * s2val[i]);
+    auto groups = integer_groups(mult, 1.0e-6);
+    struct Info { std::vector<double> e; std::vector<double> C; int bsz; };
+    std::map<int, Info> info;  // keyed by multiplicity, later groups replace earlier ones
+    for (const auto& gr : groups) {
+        int bsz = gr.end - gr.start;
+        std::vector<double> blk(size_t(bsz) * bsz), bev, bvec;
+        for (int i = 0; i < bsz; ++i)
+            for (int j = 0; j < bsz; ++j) blk[size_t(i) * bsz + j] = HS2[size_t(gr.start + i) * G + gr.start + j];
+        jacobi_eigh(bsz, blk, bev, bvec);
+        Info inf;
+        inf.bsz = bsz;
+        inf.e = bev;
+        for (double& e : inf.e) e += hi_.e_core;
+        inf.C.assign(size_t(G) * bsz, 0.0);
+        for (int I = 0; I < G; ++I)
+            for (int r = 0; r < bsz; ++r) {
+                double s = 0.0;
+                for (int j = 0; j < bsz; ++j) s += U[size_t(gr.start + j) * G + I] * bvec[size_t(r) * bsz + j];
+                inf.C[size_t(I) * bsz + r] = s;
+            }
+        info[gr.m] = std::move(inf);
+    }
+    auto it = info.find(opt_.multiplicity);
+    if (it == info.end()) throw Error(-1, "No guess with the requested multiplicity was found.");
+    const Info& tg = it->second;
+    if (tg.bsz < num_guess_states)
+        throw Error(-1, "Found " + std::to_string(tg.bsz) + " guess(es) with the requested multiplicity but " +
+                            std::to_string(num_guess_states) +
+                            " were requested. Increase the value of DL_DETS_PER_GUESS");
+    std::vector<SparseVec> guesses, bad;
+    double gmax = -1.7976931348623157e308;
+    for (int r = 0; r < num_guess_states; ++r) {
+        SparseVec g(G);
+        for (int I = 0; I < G; ++I) g[I] = {pos[I], tg.C[size_t(I) * tg.bsz + r]};
+        guesses.push_back(g);
+        gmax = std::max(gmax, tg.e[r]);
+    }
+    for (const auto& [mult_, inf] : info) {
+        if (mult_ == opt_.multiplicity) continue;
+        for (int r = 0; r < inf.bsz; ++r)
+            if (inf.e[r] < gmax) {
+                SparseVec g(G);
+                for (int I = 0; I < G; ++I) g[I] = {pos[I], inf.C[size_t(I) * inf.bsz + r]};
+                bad.push_back(g);
+            }
+    }
+    return {guesses, bad};
+}
+
+double Solver::compute_energy() {
+    if (!have_ints_) throw Error(-2, "integrals have not been set");
+    auto t0 = std::chrono::steady_clock::now();
+    const int nroot = opt_.nroot;
+    bool first_run = false;
+    if (!dl_ || dl_->nroot() != nroot) {
+        dl_.reset(new DavidsonLiu(ndet_, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
+        first_run = true;
+    }
+    dl_->set_print_level(opt_.print);
+    const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, ndet_));
+    compute_hdiag();
+    dl_->add_h_diag_device(hdiag_.p, ndet_);
+    bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= ndet_;
+    if (first_run || use_initial_guess) {
+        dl_->reset();
+        auto [g, bad] = initial_guess(num_guess_states);
+        dl_->add_guesses(g);
+        dl_->add_project_out_vectors(bad);
+    }
+    int nsig64 = 0, nsig = 0, iters = 0;
+    bool converged = false;
+    if (opt_.precision == FCI_PREC_MIXED) {
+        // mixed phase: FP32 sigma until the residual reaches mixed_switch or stalls
+        dl_->add_sigma_builder([this](const double* b, double* s) { sigma_mixed(b, s); });
+        dl_->set_e_convergence(std::max(opt_.e_convergence, 1e-7));
+        dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
+        dl_->set_maxiter(opt_.maxiter);
+        // end the mixed phase when the largest residual has not dropped by 20% in 2 iterations
+        auto hist = std::make_shared<std::vector<double>>();
+        dl_->set_iteration_observer([hist](int, const std::vector<double>& r, const std::vector<double>&) {
+            double mx = 0.0;
+            for (double x : r) mx = std::max(mx, x);
+            hist->push_back(mx);
+            size_t k = hist->size();
+            return k >= 3 && (*hist)[k - 1] > 0.8 * (*hist)[k - 3];
+        });
+        dl_->solve();
+        dl_->set_iteration_observer(nullptr);
+        nsig += dl_->sigma_builds();
+        iters += dl_->iterations();
+    }
+    // FP64 (whole solve, or refinement from the mixed-precision vectors)
+    dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
+    dl_->set_e_convergence(opt_.e_convergence);
+    dl_->set_r_convergence(opt_.r_convergence);
+    dl_->set_maxiter(opt_.maxiter);
+    converged = dl_->solve();
+    nsig64 = dl_->sigma_builds();
+    nsig += nsig64;
+    iters += dl_->iterations();
+    stat_iter_ = iters;
+    stat_nsigma_ = nsig;
+    stat_nsigma64_ = nsig64;
+    stat_conv_ = converged;
+    if (!converged && opt_.die_if_not_converged)
+        throw Error(-4,
+                    "Davidson-Liu solver did not converge.\nPlease try to increase the number of Davidson-Liu "
+                    "iterations (DL_MAXITER). You can also try to increase:\n - the maximum size of the subspace "
+                    "(DL_SUBSPACE_PER_ROOT)\n - the number of guess states (DL_GUESS_PER_ROOT)");
+    energies_.assign(nroot, 0.0);
+    spin2_.assign(nroot, 0.0);
+    for (int r = 0; r < nroot; ++r) {
+        energies_[r] = dl_->eigenvalues()[r];
+        const double* v = dl_->eigenvector_device(r);
+        spin2_[r] = spin2_device(sa_, sb_, v, v, 0);
+    }
+    solved_ = true;
+    stat_sec_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (opt_.root < 0 || opt_.root >= nroot) throw Error(-1, "root index out of range");
+    return energies_[opt_.root];
+}
+
+}  // namespace fci
+
+// ===================================================================================================
+// C ABI
+// ===================================================================================================
+using fci::Error;
+
+struct fci_solver {
+    std::unique_ptr<fci::Solver> s;
+    std::string err;
+};
+struct fci_dl {
+    std::unique_ptr<fci::DavidsonLiu> d;
+    fci::DBuf<double> M;
+    std::string err;
+    std::vector<double> evals;
+};
+
+static thread_local std::string g_err;
+
+template <class F> static int guarded(std::string* err, F&& f) {
+    try {
+        f();
+        return FCI_OK;
+    } catch (const Error& e) {
+        if (err) *err = e.what();
+        g_err = e.what();
+        return e.status;
+    } catch (const std::exception& e) {
+        if (err) *err = e.what();
+        g_err = e.what();
+        return FCI_ERR_ARG;
+    }
+}
+
+extern "C" {
+
+void fci_default_options(fci_options* o) {
+    // FCISolver::set_options defaults from forte/options.yaml (DL_* :821-840, E/R_CONVERGENCE)
+    o->nroot = 1;
+    o->root = 0;
+    o->multiplicity = 1;
+    o->guess_per_root = 1;
+    o->ndets_per_guess = 50;
+    o->collapse_per_root = 2;
+    o->subspace_per_root = 10;
+    o->maxiter = 100;
+    o->e_convergence = 1.0e-10;
+    o->r_convergence = 1.0e-6;
+    o->die_if_not_converged = 1;
+    o->print = 0;
+    o->precision = FCI_PREC_FP64;
+    o->preconditioner = FCI_PRECOND_DIAGONAL;
+    o->mixed_switch = 1.0e-4;
+}
+
+int fci_create(int n, int na, int nb, const double* oa, const double* ob, const double* taa, const double* tab,
+               const double* tbb, double e_core, fci_solver** out) {
+    if (!out) return FCI_ERR_ARG;
+    *out = nullptr;
+    auto h = std::make_unique<fci_solver>();
+    int st = guarded(nullptr, [&] {
+        if (!oa || !ob || !taa || !tab || !tbb) throw Error(FCI_ERR_ARG, "null integral pointer");
+        h->s.reset(new fci::Solver(n, na, nb));
+        h->s->set_integrals(oa, ob, taa, tab, tbb, e_core);
+    });
+    if (st == FCI_OK) *out = h.release();
+    return st;
+}
+
+void fci_destroy(fci_solver* s) { delete s; }
+
+const char* fci_last_error(const fci_solver* s) { return s ? s->err.c_str() : g_err.c_str(); }
+
+int fci_set_options(fci_solver* s, const fci_options* o) {
+    if (!s || !o) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        if (o->nroot < 1) throw Error(FCI_ERR_ARG, "nroot must be >= 1");
+        if (o->guess_per_root < 1 || o->ndets_per_guess < 1) throw Error(FCI_ERR_ARG, "invalid guess options");
+        if (o->precision != FCI_PREC_FP64 && o->precision != FCI_PREC_MIXED) throw Error(FCI_ERR_ARG, "invalid precision");
+        if (o->preconditioner != FCI_PRECOND_DIAGONAL && o->preconditioner != FCI_PRECOND_LEARNED)
+            throw Error(FCI_ERR_ARG, "invalid preconditioner");
+        s->s->opt_ = *o;
+    });
+}
+
+int fci_set_integrals(fci_solver* s, const double* oa, const double* ob, const double* taa, const double* tab,
+                      const double* tbb, double e_core) {
+    if (!s) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] { s->s->set_integrals(oa, ob, taa, tab, tbb, e_core); });
+}
+
+int fci_compute_energy(fci_solver* s, double* energy) {
+    if (!s) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        double e = s->s->compute_energy();
+        if (energy) *energy = e;
+    });
+}
+
+int fci_energies(const fci_solver* s, double* energies, double* spin2) {
+    if (!s) return FCI_ERR_ARG;
+    if (!s->s->solved_) return FCI_ERR_STATE;
+    for (size_t r = 0; r < s->s->energies_.size(); ++r) {
+        if (energies) energies[r] = s->s->energies_[r];
+        if (spin2) spin2[r] = s->s->spin2_[r];
+    }
+    return FCI_OK;
+}
+
+int fci_solve_stats(const fci_solver* s, int* it, int* ns, int* ns64, double* sec, int* conv, int* fb) {
+    if (!s) return FCI_ERR_ARG;
+    if (it) *it = s->s->stat_iter_;
+    if (ns) *ns = s->s->stat_nsigma_;
+    if (ns64) *ns64 = s->s->stat_nsigma64_;
+    if (sec) *sec = s->s->stat_sec_;
+    if (conv) *conv = s->s->stat_conv_;
+    if (fb) *fb = s->s->stat_fellback_;
+    return FCI_OK;
+}
+
+int fci_evec(const fci_solver* s, int root, double* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    auto* self = const_cast<fci_solver*>(s);
+    return guarded(&self->err, [&] {
+        if (!s->s->solved_) throw Error(FCI_ERR_STATE, "compute_energy has not been called");
+        if (root < 0 || root >= s->s->dl_->nroot()) throw Error(FCI_ERR_ARG, "root index out of range");
+        s->s->dl_->eigenvector_host(root, out);
+    });
+}
+
+int64_t fci_ndet(const fci_solver* s) { return s ? s->s->ndet_ : -1; }
+int64_t fci_nstr(const fci_solver* s, int spin) { return s ? (spin ? s->s->sb_.nstr : s->s->sa_.nstr) : -1; }
+
+int fci_rdms(fci_solver* s, int rl, int rr, int level, double* g1a, double* g1b, double* g2aa, double* g2ab,
+             double* g2bb) {
+    if (!s) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (!S.solved_) throw Error(FCI_ERR_STATE, "FCISolver::rdms: no wave function; call compute_energy first");
+        if (rl < 0 || rr < 0 || rl >= S.dl_->nroot() || rr >= S.dl_->nroot())
+            throw Error(FCI_ERR_ARG, "FCISolver::rdms: root index out of range");
+        if (level < 1 || level > 2) throw Error(FCI_ERR_ARG, "FCISolver::rdms: only levels 1 and 2 are implemented");
+        const int n = S.n_;
+        const size_t n2 = size_t(n) * n, n4 = n2 * n2;
+        const double* L = S.dl_->eigenvector_device(rl);
+        const double* R = S.dl_->eigenvector_device(rr);
+        fci::DBuf<double> d1a(n2), d1b(n2);
+        fci::rdm1_device(S.sa_, S.sb_, L, R, d1a.p, d1b.p, 0);
+        HIPCHK(hipDeviceSynchronize());
+        if (g1a) d1a.download(g1a, n2);
+        if (g1b) d1b.download(g1b, n2);
+        if (level >= 2) {
+            const int np = n * (n - 1) / 2;
+            fci::DBuf<double> aa(size_t(np > 0 ? np : 1) * (np > 0 ? np : 1)), bb(aa.n), ab(n4);
+            fci::rdm2_same_device(S.sa_, S.sb_, L, R, aa.p, bb.p, 0);
+            fci::rdm2_ab_device(S.sa_, S.sb_, L, R, ab.p, 0);
+            HIPCHK(hipDeviceSynchronize());
+            if (g2ab) ab.download(g2ab, n4);
+            // expand the pair matrices with the reference's fill (fci_vector_rdm.cc:240-285): the
+            // element of the larger pair index is copied to its transpose
+            for (int spin = 0; spin < 2; ++spin) {
+                double* g = spin ? g2bb : g2aa;
+                if (!g) continue;
+                std::memset(g, 0, n4 * sizeof(double));
+                if (np == 0) continue;
+                std::vector<double> P = (spin ? bb : aa).to_host();
+                for (int p = 0; p < n; ++p)
+                    for (int q = 0; q < p; ++q)
+                        for (int r = 0; r < n; ++r)
+                            for (int t = 0; t < r; ++t) {
+                                int A = p * (p - 1) / 2 + q, B = r * (r - 1) / 2 + t;
+                                double el = (A >= B) ? P[size_t(A) * np + B] : P[size_t(B) * np + A];
+                                auto at = [&](int a, int b, int c, int d) -> double& {
+                                    return g[((size_t(a) * n + b) * n + c) * n + d];
+                                };
+                                at(p, q, r, t) = el;
+                                at(q, p, r, t) = -el;
+                                at(p, q, t, r) = -el;
+                                at(q, p, t, r) = el;
+                            }
+            }
+        }
+    });
+}
+
+int fci_spin2_of(fci_solver* s, const double* C, double* spin2) {
+    if (!s || !C || !spin2) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        fci::DBuf<double> d(S.ndet_);
+        d.upload(C, S.ndet_);
+        *spin2 = fci::spin2_device(S.sa_, S.sb_, d.p, d.p, 0);
+    });
+}
+
+int fci_sigma(fci_solver* s, const double* C, double* sigma) {
+    if (!s || !C || !sigma) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        fci::DBuf<double> dc(S.ndet_), ds(S.ndet_);
+        dc.upload(C, S.ndet_);
+        S.sigma64(dc.p, ds.p);
+        HIPCHK(hipDeviceSynchronize());
+        ds.download(sigma, S.ndet_);
+    });
+}
+
+int fci_sigma_device(fci_solver* s, const void* C, void* sigma, int dtype, void* stream) {
+    if (!s || !C || !sigma) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        S.eng_->set_timing(true);
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        if (dtype == 0)
+            S.eng_->apply(static_cast<const double*>(C), static_cast<double*>(sigma), st);
+        else if (dtype == 1)
+            S.eng_->apply(static_cast<const float*>(C), static_cast<float*>(sigma), st);
+        else
+            throw Error(FCI_ERR_ARG, "dtype must be 0 (FP64) or 1 (FP32)");
+        S.eng_->set_timing(false);
+    });
+}
+
+int fci_sigma_timings(const fci_solver* s, double* ms3) {
+    if (!s || !ms3) return FCI_ERR_ARG;
+    s->s->eng_->timings(ms3);
+    return FCI_OK;
+}
+
+int fci_hdiag(fci_solver* s, double* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        s->s->compute_hdiag();
+        s->s->hdiag_.download(out, s->s->ndet_);
+    });
+}
+
+int fci_strings(fci_solver* s, int spin, uint64_t* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        const fci::SpinTables& t = spin ? s->s->sb_ : s->s->sa_;
+        auto h = t.str.to_host();
+        for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+    });
+}
+
+static bool valid_orb(const fci::Solver& S, int p) { return p >= 0 && p < S.n_; }
+
+int64_t fci_vo_size(fci_solver* s, int spin, int p, int q) {
+    if (!s || !valid_orb(*s->s, p) || !valid_orb(*s->s, q)) return -1;
+    const fci::SpinTables& t = spin ? s->s->sb_ : s->s->sa_;
+    int nred = t.n - 1 - (p == q ? 0 : 1), kred = t.k - 1;
+    return (kred >= 0 && kred <= nred) ? int64_t(fci::binom_host(nred, kred)) : 0;
+}
+int fci_vo_list(fci_solver* s, int spin, int p, int q, double* sign, uint32_t* I, uint32_t* J) {
+    if (!s || !valid_orb(*s->s, p) || !valid_orb(*s->s, q)) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        std::vector<double> sg;
+        std::vector<uint32_t> a, b;
+        int64_t c = fci::forte_vo_list(spin ? s->s->sb_ : s->s->sa_, s->s->binom_.p, p, q, &sg, &a, &b);
+        std::copy(sg.begin(), sg.begin() + c, sign);
+        std::copy(a.begin(), a.begin() + c, I);
+        std::copy(b.begin(), b.begin() + c, J);
+    });
+}
+int64_t fci_oo_size(fci_solver* s, int spin, int pq) {
+    if (!s || pq < 0 || pq >= s->s->n_ * (s->s->n_ - 1) / 2) return -1;
+    const fci::SpinTables& t = spin ? s->s->sb_ : s->s->sa_;
+    return t.k >= 2 ? int64_t(fci::binom_host(t.n - 2, t.k - 2)) : 0;
+}
+int fci_oo_list(fci_solver* s, int spin, int pq, double* sign, uint32_t* I, uint32_t* J) {
+    if (!s || pq < 0 || pq >= s->s->n_ * (s->s->n_ - 1) / 2) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        int p = 1;
+        while ((p + 1) * p / 2 <= pq) ++p;  // pair index p(p-1)/2 + q, p > q
+        int q = pq - p * (p - 1) / 2;
+        std::vector<double> sg;
+        std::vector<uint32_t> a, b;
+        int64_t c = fci::forte_oo_list(spin ? s->s->sb_ : s->s->sa_, s->s->binom_.p, p, q, &sg, &a, &b);
+        std::copy(sg.begin(), sg.begin() + c, sign);
+        std::copy(a.begin(), a.begin() + c, I);
+        std::copy(b.begin(), b.begin() + c, J);
+    });
+}
+static bool vvoo_ok(const fci::Solver& S, int p, int q, int r, int t) {
+    return valid_orb(S, p) && valid_orb(S, q) && valid_orb(S, r) && valid_orb(S, t) && p > q && r > t &&
+           !(p == r && q == t);
+}
+int64_t fci_vvoo_size(fci_solver* s, int spin, int p, int q, int r, int t) {
+    if (!s || !vvoo_ok(*s->s, p, q, r, t)) return -1;
+    const fci::SpinTables& tb = spin ? s->s->sb_ : s->s->sa_;
+    int overlap = (p == r) || (p == t) || (q == r) || (q == t);
+    int nred = tb.n - 4 + overlap, kred = tb.k - 2;
+    return (kred >= 0 && nred >= 0 && nred >= kred) ? int64_t(fci::binom_host(nred, kred)) : 0;
+}
+int fci_vvoo_list(fci_solver* s, int spin, int p, int q, int r, int t, double* sign, uint32_t* I, uint32_t* J) {
+    if (!s || !vvoo_ok(*s->s, p, q, r, t)) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        std::vector<double> sg;
+        std::vector<uint32_t> a, b;
+        int64_t c = fci::forte_vvoo_list(spin ? s->s->sb_ : s->s->sa_, s->s->binom_.p, p, q, r, t, &sg, &a, &b);
+        std::copy(sg.begin(), sg.begin() + c, sign);
+        std::copy(a.begin(), a.begin() + c, I);
+        std::copy(b.begin(), b.begin() + c, J);
+    });
+}
+
+// ---- device buffers ---------------------------------------------------------------------------
+int fci_device_alloc(size_t bytes, void** out) {
+    if (!out) return FCI_ERR_ARG;
+    return guarded(nullptr, [&] {
+        fci::check_device();
+        HIPCHK(hipMalloc(out, bytes));
+    });
+}
+int fci_device_free(void* p) {
+    return guarded(nullptr, [&] { HIPCHK(hipFree(p)); });
+}
+int fci_device_upload(void* dst, const void* src, size_t bytes) {
+    return guarded(nullptr, [&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int fci_device_download(void* dst, const void* src, size_t bytes) {
+    return guarded(nullptr, [&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int fci_device_synchronize(void) {
+    return guarded(nullptr, [&] { HIPCHK(hipDeviceSynchronize()); });
+}
+
+int fci_sigma_bench(fci_solver* s, const void* C, void* sigma, int dtype, int nrep, double* ms_per_sigma,
+                    double* ms3) {
+    if (!s || !C || !sigma || nrep < 1) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (dtype != 0 && dtype != 1) throw Error(FCI_ERR_ARG, "dtype must be 0 (FP64) or 1 (FP32)");
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double acc3[3] = {0, 0, 0};
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int i = 0; i < nrep; ++i) {
+            if (dtype == 0)
+                S.eng_->apply(static_cast<const double*>(C), static_cast<double*>(sigma), 0);
+            else
+                S.eng_->apply(static_cast<const float*>(C), static_cast<float*>(sigma), 0);
+        }
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_per_sigma) *ms_per_sigma = double(ms) / nrep;
+        // per-kernel-group split from one more, event-instrumented build
+        S.eng_->set_timing(true);
+        if (dtype == 0)
+            S.eng_->apply(static_cast<const double*>(C), static_cast<double*>(sigma), 0);
+        else
+            S.eng_->apply(static_cast<const float*>(C), static_cast<float*>(sigma), 0);
+        S.eng_->set_timing(false);
+        S.eng_->timings(acc3);
+        if (ms3)
+            for (int i = 0; i < 3; ++i) ms3[i] = acc3[i];
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+// ---- standalone Davidson-Liu ------------------------------------------------------------------
+__global__ void k_dense_matvec(const double* __restrict__ M, const double* __restrict__ x, double* __restrict__ y,
+                               int64_t n) {
+    // one wavefront per row
+    int64_t row = int64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s += M[row * n + j] * x[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[row] = s;
+}
+
+int fci_dl_create(int64_t size, int nroot, int cpr, int spr, fci_dl** out) {
+    if (!out) return FCI_ERR_ARG;
+    *out = nullptr;
+    auto h = std::make_unique<fci_dl>();
+    int st = guarded(nullptr, [&] {
+        fci::check_device();
+        h->d.reset(new fci::DavidsonLiu(size, nroot, cpr, spr));
+    });
+    if (st == FCI_OK) *out = h.release();
+    return st;
+}
+void fci_dl_destroy(fci_dl* d) { delete d; }
+const char* fci_dl_last_error(const fci_dl* d) { return d ? d->err.c_str() : g_err.c_str(); }
+int fci_dl_add_h_diag(fci_dl* d, const double* h, int64_t n) {
+    if (!d || !h) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] { d->d->add_h_diag_host(h, n); });
+}
+static std::vector<fci::SparseVec> to_sparse(int n, const int64_t* off, const int64_t* idx, const double* val) {
+    std::vector<fci::SparseVec> v(n);
+    for (int i = 0; i < n; ++i)
+        for (int64_t e = off[i]; e < off[i + 1]; ++e) v[i].push_back({idx[e], val[e]});
+    return v;
+}
+int fci_dl_add_guesses(fci_dl* d, int n, const int64_t* off, const int64_t* idx, const double* val) {
+    if (!d || n < 0 || (n > 0 && (!off || !idx || !val))) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] { d->d->add_guesses(to_sparse(n, off, idx, val)); });
+}
+int fci_dl_add_project_out_vectors(fci_dl* d, int n, const int64_t* off, const int64_t* idx, const double* val) {
+    if (!d || n < 0 || (n > 0 && (!off || !idx || !val))) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] { d->d->add_project_out_vectors(to_sparse(n, off, idx, val)); });
+}
+int fci_dl_set_dense_sigma(fci_dl* d, const double* M) {
+    if (!d || !M) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] {
+        int64_t n = d->d->size();
+        d->M.alloc(size_t(n) * n);
+        d->M.upload(M, size_t(n) * n);
+        double* Mp = d->M.p;
+        d->d->add_sigma_builder([Mp, n](const double* b, double* s) {
+            unsigned g = unsigned((n + 3) / 4);
+            k_dense_matvec<<<g, 256>>>(Mp, b, s, n);
+            HIPCHK(hipGetLastError());
+        });
+    });
+}
+int fci_dl_set_e_convergence(fci_dl* d, double v) { if (!d) return FCI_ERR_ARG; d->d->set_e_convergence(v); return FCI_OK; }
+int fci_dl_set_r_convergence(fci_dl* d, double v) { if (!d) return FCI_ERR_ARG; d->d->set_r_convergence(v); return FCI_OK; }
+int fci_dl_set_maxiter(fci_dl* d, int v) { if (!d) return FCI_ERR_ARG; d->d->set_maxiter(v); return FCI_OK; }
+int fci_dl_set_print_level(fci_dl* d, int v) { if (!d) return FCI_ERR_ARG; d->d->set_print_level(v); return FCI_OK; }
+int fci_dl_reset(fci_dl* d) {
+    if (!d) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] { d->d->reset(); });
+}
+int fci_dl_solve(fci_dl* d, int* converged) {
+    if (!d) return FCI_ERR_ARG;
+    return guarded(&d->err, [&] {
+        bool c = d->d->solve();
+        if (converged) *converged = c;
+    });
+}
+int fci_dl_eigenvalues(const fci_dl* d, double* out) {
+    if (!d || !out) return FCI_ERR_ARG;
+    const auto& l = d->d->eigenvalues();
+    for (int k = 0; k < d->d->nroot(); ++k) out[k] = l[k];
+    return FCI_OK;
+}
+int fci_dl_eigenvector(const fci_dl* d, int k, double* out) {
+    if (!d || !out || k < 0 || k >= d->d->nroot()) return FCI_ERR_ARG;
+    auto* self = const_cast<fci_dl*>(d);
+    return guarded(&self->err, [&] { d->d->eigenvector_host(k, out); });
+}
+
+// Seeded integrals of the benchmark configurations (same recipe as the CPU oracle's generator):
+// SplitMix64(seed); u = ((x >> 11) + 0.5) 2^-53; N = sqrt(-2 ln u1) cos(2 pi u2);
+// h_pp = -2 + 1.5 p/(n-1); h_pq = 0.05 N (p<q, row-major); L^K_pq = 0.1 N (p<=q), L^K_pp += 0.5,
+// K = 0..2n-1; (pq|rs) = sum_K L^K_pq L^K_rs.
+void fci_seeded_integrals(int n, uint64_t seed, double* h, double* eri) {
+    uint64_t st = seed;
+    auto next = [&st]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    auto uni = [&]() { return (double(next() >> 11) + 0.5) * 0x1.0p-53; };
+    auto normal = [&]() {
+        double u1 = uni(), u2 = uni();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586476925286766559 * u2);
+    };
+    for (int p = 0; p < n; ++p) h[p * n + p] = (n > 1) ? -2.0 + 1.5 * double(p) / double(n - 1) : -2.0;
+    for (int p = 0; p < n; ++p)
+        for (int q = p + 1; q < n; ++q) {
+            double v = 0.05 * normal();
+            h[p * n + q] = v;
+            h[q * n + p] = v;
+        }
+    const int nk = 2 * n;
+    const size_t n2 = size_t(n) * n;
+    std::vector<double> L(size_t(nk) * n2);
+    for (int K = 0; K < nk; ++K) {
+        double* l = L.data() + size_t(K) * n2;
+        for (int p = 0; p < n; ++p)
+            for (int q = p; q < n; ++q) {
+                double v = 0.1 * normal();
+                l[p * n + q] = v;
+                l[q * n + p] = v;
+            }
+        for (int p = 0; p < n; ++p) l[p * n + p] += 0.5;
+    }
+    for (size_t pq = 0; pq < n2; ++pq)
+        for (size_t rs = 0; rs < n2; ++rs) {
+            double s = 0.0;
+            for (int K = 0; K < nk; ++K) s += L[size_t(K) * n2 + pq] * L[size_t(K) * n2 + rs];
+            eri[pq * n2 + rs] = s;
+        }
+}
+
+}  // extern "C"

@@ -1,0 +1,428 @@
+// fci_tables.hip -- strings, excitation tables, H diagonal and k-smallest selection on gfx950.
+//
+// String order. Forte enumerates k-of-n strings with std::next_permutation over bits 0..n-1
+// starting from 0..01..1 (forte/fci/fci_string_lists.cc:258-293) and addresses them by order of
+// appearance (fci_string_address.cc:45-50). That order is the colex rank of the bit-reversed
+// occupation: with c_1 < ... < c_k the values n-1-i over set bits i, address = sum_t C(c_t, t).
+// Every table below is therefore built in parallel: a thread unranks its index into a bit
+// pattern, or ranks a bit pattern into an address, with a small binomial table.
+//
+// Forte-order lists (parity only). make_vo / make_oo / make_vvoo
+// (fci_string_vo_list.cc:90-140, fci_string_oo_list.cc:72-115, fci_string_vvoo_list.cc:88-187)
+// enumerate the remaining orbitals with the same next_permutation order, so entry t of a list
+// is "unrank t in the reduced space, then re-insert the fixed orbitals".
+#include "fci_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+
+namespace fci {
+
+[[noreturn]] void hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", int(e), hipGetErrorString(e),
+             what, file, line);
+    throw Error(-3, buf);
+}
+
+std::vector<uint32_t> binom_table() {
+    std::vector<uint32_t> b(33 * 33, 0);
+    for (int a = 0; a <= 32; ++a)
+        for (int c = 0; c <= a; ++c) b[a * 33 + c] = uint32_t(binom_host(a, c));
+    return b;
+}
+
+__device__ __forceinline__ uint32_t d_rank(uint32_t s, int n, const uint32_t* __restrict__ B) {
+    uint32_t r = 0;
+    int t = 0;
+    while (s) {
+        int i = 31 - __clz(s);
+        s &= ~(1u << i);
+        ++t;
+        r += B[(n - 1 - i) * 33 + t];
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t d_unrank(uint32_t a, int n, int k, const uint32_t* __restrict__ B) {
+    uint32_t s = 0;
+    int c = n;
+    for (int t = k; t >= 1; --t) {
+        int cc = c - 1;
+        while (B[cc * 33 + t] > a) --cc;
+        a -= B[cc * 33 + t];
+        s |= 1u << (n - 1 - cc);
+        c = cc;
+    }
+    return s;
+}
+
+// (-1)^(set bits of s strictly between positions x and y)
+__device__ __forceinline__ uint32_t d_between_parity(uint32_t s, int x, int y) {
+    int lo = x < y ? x : y, hi = x < y ? y : x;
+    if (hi - lo < 2) return 0u;
+    uint32_t mask = ((1u << hi) - 1u) & ~((2u << lo) - 1u);
+    return __popc(s & mask) & 1u;
+}
+
+__device__ __forceinline__ int d_pair_packed(int p, int q) {
+    int hi = p > q ? p : q, lo = p > q ? q : p;
+    return hi * (hi + 1) / 2 + lo;
+}
+
+__global__ void k_strings(uint32_t* out, int64_t nstr, int n, int k, const uint32_t* B) {
+    int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (a < nstr) out[a] = d_unrank(uint32_t(a), n, k, B);
+}
+
+// singles into J (p occupied in J, q empty in J): J = sign a+_p a_q I
+__global__ void k_into(const uint32_t* __restrict__ str, uint32_t* out, int64_t nstr, int n, int k,
+                       int ns, const uint32_t* B) {
+    int64_t J = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (J >= nstr) return;
+    uint32_t s = str[J];
+    uint32_t* o = out + J * ns;
+    int e = 0;
+    for (int p = 0; p < n; ++p) {
+        if (!((s >> p) & 1u)) continue;
+        for (int q = 0; q < n; ++q) {
+            if ((s >> q) & 1u) continue;
+            uint32_t I = s ^ (1u << p) ^ (1u << q);
+            uint32_t neg = d_between_parity(s, p, q);
+            o[e++] = d_rank(I, n, B) | (uint32_t(d_pair_packed(p, q)) << 21) | (neg << 31);
+        }
+    }
+}
+
+// out of I along the packed pair P = {r >= s}; stored pair-major out[P*nstr + I]
+__global__ void k_out(const uint32_t* __restrict__ str, uint32_t* out, int64_t nstr, int n, int np,
+                      const uint32_t* B) {
+    int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nstr * np) return;
+    int64_t I = t / np;
+    int P = int(t - I * np);
+    int r = 0;
+    while ((r + 1) * (r + 2) / 2 <= P) ++r;
+    int s = P - r * (r + 1) / 2;
+    uint32_t x = str[I];
+    uint32_t br = (x >> r) & 1u, bs = (x >> s) & 1u;
+    uint32_t v = kInvalid;
+    if (r == s) {
+        if (br) v = uint32_t(I);
+    } else if (br != bs) {
+        uint32_t Jb = x ^ (1u << r) ^ (1u << s);
+        v = d_rank(Jb, n, B) | (d_between_parity(x, r, s) << 31);
+    }
+    out[int64_t(P) * nstr + I] = v;  // pair-major: consecutive strings are adjacent
+}
+
+// pair cliques: hole string K (k-d electrons), members K + d electrons of vir(K)
+__global__ void k_clique(uint32_t* out, int64_t nK, int n, int k, int d, int M, const uint32_t* B) {
+    int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= nK) return;
+    uint32_t K = d_unrank(uint32_t(x), n, k - d, B);
+    uint32_t* o = out + x * M;
+    int m = 0;
+    if (d == 2) {
+        for (int a = 0; a < n; ++a) {
+            if ((K >> a) & 1u) continue;
+            for (int b = 0; b < a; ++b) {
+                if ((K >> b) & 1u) continue;
+                uint32_t I = K | (1u << a) | (1u << b);
+                // a+_a a+_b |K>: (-1)^(n_<b(K)) (-1)^(n_<a(K)+1)
+                uint32_t nb_ = __popc(K & ((1u << b) - 1u)), na_ = __popc(K & ((1u << a) - 1u));
+                uint32_t neg = (na_ + nb_ + 1u) & 1u;
+                o[m++] = d_rank(I, n, B) | (uint32_t(a * (a - 1) / 2 + b) << 21) | (neg << 31);
+            }
+        }
+    } else {
+        for (int a = 0; a < n; ++a) {
+            if ((K >> a) & 1u) continue;
+            uint32_t I = K | (1u << a);
+            uint32_t neg = __popc(K & ((1u << a) - 1u)) & 1u;
+            o[m++] = d_rank(I, n, B) | (uint32_t(a) << 21) | (neg << 31);
+        }
+    }
+}
+
+static inline unsigned nblk(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream_t st) {
+    t.n = n;
+    t.k = k;
+    t.nstr = int64_t(binom_host(n, k));
+    if (t.nstr > int64_t(kAddrMask)) throw Error(-1, "string space too large for 21-bit addresses");
+    t.str.alloc(t.nstr);
+    k_strings<<<nblk(t.nstr, 256), 256, 0, st>>>(t.str.p, t.nstr, n, k, B);
+    t.ns = k * (n - k);
+    if (t.ns > 0) {
+        t.into.alloc(size_t(t.nstr) * t.ns);
+        k_into<<<nblk(t.nstr, 128), 128, 0, st>>>(t.str.p, t.into.p, t.nstr, n, k, t.ns, B);
+    }
+    t.np = n * (n + 1) / 2;
+    t.out.alloc(size_t(t.nstr) * t.np);
+    k_out<<<nblk(t.nstr * t.np, 256), 256, 0, st>>>(t.str.p, t.out.p, t.nstr, n, t.np, B);
+    t.d = k >= 2 ? 2 : (k == 1 ? 1 : 0);
+    if (t.d > 0) {
+        t.nK = int64_t(binom_host(n, k - t.d));
+        t.M = int(binom_host(n - k + t.d, t.d));
+        t.clq.alloc(size_t(t.nK) * t.M);
+        k_clique<<<nblk(t.nK, 64), 64, 0, st>>>(t.clq.p, t.nK, n, k, t.d, t.M, B);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// ---- Forte-order lists ---------------------------------------------------------------------
+// Insert the reduced-space bits b (n' bits) into the orbitals not in `skip` (ascending).
+__device__ __forceinline__ uint32_t d_expand(uint32_t b, int n, uint32_t skip) {
+    uint32_t s = 0;
+    int kk = 0;
+    for (int i = 0; i < n; ++i) {
+        if ((skip >> i) & 1u) continue;
+        if ((b >> kk) & 1u) s |= 1u << i;
+        ++kk;
+    }
+    return s;
+}
+
+__global__ void k_forte_vo(int64_t cnt, int n, int nred, int kred, int p, int q, const uint32_t* B,
+                           double* sign, uint32_t* I, uint32_t* J) {
+    int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint32_t b = d_unrank(uint32_t(t), nred, kred, B);
+    uint32_t base = d_expand(b, n, (1u << p) | (1u << q));
+    uint32_t Is = (base & ~(1u << p)) | (1u << q);
+    uint32_t Js = (base & ~(1u << q)) | (1u << p);
+    sign[t] = d_between_parity(base, p, q) ? -1.0 : 1.0;
+    I[t] = d_rank(Is, n, B);
+    J[t] = d_rank(Js, n, B);
+}
+
+__global__ void k_forte_oo(int64_t cnt, int n, int p, int q, int kred, const uint32_t* B,
+                           double* sign, uint32_t* I, uint32_t* J) {
+    int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint32_t b = d_unrank(uint32_t(t), n - 2, kred, B);
+    uint32_t s = d_expand(b, n, (1u << p) | (1u << q)) | (1u << p) | (1u << q);
+    uint32_t a = d_rank(s, n, B);
+    sign[t] = 1.0;
+    I[t] = a;
+    J[t] = a;
+}
+
+__global__ void k_forte_vvoo(int64_t cnt, int n, int nred, int kred, int p, int q, int r, int s,
+                             const uint32_t* B, double* sign, uint32_t* I, uint32_t* J) {
+    int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    uint32_t b = d_unrank(uint32_t(t), nred, kred, B);
+    uint32_t skip = (1u << p) | (1u << q) | (1u << r) | (1u << s);
+    uint32_t Is = d_expand(b, n, skip) | (1u << r) | (1u << s);
+    // a+_p a+_q a_s a_r applied to I (fci_string_vvoo_list.cc:166-182)
+    uint32_t Js = Is;
+    uint32_t neg = __popc(Js & (((1u << r) - 1u) & ~((1u << s) - 1u))) & 1u;  // bits in [s, r)
+    Js &= ~((1u << r) | (1u << s));
+    Js |= 1u << q;
+    neg ^= __popc(Js & ((1u << q) - 1u)) & 1u;
+    Js |= 1u << p;
+    neg ^= __popc(Js & ((1u << p) - 1u)) & 1u;
+    sign[t] = neg ? -1.0 : 1.0;
+    I[t] = d_rank(Is, n, B);
+    J[t] = d_rank(Js, n, B);
+}
+
+static int64_t run_list(int64_t cnt, std::vector<double>* sign, std::vector<uint32_t>* I,
+                        std::vector<uint32_t>* J, const std::function<void(double*, uint32_t*, uint32_t*)>& launch) {
+    if (cnt <= 0) {
+        if (sign) sign->clear();
+        if (I) I->clear();
+        if (J) J->clear();
+        return 0;
+    }
+    if (!sign) return cnt;
+    DBuf<double> ds(cnt);
+    DBuf<uint32_t> dI(cnt), dJ(cnt);
+    launch(ds.p, dI.p, dJ.p);
+    HIPCHK(hipGetLastError());
+    *sign = ds.to_host();
+    *I = dI.to_host();
+    *J = dJ.to_host();
+    return cnt;
+}
+
+int64_t forte_vo_list(const SpinTables& t, const uint32_t* B, int p, int q, std::vector<double>* sign,
+                      std::vector<uint32_t>* I, std::vector<uint32_t>* J) {
+    int nred = t.n - 1 - (p == q ? 0 : 1), kred = t.k - 1;
+    int64_t cnt = (kred >= 0 && kred <= nred) ? int64_t(binom_host(nred, kred)) : 0;
+    return run_list(cnt, sign, I, J, [&](double* s, uint32_t* a, uint32_t* b) {
+        k_forte_vo<<<nblk(cnt, 256), 256>>>(cnt, t.n, nred, kred, p, q, B, s, a, b);
+    });
+}
+
+int64_t forte_oo_list(const SpinTables& t, const uint32_t* B, int p, int q, std::vector<double>* sign,
+                      std::vector<uint32_t>* I, std::vector<uint32_t>* J) {
+    int kred = t.k - 2;
+    int64_t cnt = kred >= 0 ? int64_t(binom_host(t.n - 2, kred)) : 0;
+    return run_list(cnt, sign, I, J, [&](double* s, uint32_t* a, uint32_t* b) {
+        k_forte_oo<<<nblk(cnt, 256), 256>>>(cnt, t.n, p, q, kred, B, s, a, b);
+    });
+}
+
+int64_t forte_vvoo_list(const SpinTables& t, const uint32_t* B, int p, int q, int r, int s,
+                        std::vector<double>* sign, std::vector<uint32_t>* I, std::vector<uint32_t>* J) {
+    int overlap = (p == r) || (p == s) || (q == r) || (q == s);
+    int nred = t.n - 4 + overlap, kred = t.k - 2;
+    int64_t cnt = (kred >= 0 && nred >= 0 && nred >= kred) ? int64_t(binom_host(nred, kred)) : 0;
+    return run_list(cnt, sign, I, J, [&](double* sg, uint32_t* a, uint32_t* b) {
+        k_forte_vvoo<<<nblk(cnt, 256), 256>>>(cnt, t.n, nred, kred, p, q, r, s, B, sg, a, b);
+    });
+}
+
+// ---- H diagonal ------------------------------------------------------------------------------
+__global__ void k_hdiag(const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb, int64_t nb,
+                        int64_t ndet, int n, const double* __restrict__ oa, const double* __restrict__ ob,
+                        const double* __restrict__ taa, const double* __restrict__ tab,
+                        const double* __restrict__ tbb, double e_core, double* __restrict__ out) {
+    __shared__ double dg_a[kMaxOrb], dg_b[kMaxOrb];
+    __shared__ double jaa[kMaxOrb * kMaxOrb], jab[kMaxOrb * kMaxOrb], jbb[kMaxOrb * kMaxOrb];
+    const int n2 = n * n, n3 = n2 * n;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        int p = i / n, q = i % n;
+        size_t ix = size_t(p) * n3 + size_t(q) * n2 + size_t(p) * n + q;
+        jaa[i] = taa[ix];
+        jab[i] = tab[ix];
+        jbb[i] = tbb[ix];
+        if (q == 0) {
+            dg_a[p] = oa[p * n + p];
+            dg_b[p] = ob[p * n + p];
+        }
+    }
+    __syncthreads();
+    int64_t I = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (I >= ndet) return;
+    uint32_t A = sa[I / nb], Bs = sb[I % nb];
+    // ActiveSpaceIntegrals::energy (active_space_integrals.cc:206-245), same summation order
+    double e = 0.0;
+    uint32_t a = A;
+    while (a) {
+        int p = __ffs(a) - 1;
+        a &= a - 1u;
+        e += dg_a[p];
+        uint32_t ac = a;
+        while (ac) {
+            int q = __ffs(ac) - 1;
+            ac &= ac - 1u;
+            e += jaa[p * n + q];
+        }
+        uint32_t bc = Bs;
+        while (bc) {
+            int q = __ffs(bc) - 1;
+            bc &= bc - 1u;
+            e += jab[p * n + q];
+        }
+    }
+    uint32_t b = Bs;
+    while (b) {
+        int p = __ffs(b) - 1;
+        b &= b - 1u;
+        e += dg_b[p];
+        uint32_t bc = b;
+        while (bc) {
+            int q = __ffs(bc) - 1;
+            bc &= bc - 1u;
+            e += jbb[p * n + q];
+        }
+    }
+    out[I] = e_core + e;
+}
+
+void hdiag_device(const SpinTables& a, const SpinTables& b, const double* oa, const double* ob,
+                  const double* taa, const double* tab, const double* tbb, double e_core,
+                  double* out, hipStream_t st) {
+    int64_t nd = a.nstr * b.nstr;
+    k_hdiag<<<nblk(nd, 256), 256, 0, st>>>(a.str.p, b.str.p, b.nstr, nd, a.n, oa, ob, taa, tab, tbb,
+                                          e_core, out);
+    HIPCHK(hipGetLastError());
+}
+
+// ---- k smallest by (value, index) ------------------------------------------------------------
+constexpr int kSelChunk = 2048;
+
+__device__ __forceinline__ bool kv_less(double va, int64_t ia, double vb, int64_t ib) {
+    return va < vb || (va == vb && ia < ib);
+}
+
+// Each block sorts a chunk of (value,index) pairs in LDS (bitonic) and keeps the first k.
+__global__ void __launch_bounds__(1024) k_select(const double* __restrict__ v, const int64_t* __restrict__ idx_in,
+                                                 int64_t n, int k, double* ov, int64_t* oi) {
+    __shared__ double sv[kSelChunk];
+    __shared__ int64_t si[kSelChunk];
+    int64_t base = int64_t(blockIdx.x) * kSelChunk;
+    for (int i = threadIdx.x; i < kSelChunk; i += blockDim.x) {
+        int64_t g = base + i;
+        if (g < n) {
+            sv[i] = v[g];
+            si[i] = idx_in ? idx_in[g] : g;
+        } else {
+            sv[i] = __longlong_as_double(0x7FF0000000000000LL);  // +inf
+            si[i] = INT64_MAX;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= kSelChunk; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < kSelChunk; i += blockDim.x) {
+                int j = i ^ stride;
+                if (j > i) {
+                    bool up = (i & size) == 0;
+                    bool sw = up ? kv_less(sv[j], si[j], sv[i], si[i]) : kv_less(sv[i], si[i], sv[j], si[j]);
+                    if (sw) {
+                        double tv = sv[i];
+                        sv[i] = sv[j];
+                        sv[j] = tv;
+                        int64_t ti = si[i];
+                        si[i] = si[j];
+                        si[j] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < k; i += blockDim.x) {
+        ov[int64_t(blockIdx.x) * k + i] = sv[i];
+        oi[int64_t(blockIdx.x) * k + i] = si[i];
+    }
+}
+
+void smallest_k(const double* d_v, int64_t n, int k, std::vector<double>& vals,
+                std::vector<int64_t>& idx, hipStream_t st) {
+    if (k > kSelChunk / 2) throw Error(-1, "smallest_k: k too large");
+    if (k > n) k = int(n);
+    const double* cur_v = d_v;
+    const int64_t* cur_i = nullptr;
+    int64_t cur_n = n;
+    DBuf<double> bv[2];
+    DBuf<int64_t> bi[2];
+    int which = 0;
+    while (true) {
+        HIPCHK(hipStreamSynchronize(st));  // the buffer about to be reallocated may still be read
+        int64_t nb = (cur_n + kSelChunk - 1) / kSelChunk;
+        bv[which].alloc(nb * k);
+        bi[which].alloc(nb * k);
+        k_select<<<unsigned(nb), 1024, 0, st>>>(cur_v, cur_i, cur_n, k, bv[which].p, bi[which].p);
+        HIPCHK(hipGetLastError());
+        cur_v = bv[which].p;
+        cur_i = bi[which].p;
+        cur_n = nb * k;
+        if (nb == 1) break;
+        which ^= 1;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    vals.resize(k);
+    idx.resize(k);
+    HIPCHK(hipMemcpy(vals.data(), cur_v, k * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(idx.data(), cur_i, k * sizeof(int64_t), hipMemcpyDeviceToHost));
+}
+
+}  // namespace fci

@@ -1,0 +1,534 @@
+// forte_amd_py.cpp -- pybind11 module _forte_amd: the host side above the C ABI.
+//
+// Mirrors the Python surface of Forte's FCI path:
+//   ActiveSpaceIntegrals-like container   forte/integrals/active_space_integrals.h:41-213
+//   FCISolver (ActiveSpaceMethod)         forte/base_classes/active_space_method.h:95-388,
+//                                         forte/fci/fci_solver.h:51-246
+//   RDMs / RDMsType                       forte/base_classes/rdms.h:116-400, api/rdms_api.cc:49-103
+//   DavidsonLiuSolver + test sigma        forte/api/dl_solver_api.cc:45-86
+// Status codes from the C ABI become exceptions (RuntimeError in Python), as the reference's
+// C++ exceptions do through pybind11.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/forte_fci.h"
+
+namespace py = pybind11;
+using arr = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+// The message is read only after the call returned (argument evaluation order is unspecified).
+#define check(call, errexpr)                                                          \
+    do {                                                                              \
+        int st_ = (call);                                                             \
+        if (st_ != FCI_OK) throw std::runtime_error(std::string(errexpr));            \
+    } while (0)
+
+struct ActiveSpaceIntegrals {
+    int n;
+    std::vector<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;
+    double scalar_energy = 0.0, frozen_core_energy = 0.0, nuclear_repulsion_energy = 0.0;
+    double e_core() const { return scalar_energy + frozen_core_energy + nuclear_repulsion_energy; }
+};
+
+std::vector<double> vec_of(const arr& a, size_t expect, const char* what) {
+    if (size_t(a.size()) != expect)
+        throw std::runtime_error(std::string(what) + ": expected " + std::to_string(expect) + " elements");
+    return std::vector<double>(a.data(), a.data() + a.size());
+}
+
+py::array_t<double> make_arr(const std::vector<ssize_t>& shape, const std::vector<double>& data) {
+    py::array_t<double> a(shape);
+    std::copy(data.begin(), data.end(), a.mutable_data());
+    return a;
+}
+
+enum class RDMsType { spin_dependent, spin_free };
+
+struct RDMs {
+    int level = 0, n = 0;
+    RDMsType type = RDMsType::spin_dependent;
+    std::vector<double> g1a, g1b, g2aa, g2ab, g2bb;
+    py::array_t<double> t2(const std::vector<double>& v) const {
+        if (v.empty()) throw std::runtime_error("RDM not available at this level");
+        return make_arr({n, n}, v);
+    }
+    py::array_t<double> t4(const std::vector<double>& v) const {
+        if (v.empty()) throw std::runtime_error("RDM not available at this level");
+        return make_arr({n, n, n, n}, v);
+    }
+    // RDMsSpinFree: G1 = g1a + g1b; G2 = g2aa + g2ab + g2ab(qpsr) + g2bb (fci_vector_rdm.cc:121-126)
+    std::vector<double> sf_g1() const {
+        std::vector<double> g(g1a.size());
+        for (size_t i = 0; i < g.size(); ++i) g[i] = g1a[i] + g1b[i];
+        return g;
+    }
+    std::vector<double> sf_g2() const {
+        if (g2aa.empty()) throw std::runtime_error("2-RDM not available at this level");
+        std::vector<double> g(g2aa.size());
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q)
+                for (int r = 0; r < n; ++r)
+                    for (int s = 0; s < n; ++s) {
+                        size_t i = ((size_t(p) * n + q) * n + r) * n + s;
+                        size_t j = ((size_t(q) * n + p) * n + s) * n + r;
+                        g[i] = g2aa[i] + g2ab[i] + g2ab[j] + g2bb[i];
+                    }
+        return g;
+    }
+};
+
+// Device buffer owned by Python (FP64 or FP32), filled and read through host copies.
+class DeviceArray {
+  public:
+    DeviceArray(int64_t n, const std::string& dtype) : n_(n) {
+        if (dtype == "float64") es_ = 8;
+        else if (dtype == "float32") es_ = 4;
+        else throw std::runtime_error("DeviceArray dtype must be float64 or float32");
+        dtype_ = dtype;
+        check(fci_device_alloc(size_t(n) * es_, &p_), fci_last_error(nullptr));
+    }
+    ~DeviceArray() {
+        if (p_) fci_device_free(p_);
+    }
+    DeviceArray(const DeviceArray&) = delete;
+    void upload(py::array a) {
+        if (a.size() != n_) throw std::runtime_error("DeviceArray.upload: size mismatch");
+        py::array b = dtype_ == "float64" ? py::array(py::array_t<double, py::array::c_style | py::array::forcecast>(a))
+                                          : py::array(py::array_t<float, py::array::c_style | py::array::forcecast>(a));
+        check(fci_device_upload(p_, b.data(), size_t(n_) * es_), fci_last_error(nullptr));
+    }
+    py::array download() const {
+        if (es_ == 8) {
+            py::array_t<double> a(n_);
+            check(fci_device_download(a.mutable_data(), p_, size_t(n_) * es_), fci_last_error(nullptr));
+            return a;
+        }
+        py::array_t<float> a(n_);
+        check(fci_device_download(a.mutable_data(), p_, size_t(n_) * es_), fci_last_error(nullptr));
+        return a;
+    }
+    uintptr_t ptr() const { return reinterpret_cast<uintptr_t>(p_); }
+    int64_t size() const { return n_; }
+    int dtype_code() const { return es_ == 8 ? 0 : 1; }
+
+  private:
+    int64_t n_;
+    size_t es_ = 8;
+    std::string dtype_;
+    void* p_ = nullptr;
+};
+
+class FCISolver {
+  public:
+    FCISolver(int na, int nb, int multiplicity, int nroot, std::shared_ptr<ActiveSpaceIntegrals> ints)
+        : ints_(std::move(ints)) {
+        check(fci_create(ints_->n, na, nb, ints_->oei_a.data(), ints_->oei_b.data(), ints_->tei_aa.data(),
+                         ints_->tei_ab.data(), ints_->tei_bb.data(), ints_->e_core(), &h_),
+              fci_last_error(nullptr));
+        fci_default_options(&o_);
+        o_.nroot = nroot;
+        o_.multiplicity = multiplicity;
+        push();
+    }
+    ~FCISolver() { fci_destroy(h_); }
+    FCISolver(const FCISolver&) = delete;
+
+    void push() { check(fci_set_options(h_, &o_), fci_last_error(h_)); }
+
+    // ActiveSpaceMethod / FCISolver setters (active_space_method.h:285-320, fci_solver.h:88-110)
+    void set_e_convergence(double v) { o_.e_convergence = v; push(); }
+    void set_r_convergence(double v) { o_.r_convergence = v; push(); }
+    void set_maxiter(int v) { o_.maxiter = v; push(); }
+    void set_die_if_not_converged(bool v) { o_.die_if_not_converged = v; push(); }
+    void set_root(int v) { o_.root = v; push(); }
+    void set_print(int v) { o_.print = v; push(); }
+    void set_ndets_per_guess_state(int v) { o_.ndets_per_guess = v; push(); }
+    void set_guess_per_root(int v) { o_.guess_per_root = v; push(); }
+    void set_collapse_per_root(int v) { o_.collapse_per_root = v; push(); }
+    void set_subspace_per_root(int v) { o_.subspace_per_root = v; push(); }
+    void set_precision(const std::string& p) {
+        if (p == "fp64") o_.precision = FCI_PREC_FP64;
+        else if (p == "mixed") o_.precision = FCI_PREC_MIXED;
+        else throw std::runtime_error("precision must be 'fp64' or 'mixed'");
+        push();
+    }
+    void set_mixed_switch(double v) { o_.mixed_switch = v; push(); }
+    // FCISolver::set_options (fci_solver.cc:156-172) from a dict of Forte option names
+    void set_options(const py::dict& d) {
+        for (auto item : d) {
+            std::string k = py::str(item.first);
+            py::handle v = item.second;
+            if (k == "DL_MAXITER") o_.maxiter = v.cast<int>();
+            else if (k == "E_CONVERGENCE") o_.e_convergence = v.cast<double>();
+            else if (k == "R_CONVERGENCE") o_.r_convergence = v.cast<double>();
+            else if (k == "ROOT") o_.root = v.cast<int>();
+            else if (k == "DL_GUESS_PER_ROOT") o_.guess_per_root = v.cast<int>();
+            else if (k == "DL_DETS_PER_GUESS") o_.ndets_per_guess = v.cast<int>();
+            else if (k == "DL_COLLAPSE_PER_ROOT") o_.collapse_per_root = v.cast<int>();
+            else if (k == "DL_SUBSPACE_PER_ROOT") o_.subspace_per_root = v.cast<int>();
+            else if (k == "PRINT") o_.print = v.cast<int>();
+            else if (k == "CI_SPIN_ADAPT") {
+                if (v.cast<bool>()) throw std::runtime_error("CI_SPIN_ADAPT is not supported by this solver");
+            } else if (k == "FCI_TEST_RDMS") {
+                // the brute-force RDM self test is the test suite's job here
+            } else if (k == "FCI_PRECISION") set_precision(v.cast<std::string>());
+            else throw std::runtime_error("unknown FCI option " + k);
+        }
+        push();
+    }
+    void set_active_space_integrals(std::shared_ptr<ActiveSpaceIntegrals> ints) {
+        ints_ = std::move(ints);
+        check(fci_set_integrals(h_, ints_->oei_a.data(), ints_->oei_b.data(), ints_->tei_aa.data(),
+                                ints_->tei_ab.data(), ints_->tei_bb.data(), ints_->e_core()),
+              fci_last_error(h_));
+    }
+
+    double compute_energy() {
+        double e = 0.0;
+        int st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_compute_energy(h_, &e);
+        }
+        check(st, fci_last_error(h_));
+        return e;
+    }
+    std::vector<double> energies() const {
+        std::vector<double> e(o_.nroot);
+        check(fci_energies(h_, e.data(), nullptr), "compute_energy has not been called");
+        return e;
+    }
+    std::vector<double> spin2() const {
+        std::vector<double> s(o_.nroot);
+        check(fci_energies(h_, nullptr, s.data()), "compute_energy has not been called");
+        return s;
+    }
+    py::dict stats() const {
+        int it, ns, ns64, conv, fb;
+        double sec;
+        fci_solve_stats(h_, &it, &ns, &ns64, &sec, &conv, &fb);
+        py::dict d;
+        d["iterations"] = it;
+        d["sigma_builds"] = ns;
+        d["sigma_builds_fp64"] = ns64;
+        d["seconds"] = sec;
+        d["converged"] = bool(conv);
+        d["fell_back"] = bool(fb);
+        return d;
+    }
+    py::array_t<double> evec(int root) const {
+        int64_t nd = fci_ndet(h_);
+        py::array_t<double> a(nd);
+        check(fci_evec(h_, root, a.mutable_data()), fci_last_error(h_));
+        return a;
+    }
+    int64_t ndet() const { return fci_ndet(h_); }
+    int64_t nstr(int spin) const { return fci_nstr(h_, spin); }
+
+    // ActiveSpaceMethod::rdms(root_list, max_rdm_level, type), active_space_method.h:134-136
+    std::vector<std::shared_ptr<RDMs>> rdms(const std::vector<std::pair<int, int>>& roots, int level, RDMsType type) {
+        std::vector<std::shared_ptr<RDMs>> out;
+        const int n = ints_->n;
+        const size_t n2 = size_t(n) * n, n4 = n2 * n2;
+        for (auto [l, r] : roots) {
+            auto R = std::make_shared<RDMs>();
+            R->level = level;
+            R->n = n;
+            R->type = type;
+            R->g1a.resize(n2);
+            R->g1b.resize(n2);
+            if (level >= 2) {
+                R->g2aa.resize(n4);
+                R->g2ab.resize(n4);
+                R->g2bb.resize(n4);
+            }
+            int st;
+            {
+                py::gil_scoped_release nogil;
+                st = fci_rdms(h_, l, r, level, R->g1a.data(), R->g1b.data(), level >= 2 ? R->g2aa.data() : nullptr,
+                              level >= 2 ? R->g2ab.data() : nullptr, level >= 2 ? R->g2bb.data() : nullptr);
+            }
+            check(st, fci_last_error(h_));
+            out.push_back(R);
+        }
+        return out;
+    }
+
+    py::array_t<double> sigma(const arr& C) {
+        int64_t nd = fci_ndet(h_);
+        if (C.size() != nd) throw std::runtime_error("sigma: vector size must equal the number of determinants");
+        py::array_t<double> s(nd);
+        int st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_sigma(h_, C.data(), s.mutable_data());
+        }
+        check(st, fci_last_error(h_));
+        return s;
+    }
+    void sigma_device(uintptr_t c, uintptr_t s, int dtype, uintptr_t stream) {
+        check(fci_sigma_device(h_, reinterpret_cast<const void*>(c), reinterpret_cast<void*>(s), dtype,
+                               reinterpret_cast<void*>(stream)),
+              fci_last_error(h_));
+    }
+    py::tuple sigma_bench(const DeviceArray& C, DeviceArray& S, int nrep) {
+        if (C.size() != fci_ndet(h_) || S.size() != C.size() || C.dtype_code() != S.dtype_code())
+            throw std::runtime_error("sigma_bench: buffers must hold ndet elements of one dtype");
+        double ms = 0.0;
+        std::vector<double> t(3);
+        int st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_sigma_bench(h_, reinterpret_cast<const void*>(C.ptr()), reinterpret_cast<void*>(S.ptr()),
+                                 C.dtype_code(), nrep, &ms, t.data());
+        }
+        check(st, fci_last_error(h_));
+        return py::make_tuple(ms, t);
+    }
+    void sigma_arrays(const DeviceArray& C, DeviceArray& S) {
+        if (C.size() != fci_ndet(h_) || S.size() != C.size() || C.dtype_code() != S.dtype_code())
+            throw std::runtime_error("sigma_arrays: buffers must hold ndet elements of one dtype");
+        check(fci_sigma_device(h_, reinterpret_cast<const void*>(C.ptr()), reinterpret_cast<void*>(S.ptr()),
+                               C.dtype_code(), nullptr),
+              fci_last_error(h_));
+        check(fci_device_synchronize(), fci_last_error(nullptr));
+    }
+    std::vector<double> sigma_timings() const {
+        std::vector<double> t(3);
+        fci_sigma_timings(h_, t.data());
+        return t;
+    }
+    double spin2_of(const arr& C) {
+        double s = 0.0;
+        if (C.size() != fci_ndet(h_)) throw std::runtime_error("spin2_of: wrong vector size");
+        check(fci_spin2_of(h_, C.data(), &s), fci_last_error(h_));
+        return s;
+    }
+    py::array_t<double> hdiag() {
+        py::array_t<double> a(fci_ndet(h_));
+        check(fci_hdiag(h_, a.mutable_data()), fci_last_error(h_));
+        return a;
+    }
+    py::array_t<uint64_t> strings(int spin) {
+        py::array_t<uint64_t> a(fci_nstr(h_, spin));
+        check(fci_strings(h_, spin, a.mutable_data()), fci_last_error(h_));
+        return a;
+    }
+    py::tuple list3(int64_t cnt, const std::function<int(double*, uint32_t*, uint32_t*)>& f) {
+        if (cnt < 0) throw std::runtime_error("invalid list key");
+        py::array_t<double> s(cnt);
+        py::array_t<uint32_t> I(cnt), J(cnt);
+        check(f(s.mutable_data(), I.mutable_data(), J.mutable_data()), fci_last_error(h_));
+        return py::make_tuple(s, I, J);
+    }
+    py::tuple vo_list(int spin, int p, int q) {
+        return list3(fci_vo_size(h_, spin, p, q),
+                     [&](double* s, uint32_t* I, uint32_t* J) { return fci_vo_list(h_, spin, p, q, s, I, J); });
+    }
+    py::tuple oo_list(int spin, int pq) {
+        return list3(fci_oo_size(h_, spin, pq),
+                     [&](double* s, uint32_t* I, uint32_t* J) { return fci_oo_list(h_, spin, pq, s, I, J); });
+    }
+    py::tuple vvoo_list(int spin, int p, int q, int r, int t) {
+        return list3(fci_vvoo_size(h_, spin, p, q, r, t), [&](double* s, uint32_t* I, uint32_t* J) {
+            return fci_vvoo_list(h_, spin, p, q, r, t, s, I, J);
+        });
+    }
+    const fci_options& options() const { return o_; }
+
+  private:
+    std::shared_ptr<ActiveSpaceIntegrals> ints_;
+    fci_solver* h_ = nullptr;
+    fci_options o_;
+};
+
+class DavidsonLiuSolver {
+  public:
+    DavidsonLiuSolver(int64_t size, int nroot, int cpr, int spr) : nroot_(nroot), size_(size) {
+        check(fci_dl_create(size, nroot, cpr, spr, &d_), fci_dl_last_error(nullptr));
+    }
+    ~DavidsonLiuSolver() { fci_dl_destroy(d_); }
+    void add_h_diag(const arr& h) { check(fci_dl_add_h_diag(d_, h.data(), h.size()), fci_dl_last_error(d_)); }
+    static void flatten(const std::vector<std::vector<std::pair<int64_t, double>>>& v, std::vector<int64_t>& off,
+                        std::vector<int64_t>& idx, std::vector<double>& val) {
+        off.assign(1, 0);
+        for (const auto& x : v) {
+            for (auto [i, c] : x) {
+                idx.push_back(i);
+                val.push_back(c);
+            }
+            off.push_back(int64_t(idx.size()));
+        }
+    }
+    void add_guesses(const std::vector<std::vector<std::pair<int64_t, double>>>& g) {
+        std::vector<int64_t> off, idx;
+        std::vector<double> val;
+        flatten(g, off, idx, val);
+        check(fci_dl_add_guesses(d_, int(g.size()), off.data(), idx.data(), val.data()), fci_dl_last_error(d_));
+    }
+    void add_project_out_vectors(const std::vector<std::vector<std::pair<int64_t, double>>>& g) {
+        std::vector<int64_t> off, idx;
+        std::vector<double> val;
+        flatten(g, off, idx, val);
+        check(fci_dl_add_project_out_vectors(d_, int(g.size()), off.data(), idx.data(), val.data()),
+              fci_dl_last_error(d_));
+    }
+    void add_test_sigma_builder(const arr& M) {
+        if (M.size() != size_ * size_) throw std::runtime_error("add_test_sigma_builder: matrix must be size x size");
+        check(fci_dl_set_dense_sigma(d_, M.data()), fci_dl_last_error(d_));
+    }
+    void set_e_convergence(double v) { fci_dl_set_e_convergence(d_, v); }
+    void set_r_convergence(double v) { fci_dl_set_r_convergence(d_, v); }
+    void set_maxiter(int v) { fci_dl_set_maxiter(d_, v); }
+    void set_print_level(int v) { fci_dl_set_print_level(d_, v); }
+    void reset() { check(fci_dl_reset(d_), fci_dl_last_error(d_)); }
+    bool solve() {
+        int c = 0, st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_dl_solve(d_, &c);
+        }
+        check(st, fci_dl_last_error(d_));
+        return c != 0;
+    }
+    std::vector<double> eigenvalues() const {
+        std::vector<double> e(nroot_);
+        fci_dl_eigenvalues(d_, e.data());
+        return e;
+    }
+    py::array_t<double> eigenvector(int k) const {
+        py::array_t<double> a(size_);
+        check(fci_dl_eigenvector(d_, k, a.mutable_data()), fci_dl_last_error(d_));
+        return a;
+    }
+
+  private:
+    int nroot_;
+    int64_t size_;
+    fci_dl* d_ = nullptr;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_forte_amd, m) {
+    m.doc() = "gfx950 FCI solver (Forte FCISolver drop-in)";
+    py::enum_<RDMsType>(m, "RDMsType")
+        .value("spin_dependent", RDMsType::spin_dependent)
+        .value("spin_free", RDMsType::spin_free);
+
+    py::class_<ActiveSpaceIntegrals, std::shared_ptr<ActiveSpaceIntegrals>>(m, "ActiveSpaceIntegrals")
+        .def(py::init([](int n, const arr& oa, const arr& ob, const arr& taa, const arr& tab, const arr& tbb,
+                         double scalar, double frozen, double nuclear) {
+                 auto a = std::make_shared<ActiveSpaceIntegrals>();
+                 size_t n2 = size_t(n) * n, n4 = n2 * n2;
+                 a->n = n;
+                 a->oei_a = vec_of(oa, n2, "oei_a");
+                 a->oei_b = vec_of(ob, n2, "oei_b");
+                 a->tei_aa = vec_of(taa, n4, "tei_aa");
+                 a->tei_ab = vec_of(tab, n4, "tei_ab");
+                 a->tei_bb = vec_of(tbb, n4, "tei_bb");
+                 a->scalar_energy = scalar;
+                 a->frozen_core_energy = frozen;
+                 a->nuclear_repulsion_energy = nuclear;
+                 return a;
+             }),
+             py::arg("nmo"), py::arg("oei_a"), py::arg("oei_b"), py::arg("tei_aa"), py::arg("tei_ab"),
+             py::arg("tei_bb"), py::arg("scalar_energy") = 0.0, py::arg("frozen_core_energy") = 0.0,
+             py::arg("nuclear_repulsion_energy") = 0.0)
+        .def_property_readonly("nmo", [](const ActiveSpaceIntegrals& a) { return a.n; })
+        .def("scalar_energy", [](const ActiveSpaceIntegrals& a) { return a.scalar_energy; })
+        .def("frozen_core_energy", [](const ActiveSpaceIntegrals& a) { return a.frozen_core_energy; })
+        .def("nuclear_repulsion_energy", [](const ActiveSpaceIntegrals& a) { return a.nuclear_repulsion_energy; })
+        .def("oei_a", [](const ActiveSpaceIntegrals& a) { return make_arr({a.n, a.n}, a.oei_a); })
+        .def("oei_b", [](const ActiveSpaceIntegrals& a) { return make_arr({a.n, a.n}, a.oei_b); })
+        .def("tei_aa", [](const ActiveSpaceIntegrals& a) { return make_arr({a.n, a.n, a.n, a.n}, a.tei_aa); })
+        .def("tei_ab", [](const ActiveSpaceIntegrals& a) { return make_arr({a.n, a.n, a.n, a.n}, a.tei_ab); })
+        .def("tei_bb", [](const ActiveSpaceIntegrals& a) { return make_arr({a.n, a.n, a.n, a.n}, a.tei_bb); });
+
+    py::class_<RDMs, std::shared_ptr<RDMs>>(m, "RDMs")
+        .def("max_rdm_level", [](const RDMs& r) { return r.level; })
+        .def("rdm_type", [](const RDMs& r) { return r.type; })
+        .def("g1a", [](const RDMs& r) { return r.t2(r.g1a); })
+        .def("g1b", [](const RDMs& r) { return r.t2(r.g1b); })
+        .def("g2aa", [](const RDMs& r) { return r.t4(r.g2aa); })
+        .def("g2ab", [](const RDMs& r) { return r.t4(r.g2ab); })
+        .def("g2bb", [](const RDMs& r) { return r.t4(r.g2bb); })
+        .def("SF_G1", [](const RDMs& r) { return r.t2(r.sf_g1()); })
+        .def("SF_G2", [](const RDMs& r) { return r.t4(r.sf_g2()); });
+
+    py::class_<FCISolver, std::shared_ptr<FCISolver>>(m, "FCISolver")
+        .def(py::init<int, int, int, int, std::shared_ptr<ActiveSpaceIntegrals>>(), py::arg("na"), py::arg("nb"),
+             py::arg("multiplicity"), py::arg("nroot"), py::arg("as_ints"))
+        .def("compute_energy", &FCISolver::compute_energy)
+        .def("energies", &FCISolver::energies)
+        .def("spin2", &FCISolver::spin2)
+        .def("stats", &FCISolver::stats)
+        .def("evec", &FCISolver::evec)
+        .def("ndet", &FCISolver::ndet)
+        .def("nstr", &FCISolver::nstr)
+        .def("rdms", &FCISolver::rdms, py::arg("root_list"), py::arg("max_rdm_level"),
+             py::arg("rdm_type") = RDMsType::spin_dependent)
+        .def("sigma", &FCISolver::sigma)
+        .def("sigma_device", &FCISolver::sigma_device, py::arg("c_ptr"), py::arg("sigma_ptr"), py::arg("dtype"),
+             py::arg("stream") = 0)
+        .def("sigma_timings", &FCISolver::sigma_timings)
+        .def("sigma_bench", &FCISolver::sigma_bench)
+        .def("sigma_arrays", &FCISolver::sigma_arrays)
+        .def("spin2_of", &FCISolver::spin2_of)
+        .def("hdiag", &FCISolver::hdiag)
+        .def("strings", &FCISolver::strings)
+        .def("vo_list", &FCISolver::vo_list)
+        .def("oo_list", &FCISolver::oo_list)
+        .def("vvoo_list", &FCISolver::vvoo_list)
+        .def("set_options", &FCISolver::set_options)
+        .def("set_active_space_integrals", &FCISolver::set_active_space_integrals)
+        .def("set_e_convergence", &FCISolver::set_e_convergence)
+        .def("set_r_convergence", &FCISolver::set_r_convergence)
+        .def("set_maxiter", &FCISolver::set_maxiter)
+        .def("set_die_if_not_converged", &FCISolver::set_die_if_not_converged)
+        .def("set_root", &FCISolver::set_root)
+        .def("set_print", &FCISolver::set_print)
+        .def("set_ndets_per_guess_state", &FCISolver::set_ndets_per_guess_state)
+        .def("set_guess_per_root", &FCISolver::set_guess_per_root)
+        .def("set_collapse_per_root", &FCISolver::set_collapse_per_root)
+        .def("set_subspace_per_root", &FCISolver::set_subspace_per_root)
+        .def("set_precision", &FCISolver::set_precision)
+        .def("set_mixed_switch", &FCISolver::set_mixed_switch);
+
+    py::class_<DeviceArray, std::shared_ptr<DeviceArray>>(m, "DeviceArray")
+        .def(py::init<int64_t, const std::string&>(), py::arg("n"), py::arg("dtype") = "float64")
+        .def("upload", &DeviceArray::upload)
+        .def("download", &DeviceArray::download)
+        .def("ptr", &DeviceArray::ptr)
+        .def("size", &DeviceArray::size);
+
+    py::class_<DavidsonLiuSolver, std::shared_ptr<DavidsonLiuSolver>>(m, "DavidsonLiuSolver")
+        .def(py::init<int64_t, int, int, int>(), py::arg("size"), py::arg("nroots"), py::arg("collapse_per_root") = 1,
+             py::arg("subspace_per_root") = 5)
+        .def("add_h_diag", &DavidsonLiuSolver::add_h_diag)
+        .def("add_guesses", &DavidsonLiuSolver::add_guesses)
+        .def("add_project_out_vectors", &DavidsonLiuSolver::add_project_out_vectors)
+        .def("add_test_sigma_builder", &DavidsonLiuSolver::add_test_sigma_builder)
+        .def("set_e_convergence", &DavidsonLiuSolver::set_e_convergence)
+        .def("set_r_convergence", &DavidsonLiuSolver::set_r_convergence)
+        .def("set_maxiter", &DavidsonLiuSolver::set_maxiter)
+        .def("set_print_level", &DavidsonLiuSolver::set_print_level)
+        .def("reset", &DavidsonLiuSolver::reset)
+        .def("solve", &DavidsonLiuSolver::solve)
+        .def("eigenvalues", &DavidsonLiuSolver::eigenvalues)
+        .def("eigenvector", &DavidsonLiuSolver::eigenvector);
+
+    m.def("seeded_integrals", [](int n, uint64_t seed) {
+        std::vector<double> h(size_t(n) * n), eri(size_t(n) * n * n * n);
+        fci_seeded_integrals(n, seed, h.data(), eri.data());
+        return py::make_tuple(make_arr({n, n}, h), make_arr({n, n, n, n}, eri));
+    });
+}

@@ -1,0 +1,175 @@
+/*
+ * forte_fci.h -- C ABI of the gfx950 FCI solver (libforte_fci_gfx950.so).
+ *
+ * The drop-in boundary is Forte's ActiveSpaceMethod / FCISolver interface
+ * (forte/base_classes/active_space_method.h:95-388, forte/fci/fci_solver.h:51-246). Forte has no
+ * C ABI of its own; these entry points are what a binding of that interface needs (a ctypes /
+ * pybind11 / cgo stub calls them with plain pointers). Every function returns 0 on success and
+ * a negative status on failure; fci_last_error() then holds the message. No C++ exception
+ * crosses this boundary. Output buffers are owned by the caller.
+ *
+ * Scope: one irrep (C1), all active orbitals, n <= 24, any na / nb. Integrals use Forte's
+ * layout (forte/integrals/active_space_integrals.h:103-127): oei n*n row-major, tei n^4
+ * row-major physicist, tei_aa / tei_bb antisymmetrized <pq||rs>, tei_ab = <pq|rs>.
+ * Determinants are ordered alpha-major, C[Ia*Nb + Ib] (forte/fci/fci_vector.cc:108-114).
+ */
+#ifndef FORTE_FCI_H
+#define FORTE_FCI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fci_solver fci_solver;
+typedef struct fci_dl fci_dl;
+
+enum fci_status {
+    FCI_OK = 0,
+    FCI_ERR_ARG = -1,     /* invalid argument */
+    FCI_ERR_STATE = -2,   /* call out of order (e.g. rdms before compute_energy) */
+    FCI_ERR_DEVICE = -3,  /* HIP runtime error or no gfx950 device */
+    FCI_ERR_CONVERGE = -4,/* Davidson-Liu did not converge and die_if_not_converged is set */
+    FCI_ERR_NUMERIC = -5  /* Davidson-Liu numerical failure (linear dependence, ...) */
+};
+
+enum fci_precision {
+    FCI_PREC_FP64 = 0,  /* every sigma in FP64 (Forte's arithmetic) */
+    FCI_PREC_MIXED = 1  /* FP32 C/sigma storage, BF16x3 MFMA contractions accumulated in FP32,
+                           FP64 Davidson algebra, FP64 refinement of the converged vector */
+};
+
+enum fci_preconditioner {
+    FCI_PRECOND_DIAGONAL = 0, /* r / (lambda - H_II), davidson_liu_solver.cc:432-448 */
+    FCI_PRECOND_LEARNED = 1   /* MLP correction with diagonal fallback */
+};
+
+/* Options of FCISolver::set_options (fci_solver.cc:156-172) and of the ActiveSpaceMethod base
+ * class (active_space_method.h:285-320). Defaults: fci_default_options(). */
+typedef struct fci_options {
+    int nroot;               /* ActiveSpaceMethod nroot_ */
+    int root;                /* ROOT */
+    int multiplicity;        /* StateInfo::multiplicity */
+    int guess_per_root;      /* DL_GUESS_PER_ROOT */
+    int ndets_per_guess;     /* DL_DETS_PER_GUESS */
+    int collapse_per_root;   /* DL_COLLAPSE_PER_ROOT */
+    int subspace_per_root;   /* DL_SUBSPACE_PER_ROOT */
+    int maxiter;             /* DL_MAXITER */
+    double e_convergence;    /* E_CONVERGENCE */
+    double r_convergence;    /* R_CONVERGENCE */
+    int die_if_not_converged;
+    int print;               /* PrintLevel: 0 quiet .. 4 debug */
+    int precision;           /* enum fci_precision */
+    int preconditioner;      /* enum fci_preconditioner */
+    double mixed_switch;     /* mixed precision: residual norm at which FP64 refinement starts */
+} fci_options;
+
+void fci_default_options(fci_options* opts);
+
+/* FCISolver(StateInfo, nroot, MOSpaceInfo, ActiveSpaceIntegrals), fci_solver.h:63-64 and
+ * fci_solver.cc:55-65. Builds the string tables and excitation lists on the device
+ * (FCIStringLists::startup, fci_string_lists.cc:52-205). */
+int fci_create(int n, int na, int nb, const double* oei_a, const double* oei_b,
+               const double* tei_aa, const double* tei_ab, const double* tei_bb, double e_core,
+               fci_solver** out);
+void fci_destroy(fci_solver* s);
+const char* fci_last_error(const fci_solver* s); /* also valid for s == NULL (global error) */
+
+/* FCISolver::set_options, fci_solver.cc:156-172 */
+int fci_set_options(fci_solver* s, const fci_options* opts);
+/* ActiveSpaceMethod::set_active_space_integrals, active_space_method.h:255 (keeps the
+ * Davidson-Liu basis for a warm restart, fci_solver.cc:204-237) */
+int fci_set_integrals(fci_solver* s, const double* oei_a, const double* oei_b,
+                      const double* tei_aa, const double* tei_ab, const double* tei_bb,
+                      double e_core);
+
+/* FCISolver::compute_energy, fci_solver.cc:177-314. *energy = energy of `root`. */
+int fci_compute_energy(fci_solver* s, double* energy);
+/* ActiveSpaceMethod::energies() / spin2() (active_space_method.h:261-264); nroot values each */
+int fci_energies(const fci_solver* s, double* energies, double* spin2);
+/* Statistics of the last solve: iterations, sigma builds, FP64 sigma builds, seconds,
+ * converged flag, learned-preconditioner fallback flag. */
+int fci_solve_stats(const fci_solver* s, int* iterations, int* nsigma, int* nsigma_fp64,
+                    double* seconds, int* converged, int* fell_back);
+/* FCISolver::evecs row `root` (fci_solver.cc:79-96), ndet doubles, host buffer */
+int fci_evec(const fci_solver* s, int root, double* out);
+int64_t fci_ndet(const fci_solver* s);
+int64_t fci_nstr(const fci_solver* s, int spin);
+
+/* FCISolver::rdms -> FCIVector::compute_rdms (fci_solver_rdms.cc:40-104,
+ * fci_vector_rdm.cc:43-393). level 1: g1a,g1b (n^2); level 2 adds g2aa,g2ab,g2bb (n^4), with
+ * <a+p a+q a_s a_r> stored at [p,q,r,s]. Pass NULL for arrays not wanted. */
+int fci_rdms(fci_solver* s, int root_left, int root_right, int level, double* g1a, double* g1b,
+             double* g2aa, double* g2ab, double* g2bb);
+/* FCIVector::compute_spin2 (fci_vector_spin2.cc:38-75) of a host vector */
+int fci_spin2_of(fci_solver* s, const double* C, double* spin2);
+
+/* FCIVector::Hamiltonian (fci_vector_hamiltonian.cc:46-82): sigma = H C.
+ * Host buffers, FP64. */
+int fci_sigma(fci_solver* s, const double* C, double* sigma);
+/* Same on device buffers already resident in HBM (no copies). dtype 0 = FP64, 1 = FP32
+ * (mixed-precision kernels). stream may be NULL (default stream). */
+int fci_sigma_device(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, void* stream);
+/* Per-kernel device time (ms) of the last fci_sigma_device call measured with HIP events:
+ * [0] same-spin beta (incl. transposes), [1] same-spin alpha, [2] alpha-beta. */
+int fci_sigma_timings(const fci_solver* s, double* ms3);
+/* FCISolver::form_Hdiag_det (fci_solver_initial_guess.cc:151-172), host buffer of ndet */
+int fci_hdiag(fci_solver* s, double* out);
+
+/* String tables, built on the device, for parity checks against FCIStringLists. */
+/* make_fci_strings (fci_string_lists.cc:258-293): nstr bit patterns in address order */
+int fci_strings(fci_solver* s, int spin, uint64_t* out);
+/* get_alfa_vo_list / get_beta_vo_list (fci_string_vo_list.cc:47-66): count, then entries */
+int64_t fci_vo_size(fci_solver* s, int spin, int p, int q);
+int fci_vo_list(fci_solver* s, int spin, int p, int q, double* sign, uint32_t* I, uint32_t* J);
+/* get_*_oo_list (fci_string_oo_list.cc:40-57) for pair index pq (make_pair_list order) */
+int64_t fci_oo_size(fci_solver* s, int spin, int pq);
+int fci_oo_list(fci_solver* s, int spin, int pq, double* sign, uint32_t* I, uint32_t* J);
+/* get_*_vvoo_list (fci_string_vvoo_list.cc:37-54) for p>q, r>s, (p,q) != (r,s) */
+int64_t fci_vvoo_size(fci_solver* s, int spin, int p, int q, int r, int t);
+int fci_vvoo_list(fci_solver* s, int spin, int p, int q, int r, int t, double* sign,
+                  uint32_t* I, uint32_t* J);
+
+/* ---- Davidson-Liu solver on its own (forte/helpers/davidson_liu_solver.h:52-240 and the
+ * test bindings of forte/api/dl_solver_api.cc:45-86). Vectors live on the device. ---------- */
+int fci_dl_create(int64_t size, int nroot, int collapse_per_root, int subspace_per_root,
+                  fci_dl** out);
+void fci_dl_destroy(fci_dl* d);
+const char* fci_dl_last_error(const fci_dl* d);
+int fci_dl_add_h_diag(fci_dl* d, const double* h_diag, int64_t size);
+/* sparse vectors: n vectors, entries [off[i], off[i+1]) of (idx, val) */
+int fci_dl_add_guesses(fci_dl* d, int n, const int64_t* off, const int64_t* idx, const double* val);
+int fci_dl_add_project_out_vectors(fci_dl* d, int n, const int64_t* off, const int64_t* idx,
+                                   const double* val);
+/* add_test_sigma_builder(M): dense size x size row-major matrix, applied on the device */
+int fci_dl_set_dense_sigma(fci_dl* d, const double* M);
+int fci_dl_set_e_convergence(fci_dl* d, double v);
+int fci_dl_set_r_convergence(fci_dl* d, double v);
+int fci_dl_set_maxiter(fci_dl* d, int v);
+int fci_dl_set_print_level(fci_dl* d, int v);
+int fci_dl_reset(fci_dl* d);
+int fci_dl_solve(fci_dl* d, int* converged);
+int fci_dl_eigenvalues(const fci_dl* d, double* out); /* nroot values */
+int fci_dl_eigenvector(const fci_dl* d, int k, double* out);
+
+/* ---- device buffers (for callers without their own device allocator) --------------------- */
+int fci_device_alloc(size_t bytes, void** out);
+int fci_device_free(void* p);
+int fci_device_upload(void* dst_dev, const void* src_host, size_t bytes);
+int fci_device_download(void* dst_host, const void* src_dev, size_t bytes);
+int fci_device_synchronize(void);
+/* Time `nrep` sigma builds on device buffers with HIP events on one stream: ms_per_sigma is the
+ * mean wall time per build; ms3 the mean per-kernel-group device time (as fci_sigma_timings). */
+int fci_sigma_bench(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, int nrep,
+                    double* ms_per_sigma, double* ms3);
+
+/* ---- seeded integrals of the benchmark configurations (host) ----------------------------- */
+/* h (n*n) and chemist ERIs (pq|rs) (n^4) from a SplitMix64 stream; see fci_solver.hip. */
+void fci_seeded_integrals(int n, uint64_t seed, double* h, double* eri);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,77 @@
+"""The C ABI library loads and exports every entry point include/forte_fci.h declares; the host
+layer fails loudly (RuntimeError) without a GPU. No compute calls run here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "forte_fci.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*(fci_[a-z0-9_]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["fci_create", "fci_compute_energy", "fci_rdms", "fci_sigma_device", "fci_destroy",
+                 "fci_last_error", "fci_dl_create", "fci_dl_solve", "fci_vo_list", "fci_vvoo_list"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import forte_amd
+    lib = ctypes.CDLL(forte_amd.LIBRARY_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_default_options_without_gpu():
+    import forte_amd
+    lib = ctypes.CDLL(forte_amd.LIBRARY_PATH)
+
+    class Opts(ctypes.Structure):
+        _fields_ = [("nroot", ctypes.c_int), ("root", ctypes.c_int), ("multiplicity", ctypes.c_int),
+                    ("guess_per_root", ctypes.c_int), ("ndets_per_guess", ctypes.c_int),
+                    ("collapse_per_root", ctypes.c_int), ("subspace_per_root", ctypes.c_int),
+                    ("maxiter", ctypes.c_int), ("e_convergence", ctypes.c_double),
+                    ("r_convergence", ctypes.c_double), ("die_if_not_converged", ctypes.c_int),
+                    ("print", ctypes.c_int), ("precision", ctypes.c_int), ("preconditioner", ctypes.c_int),
+                    ("mixed_switch", ctypes.c_double)]
+    o = Opts()
+    lib.fci_default_options(ctypes.byref(o))
+    # forte/options.yaml DL_* defaults and E/R convergence of ActiveSpaceSolver
+    assert (o.nroot, o.guess_per_root, o.ndets_per_guess, o.collapse_per_root, o.subspace_per_root,
+            o.maxiter) == (1, 1, 50, 2, 10, 100)
+    assert o.e_convergence == 1e-10 and o.r_convergence == 1e-6
+
+
+def test_seeded_integrals_match_oracle(oracle):
+    import numpy as np
+    import forte_amd
+    for n, seed in [(4, 1), (8, 1), (14, 2)]:
+        h, eri = forte_amd.seeded_integrals(n, seed)
+        h2, eri2 = oracle.seeded_integrals(n, seed)
+        assert np.array_equal(h, h2) and np.array_equal(eri, eri2)
+
+
+def test_fails_loudly_without_gpu():
+    import forte_amd
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    h, eri = forte_amd.seeded_integrals(4, 1)
+    ints = forte_amd.make_active_space_integrals(h, eri)
+    with pytest.raises(RuntimeError, match="HIP device|gfx950"):
+        forte_amd.FCISolver(2, 2, 1, 1, ints)
+    with pytest.raises(RuntimeError):
+        forte_amd.DavidsonLiuSolver(4, 1)

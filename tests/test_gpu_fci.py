@@ -1,0 +1,223 @@
+"""GPU parity: the gfx950 solver against the CPU oracle and the reference's known answers."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import fcidump_path, seeded_problem
+
+pytestmark = pytest.mark.gpu
+
+KAT = {2: -1.163012998021681, 3: -1.926739016209154, 4: -1.252893688642302, 5: -1.134690658798176}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_solver(p, nroot=1, mult=1):
+    import forte_amd
+    ints = forte_amd.ActiveSpaceIntegrals(p["n"], p["h"], p["h"], p["aa"], p["ab"], p["aa"], p["e_core"])
+    return forte_amd.FCISolver(p["na"], p["nb"], mult, nroot, ints)
+
+
+def cpu_oracle(O, p):
+    return O.Oracle(p["n"], p["na"], p["nb"], p["h"], p["h"], p["aa"], p["ab"], p["aa"], p["e_core"])
+
+
+TABLE_CASES = [(4, 2, 2), (5, 3, 2), (6, 3, 3), (6, 1, 5), (7, 3, 4), (8, 4, 4), (8, 2, 6), (3, 3, 0)]
+
+
+@pytest.mark.parametrize("n,na,nb", TABLE_CASES)
+def test_tables_bit_exact(oracle, n, na, nb):
+    say("tables", n, na, nb)
+    p = seeded_problem(n, na, nb, 3)
+    g, o = gpu_solver(p), cpu_oracle(oracle, p)
+    for spin in (0, 1):
+        assert np.array_equal(g.strings(spin), o.strings(spin))
+        for a, b in itertools.product(range(n), repeat=2):
+            for x, y in zip(g.vo_list(spin, a, b), o.vo(spin, a, b)):
+                assert np.array_equal(x, y), ("vo", spin, a, b)
+        for pq in range(n * (n - 1) // 2):
+            for x, y in zip(g.oo_list(spin, pq), o.oo(spin, pq)):
+                assert np.array_equal(x, y), ("oo", spin, pq)
+        for a, b, c, d in itertools.product(range(n), repeat=4):
+            if a > b and c > d and not (a == c and b == d):
+                for x, y in zip(g.vvoo_list(spin, a, b, c, d), o.vvoo(spin, a, b, c, d)):
+                    assert np.array_equal(x, y), ("vvoo", spin, a, b, c, d)
+
+
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 4, 4), (10, 5, 4), (12, 6, 6)])
+def test_hdiag_bit_exact(oracle, n, na, nb):
+    say("hdiag", n, na, nb)
+    p = seeded_problem(n, na, nb, 4, e_core=0.7)
+    assert np.array_equal(gpu_solver(p).hdiag(), cpu_oracle(oracle, p).hdiag())
+
+
+SIGMA_CASES = [(2, 1, 1), (3, 2, 1), (4, 2, 2), (4, 3, 2), (5, 1, 1), (6, 3, 3), (6, 2, 4), (6, 5, 1),
+               (4, 4, 0), (4, 0, 4), (6, 6, 6), (7, 1, 6), (8, 4, 4), (8, 3, 5), (9, 2, 2), (10, 5, 5),
+               (10, 2, 8), (12, 6, 6), (12, 4, 7)]
+
+
+@pytest.mark.parametrize("n,na,nb", SIGMA_CASES)
+def test_sigma_fp64(oracle, n, na, nb):
+    say("sigma fp64", n, na, nb)
+    p = seeded_problem(n, na, nb, 7 + n, e_core=0.31)
+    g, o = gpu_solver(p), cpu_oracle(oracle, p)
+    C = np.random.default_rng(n * 100 + na).standard_normal(o.ndet)
+    ref = o.sigma(C)
+    got = g.sigma(C)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.abs(got - ref).max() < 1e-12 * scale
+
+
+@pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 4, 4), (10, 5, 5), (12, 6, 6), (9, 3, 5)])
+def test_sigma_mixed(oracle, n, na, nb):
+    import forte_amd
+    say("sigma mixed", n, na, nb)
+    p = seeded_problem(n, na, nb, 17 + n)
+    g, o = gpu_solver(p), cpu_oracle(oracle, p)
+    C = np.random.default_rng(n).standard_normal(o.ndet)
+    C /= np.linalg.norm(C)
+    ref = o.sigma(C)
+    c32 = forte_amd.DeviceArray(o.ndet, "float32")
+    s32 = forte_amd.DeviceArray(o.ndet, "float32")
+    c32.upload(C.astype(np.float32))
+    g.sigma_arrays(c32, s32)
+    got = s32.download().astype(np.float64)
+    # BF16x3 products with FP32 accumulation: relative error ~1e-5 of the vector norm
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    say("  relative error", err)
+    assert err < 5e-5
+
+
+def test_sigma_device_fp64_matches_host(oracle):
+    import forte_amd
+    p = seeded_problem(8, 4, 4, 5)
+    g = gpu_solver(p)
+    C = np.random.default_rng(0).standard_normal(g.ndet())
+    c = forte_amd.DeviceArray(g.ndet(), "float64")
+    s = forte_amd.DeviceArray(g.ndet(), "float64")
+    c.upload(C)
+    g.sigma_arrays(c, s)
+    ref = g.sigma(C)
+    assert np.abs(s.download() - ref).max() < 1e-12 * np.abs(ref).max()
+    t = g.sigma_timings()
+    assert len(t) == 3 and all(x >= 0 for x in t)
+    ms, t3 = g.sigma_bench(c, s, 3)
+    assert ms > 0 and len(t3) == 3
+    assert np.abs(s.download() - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("k", sorted(KAT))
+def test_kat_energy_gpu(k):
+    import forte_amd
+    say("kat", k)
+    ints, na, nb, mult = forte_amd.integrals_from_fcidump(fcidump_path(k))
+    s = forte_amd.FCISolver(na, nb, mult, 1, ints)
+    s.set_e_convergence(1e-12)
+    e = s.compute_energy()
+    assert abs(e - KAT[k]) < 1e-9
+    assert abs(s.spin2()[0] - (mult * mult - 1) / 4.0) < 1e-8
+
+
+@pytest.mark.parametrize("n,na,nb,nroot,mult", [(6, 3, 3, 3, 1), (8, 4, 4, 1, 1), (7, 4, 3, 2, 2),
+                                                (8, 4, 4, 2, 3), (10, 5, 5, 1, 1)])
+def test_energies_vs_oracle(oracle, n, na, nb, nroot, mult):
+    say("energies", n, na, nb, nroot, mult)
+    p = seeded_problem(n, na, nb, 1 if n == 8 else 50 + n)
+    g, o = gpu_solver(p, nroot, mult), cpu_oracle(oracle, p)
+    g.set_e_convergence(1e-10)
+    eg = g.compute_energy()
+    r = o.solve(nroot=nroot, multiplicity=mult, e_conv=1e-10)
+    assert np.abs(np.array(g.energies()) - r["energies"]).max() < 1e-9
+    assert np.abs(np.array(g.spin2()) - r["spin2"]).max() < 1e-7
+    assert eg == g.energies()[0]
+
+
+def test_rdms_vs_oracle(oracle):
+    import forte_amd
+    say("rdms")
+    p = seeded_problem(8, 4, 4, 1)
+    g, o = gpu_solver(p, 2), cpu_oracle(oracle, p)
+    g.set_e_convergence(1e-12)
+    g.set_r_convergence(1e-8)
+    g.compute_energy()
+    for (l, r) in [(0, 0), (1, 1), (0, 1)]:
+        rd = g.rdms([(l, r)], 2)[0]
+        cl, cr = g.evec(l), g.evec(r)
+        ref = o.rdms(cl, cr)
+        for a, b in zip([rd.g1a(), rd.g1b(), rd.g2aa(), rd.g2ab(), rd.g2bb()], ref):
+            assert np.abs(a - b).max() < 1e-12, (l, r)
+    # spin-free combination (fci_vector_rdm.cc:121-126) and traces
+    rd = g.rdms([(0, 0)], 2, forte_amd.RDMsType.spin_free)[0]
+    G1, G2 = rd.SF_G1(), rd.SF_G2()
+    assert abs(np.trace(G1) - 8) < 1e-10
+    assert abs(np.einsum("pqpq", G2) - 8 * 7) < 1e-9
+    # energy from RDMs equals the eigenvalue
+    r0 = g.rdms([(0, 0)], 2)[0]
+    e = np.einsum("pq,pq", p["h"], r0.g1a() + r0.g1b()) + 0.25 * np.einsum("pqrs,pqrs", p["aa"], r0.g2aa() + r0.g2bb()) \
+        + np.einsum("pqrs,pqrs", p["ab"], r0.g2ab())
+    assert abs(e - g.energies()[0]) < 1e-9
+
+
+def test_rdms_against_oracle_eigenvector(oracle):
+    """Root 0 RDMs of the GPU solve agree with the oracle's own solve (sign-invariant)."""
+    p = seeded_problem(8, 4, 4, 1)
+    g, o = gpu_solver(p), cpu_oracle(oracle, p)
+    g.set_e_convergence(1e-12)
+    g.set_r_convergence(1e-8)
+    g.compute_energy()
+    r = o.solve(nroot=1, e_conv=1e-12, r_conv=1e-8)
+    rd = g.rdms([(0, 0)], 2)[0]
+    ref = o.rdms(r["evecs"][0])
+    for a, b in zip([rd.g1a(), rd.g1b(), rd.g2aa(), rd.g2ab(), rd.g2bb()], ref):
+        assert np.abs(a - b).max() < 1e-8
+
+
+def test_warm_restart_and_new_integrals(oracle):
+    import forte_amd
+    p = seeded_problem(6, 3, 3, 61)
+    g = gpu_solver(p)
+    e1 = g.compute_energy()
+    e2 = g.compute_energy()
+    assert abs(e1 - e2) < 1e-9
+    q = seeded_problem(6, 3, 3, 62)
+    g.set_active_space_integrals(forte_amd.ActiveSpaceIntegrals(6, q["h"], q["h"], q["aa"], q["ab"], q["aa"]))
+    e3 = g.compute_energy()
+    r = cpu_oracle(oracle, q).solve(nroot=1)
+    assert abs(e3 - r["energies"][0]) < 1e-9
+
+
+def test_mixed_precision_solve(oracle):
+    say("mixed solve")
+    p = seeded_problem(10, 5, 5, 9)
+    g64, gmx = gpu_solver(p), gpu_solver(p)
+    g64.set_e_convergence(1e-10)
+    gmx.set_e_convergence(1e-10)
+    gmx.set_precision("mixed")
+    e64 = g64.compute_energy()
+    emx = gmx.compute_energy()
+    st = gmx.stats()
+    say("  fp64", e64, g64.stats(), "mixed", emx, st)
+    assert abs(e64 - emx) < 1e-9
+    assert st["sigma_builds_fp64"] < st["sigma_builds"]
+
+
+def test_errors_are_runtime_errors():
+    import forte_amd
+    p = seeded_problem(4, 2, 2, 1)
+    g = gpu_solver(p)
+    with pytest.raises(RuntimeError):
+        g.rdms([(0, 0)], 2)  # before compute_energy
+    with pytest.raises(RuntimeError):
+        g.set_precision("fp16")
+    g.compute_energy()
+    with pytest.raises(RuntimeError):
+        g.rdms([(0, 5)], 1)
+    with pytest.raises(RuntimeError):
+        g.rdms([(0, 0)], 3)
+    with pytest.raises(RuntimeError):
+        forte_amd.FCISolver(5, 2, 1, 1, forte_amd.ActiveSpaceIntegrals(4, p["h"], p["h"], p["aa"], p["ab"], p["aa"]))
