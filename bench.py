@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""CAS(16,16) FCI on one MI355X: sigma throughput and time to solution.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--solve 0|1] [--cpu 0|1] [--n 16]
+
+A step is one sigma build H C over all 165,636,900 determinants of CAS(16,16) (Ms = 0, C1,
+seeded integrals, seed 3) with the mixed-precision kernels (FP32 C and sigma, BF16x3 MFMA
+contractions accumulated in FP32). `value` is the sigma throughput in GB/s, defined as
+9 * N_det * 4 bytes / t_sigma (C read by the two same-spin and the alpha-beta passes, sigma
+written and re-read), inputs resident in HBM. The line also carries:
+  roofline      MFMA roofline of the dominant kernel (the alpha-beta contraction), timed with
+                HIP events on its own stream; algorithmic flops = 3 * 2 * N_det * P * E with
+                P = n(n+1)/2 packed beta pairs and E = k(n-k)+1 alpha excitations (BF16x3 = 3
+                BF16 products), against the 2.5 PFLOP/s dense BF16 peak;
+  fp64_sigma    the FP64 sigma of the plain solver on the same vector;
+  time_to_solution  full Davidson-Liu solves (FP64 vs mixed + FP64 refinement) to the same
+                    thresholds, and |E_mixed - E_fp64|;
+  cpu_baseline  the CPU restatement of Forte's sigma (oracle/fci_oracle.c, one core) on a
+                strided sample of its loops, extrapolated to one full sigma.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def traffic_from_profiles(n):
+    """HBM bytes per launch of the alpha-beta kernel from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", f"k_ab_mixed_cas{n}_{n}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(n, h, eri, sample_stride):
+    """Forte-structured CPU sigma (oracle, FP64, 1 thread) on a strided sample, extrapolated."""
+    import numpy as np
+    from oracle import oracle as O
+
+    t0 = time.time()
+    o = O.Oracle.from_chem(h, eri, n // 2, n // 2)
+    setup = time.time() - t0
+    C = np.random.default_rng(7).standard_normal(o.ndet)
+    C /= np.linalg.norm(C)
+    t, w = o.sigma_sample(C, sample_stride)
+    est = float(sum(ti / wi for ti, wi in zip(t, w) if wi > 0))
+    return dict(value=round(9 * o.ndet * 8 / est / 1e9, 4), unit="GB/s", cores=1, kind="port",
+                seconds_per_sigma_extrapolated=round(est, 2), sampled_seconds=round(float(t.sum()), 2),
+                list_setup_seconds=round(setup, 2),
+                sample=(f"FP64 Forte-structured sigma (oracle/fci_oracle.c) of the same CAS({n},{n}) "
+                        f"vector: every term's outer loop (H1: pq, H2_aaaa: pair pq, H2_aabb: rs) run "
+                        f"on indices i % {sample_stride} == 0, each term extrapolated by the fraction "
+                        f"of its list entries covered"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=16)
+    ap.add_argument("--solve", type=int, default=1)
+    ap.add_argument("--cpu", type=int, default=1)
+    ap.add_argument("--cpu-stride", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this FCI solver runs on one GPU; multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the sigma build would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import forte_amd
+
+    n = args.n
+    na = nb = n // 2
+    h, eri = forte_amd.seeded_integrals(n, 3)
+    ints = forte_amd.make_active_space_integrals(h, eri)
+    t0 = time.time()
+    solver = forte_amd.FCISolver(na, nb, 1, 1, ints)
+    setup_s = time.time() - t0
+    nd = solver.ndet()
+    x = np.random.default_rng(0).standard_normal(nd)
+    x /= np.linalg.norm(x)
+
+    # ---- mixed-precision sigma (the timed steps) ----------------------------------------------
+    C = forte_amd.DeviceArray(nd, "float32")
+    S = forte_amd.DeviceArray(nd, "float32")
+    C.upload(x.astype(np.float32))
+    print(f"# CAS({n},{n}) ndet={nd} setup {setup_s:.2f}s; warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        solver.sigma_bench(C, S, args.warmup)
+    forte_amd.device_synchronize()
+    t0 = time.perf_counter()
+    ms_event, t3 = solver.sigma_bench(C, S, args.steps)
+    forte_amd.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = 9 * nd * 4 / (ms_per_step * 1e-3) / 1e9
+    print(f"# mixed sigma {ms_per_step:.2f} ms/step (events {ms_event:.2f}); beta-same {t3[0]:.2f} "
+          f"alpha-same {t3[1]:.2f} alpha-beta {t3[2]:.2f} ms", file=sys.stderr, flush=True)
+    np_ = n * (n + 1) // 2
+    E = na * (n - na) + 1
+    ab_flops = 3 * 2 * nd * np_ * E
+    achieved = ab_flops / (t3[2] * 1e-3) / 1e12
+    del C, S
+
+    # ---- FP64 sigma on the same vector -----------------------------------------------------------
+    C64 = forte_amd.DeviceArray(nd, "float64")
+    S64 = forte_amd.DeviceArray(nd, "float64")
+    C64.upload(x)
+    solver.sigma_bench(C64, S64, 1)
+    ms64, t3_64 = solver.sigma_bench(C64, S64, max(1, args.steps // 4))
+    print(f"# fp64 sigma {ms64:.2f} ms; beta-same {t3_64[0]:.2f} alpha-same {t3_64[1]:.2f} "
+          f"alpha-beta {t3_64[2]:.2f} ms", file=sys.stderr, flush=True)
+    del C64, S64
+
+    result = {
+        "metric": "CAS(16,16) FCI time to |dE|<1e-8 Eh: mixed+learned vs FP64 plain; sigma GB/s",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+bf16x3",
+        "data": "synthetic (seeded integrals, seed 3; random normalized C)",
+        "config": {"workload": f"FCI sigma CAS({n},{n}) Ms=0 C1", "ndet": nd, "n_orbitals": n,
+                   "n_alpha": na, "n_beta": nb, "precision": "mixed"},
+        "roofline": {"bound": "mfma", "kernel": "k_ab_mixed (alpha-beta BF16x3 contraction)",
+                     "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(n),
+                     "kernel_ms": round(t3[2], 3), "algorithmic_flops": ab_flops},
+        "sigma_split_ms": {"beta_same_spin_with_transposes": round(t3[0], 3),
+                           "alpha_same_spin": round(t3[1], 3), "alpha_beta": round(t3[2], 3)},
+        "fp64_sigma": {"ms": round(ms64, 3), "GBps": round(9 * nd * 8 / (ms64 * 1e-3) / 1e9, 3),
+                       "split_ms": [round(t, 3) for t in t3_64]},
+    }
+
+    # ---- time to solution ------------------------------------------------------------------------
+    if args.solve:
+        tts = {}
+        energies = {}
+        for prec in ("fp64", "mixed"):
+            s = forte_amd.FCISolver(na, nb, 1, 1, ints)
+            s.set_e_convergence(1e-9)
+            s.set_r_convergence(1e-4)
+            s.set_precision(prec)
+            forte_amd.device_synchronize()
+            t0 = time.perf_counter()
+            e = s.compute_energy()
+            forte_amd.device_synchronize()
+            tts[prec] = time.perf_counter() - t0
+            energies[prec] = e
+            st = s.stats()
+            print(f"# solve {prec}: E={e:.12f} {tts[prec]:.2f}s {st}", file=sys.stderr, flush=True)
+            tts[prec + "_stats"] = {k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64")}
+            del s
+        result["time_to_solution"] = {
+            "fp64_s": round(tts["fp64"], 3), "mixed_s": round(tts["mixed"], 3),
+            "speedup": round(tts["fp64"] / tts["mixed"], 3),
+            "abs_dE_mixed_vs_fp64": abs(energies["mixed"] - energies["fp64"]),
+            "energy_fp64": energies["fp64"], "thresholds": {"e_conv": 1e-9, "r_conv": 1e-4},
+            "fp64_stats": tts["fp64_stats"], "mixed_stats": tts["mixed_stats"],
+            "preconditioner": "diagonal"}
+
+    if args.cpu:
+        result["cpu_baseline"] = cpu_baseline(n, h, eri, args.cpu_stride)
+        print(f"# cpu baseline {result['cpu_baseline']}", file=sys.stderr, flush=True)
+    else:
+        result["cpu_baseline"] = None
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -25,6 +25,7 @@ RDMs = _ext.RDMs
 RDMsType = _ext.RDMsType
 seeded_integrals = _ext.seeded_integrals
 DeviceArray = _ext.DeviceArray
+device_synchronize = _ext.device_synchronize
 
 LIBRARY_PATH = os.path.join(_HERE, "_lib", "libforte_fci_gfx950.so")
 
@@ -64,6 +65,6 @@ def make_active_space_method(method, na, nb, multiplicity, nroot, as_ints, optio
 
 __all__ = [
     "ActiveSpaceIntegrals", "FCISolver", "DavidsonLiuSolver", "RDMs", "RDMsType",
-    "seeded_integrals", "DeviceArray", "make_active_space_integrals", "integrals_from_fcidump",
+    "seeded_integrals", "DeviceArray", "device_synchronize", "make_active_space_integrals", "integrals_from_fcidump",
     "make_active_space_method", "read_fcidump", "chem_to_forte", "LIBRARY_PATH",
 ]

@@ -244,6 +244,169 @@ __global__ __launch_bounds__(256) void k_clique_mixed(const float* __restrict__ 
         }
 }
 
+// Column-tile-major variant: the whole operator g' lives in LDS, each wave owns one hole string
+// K at a time, and the XCD-contiguous work order keeps one column tile of X and Y resident in
+// that XCD's L2 while every K of the tile passes through it. The MFMA A operand is read from
+// g' through the member pair indices, so no per-K copy of W_K is made.
+constexpr int CQ2_KCH = 64;    // hole strings per block
+
+template <int TW, int CQ2_WAVES>
+__global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_f64(const double* __restrict__ X, double* __restrict__ Y,
+                                                                int64_t L, const uint32_t* __restrict__ clq, int M,
+                                                                int64_t nK, const double* __restrict__ G, int gdim,
+                                                                int nct, int nkc) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int NT = TW / 16;
+    double* Gs = reinterpret_cast<double*>(smem);                     // [gdim][gdim]
+    const int Mp = (M + 15) & ~15;
+    double* Dall = Gs + size_t(gdim) * gdim;                          // per wave [Mp][TW]
+    uint32_t* memall = reinterpret_cast<uint32_t*>(Dall + size_t(CQ2_WAVES) * Mp * TW);  // per wave [Mp]
+    const int64_t nwork = int64_t(nct) * nkc;
+    const int64_t w = xcd_remap(blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int kc = int(w % nkc), ct = int(w / nkc);
+    const int64_t c0 = int64_t(ct) * TW;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < gdim * gdim; i += 64 * CQ2_WAVES) Gs[i] = G[i];
+    __syncthreads();
+    double* Ds = Dall + size_t(wave) * Mp * TW;
+    uint32_t* mem = memall + size_t(wave) * Mp;
+    const int nmt = Mp / 16;
+    for (int64_t K = int64_t(kc) * CQ2_KCH + wave; K < min(int64_t(kc + 1) * CQ2_KCH, nK); K += CQ2_WAVES) {
+        for (int i = lane; i < Mp; i += 64) mem[i] = i < M ? clq[K * M + i] : 0u;
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < Mp * TW; i += 64) {
+            int m = i / TW, c = i % TW;
+            double v = 0.0;
+            if (m < M && c0 + c < L) {
+                uint32_t e = mem[m];
+                v = X[int64_t(e & kAddrMask) * L + c0 + c];
+                if (e >> 31) v = -v;
+            }
+            Ds[i] = v;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int mt = 0; mt < nmt; ++mt) {
+            d4 acc[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = d4{0.0, 0.0, 0.0, 0.0};
+            const int row = mt * 16 + (lane & 15);
+            const int prow = row < M ? int((mem[row] >> 21) & 0x3FFu) : 0;
+            for (int k0 = 0; k0 < Mp; k0 += 4) {
+                const int kk = k0 + (lane >> 4);
+                double a = 0.0;
+                if (row < M && kk < M) a = Gs[prow * gdim + int((mem[kk] >> 21) & 0x3FFu)];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    double b = Ds[kk * TW + nt * 16 + (lane & 15)];
+                    acc[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[nt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int64_t col = c0 + nt * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * v + (lane >> 4);  // f64 output row
+                    if (r < M && col < L) {
+                        uint32_t e = mem[r];
+                        atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], (e >> 31) ? -acc[nt][v] : acc[nt][v]);
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template <int TW, int CQ2_WAVES>
+__global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* __restrict__ X, float* __restrict__ Y,
+                                                                  int64_t L, const uint32_t* __restrict__ clq, int M,
+                                                                  int64_t nK, const float* __restrict__ G, int gdim,
+                                                                  int nct, int nkc) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int NT = TW / 16;
+    float* Gs = reinterpret_cast<float*>(smem);                       // [gdim][gdim]
+    const int Mk = (M + 31) & ~31;                                    // K padded to 32
+    const int Mp = (M + 15) & ~15;                                    // rows padded to 16
+    const int KPP = Mk + 8;
+    __bf16* Dall = reinterpret_cast<__bf16*>(Gs + ((size_t(gdim) * gdim + 3) & ~size_t(3)));  // per wave hi,lo [TW][KPP]
+    uint32_t* memall = reinterpret_cast<uint32_t*>(Dall + size_t(CQ2_WAVES) * 2 * TW * KPP);
+    const int64_t nwork = int64_t(nct) * nkc;
+    const int64_t w = xcd_remap(blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int kc = int(w % nkc), ct = int(w / nkc);
+    const int64_t c0 = int64_t(ct) * TW;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < gdim * gdim; i += 64 * CQ2_WAVES) Gs[i] = G[i];
+    __syncthreads();
+    __bf16* Dh = Dall + size_t(wave) * 2 * TW * KPP;
+    __bf16* Dl = Dh + TW * KPP;
+    uint32_t* mem = memall + size_t(wave) * Mk;
+    const int nmt = Mp / 16;
+    for (int64_t K = int64_t(kc) * CQ2_KCH + wave; K < min(int64_t(kc + 1) * CQ2_KCH, nK); K += CQ2_WAVES) {
+        for (int i = lane; i < Mk; i += 64) mem[i] = i < M ? clq[K * M + i] : 0u;
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < Mk * TW; i += 64) {
+            int m = i / TW, c = i % TW;
+            float v = 0.f;
+            if (m < M && c0 + c < L) {
+                uint32_t e = mem[m];
+                v = X[int64_t(e & kAddrMask) * L + c0 + c];
+                if (e >> 31) v = -v;
+            }
+            split_bf16(v, Dh[c * KPP + m], Dl[c * KPP + m]);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        for (int mt = 0; mt < nmt; ++mt) {
+            f4 acc[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = f4{0.f, 0.f, 0.f, 0.f};
+            const int row = mt * 16 + (lane & 15);
+            const int prow = row < M ? int((mem[row] >> 21) & 0x3FFu) : 0;
+            for (int k0 = 0; k0 < Mk; k0 += 32) {
+                bf8 ah, al;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) {
+                    const int kk = k0 + 8 * (lane >> 4) + x;
+                    float g = 0.f;
+                    if (row < M && kk < M) g = Gs[prow * gdim + int((mem[kk] >> 21) & 0x3FFu)];
+                    __bf16 h, l;
+                    split_bf16(g, h, l);
+                    ah[x] = h;
+                    al[x] = l;
+                }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int off = (nt * 16 + (lane & 15)) * KPP + k0 + 8 * (lane >> 4);
+                    bf8 bh = *reinterpret_cast<const bf8*>(Dh + off);
+                    bf8 bl = *reinterpret_cast<const bf8*>(Dl + off);
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[nt], 0, 0, 0);
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[nt], 0, 0, 0);
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[nt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int64_t col = c0 + nt * 16 + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * (lane >> 4) + v;  // f32-output row
+                    if (r < M && col < L) {
+                        uint32_t e = mem[r];
+                        atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], (e >> 31) ? -acc[nt][v] : acc[nt][v]);
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ---- alpha-beta: per alpha row gather -> MFMA -> LDS scatter ----------------------------------
 constexpr int AB_TW = 64;      // Ib columns per tile (4 MFMA column tiles)
 constexpr int AB_MAXE = 160;   // max single excitations + 1 per string
@@ -296,6 +459,18 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
         }
     }
     for (int64_t c0 = 0; c0 < Nb; c0 += AB_TW) {
+        // scatter targets of this tile, loaded before the gather so their latency overlaps it
+        uint32_t tgt[AB_MTW][4][4];
+#pragma unroll
+        for (int j = 0; j < AB_MTW; ++j)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rs = (wave + 4 * j) * 16 + 4 * v + (lane >> 4);  // f64 output row
+                    const int64_t Ib = c0 + nt * 16 + (lane & 15);
+                    tgt[j][nt][v] = (rs < np && Ib < Nb) ? out_b[int64_t(rs) * Nb + Ib] : kInvalid;
+                }
         for (int i = tid; i < KP * AB_TW; i += 256) {
             int e = i / AB_TW, c = i % AB_TW;
             double v = 0.0;
@@ -328,13 +503,9 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                const int64_t Ib = c0 + nt * 16 + (lane & 15);
-                if (Ib >= Nb) continue;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int rs = mtile * 16 + 4 * v + (lane >> 4);  // f64 output row
-                    if (rs >= np) continue;
-                    uint32_t t = out_b[int64_t(rs) * Nb + Ib];
+                    const uint32_t t = tgt[j][nt][v];
                     if (t == kInvalid) continue;
                     double y = (t >> 31) ? -acc[nt][v] : acc[nt][v];
                     if (use_lds_row)
@@ -403,6 +574,17 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
         }
     }
     for (int64_t c0 = 0; c0 < Nb; c0 += AB_TW) {
+        uint32_t tgt[AB_MTW][4][4];
+#pragma unroll
+        for (int j = 0; j < AB_MTW; ++j)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rs = (wave + 4 * j) * 16 + 4 * (lane >> 4) + v;  // f32-output row
+                    const int64_t Ib = c0 + nt * 16 + (lane & 15);
+                    tgt[j][nt][v] = (rs < np && Ib < Nb) ? out_b[int64_t(rs) * Nb + Ib] : kInvalid;
+                }
         for (int i = tid; i < KP * AB_TW; i += 256) {
             int e = i / AB_TW, c = i % AB_TW;
             float v = 0.f;
@@ -439,13 +621,9 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
-                const int64_t Ib = c0 + nt * 16 + (lane & 15);
-                if (Ib >= Nb) continue;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int rs = mtile * 16 + 4 * (lane >> 4) + v;  // f32-output row
-                    if (rs >= np) continue;
-                    uint32_t t = out_b[int64_t(rs) * Nb + Ib];
+                    const uint32_t t = tgt[j][nt][v];
                     if (t == kInvalid) continue;
                     float y = (t >> 31) ? -acc[nt][v] : acc[nt][v];
                     if (use_lds_row)
@@ -491,6 +669,35 @@ template <class T>
 static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, const DevInts& ints, int spin,
                           hipStream_t st) {
     if (sp.d == 0 || sp.nK == 0) return;
+    {
+        // operator-in-LDS variant when it fits
+        const int gd = ints.gdim[spin];
+        const int Mp = (sp.M + 15) & ~15, Mk = (sp.M + 31) & ~31;
+        constexpr int TW64 = 16, TW32 = 32, WV64 = 4, WV32 = 8;
+        size_t sh = sizeof(T) == 8
+                        ? size_t(gd) * gd * 8 + size_t(WV64) * Mp * TW64 * 8 + size_t(WV64) * Mp * 4
+                        : ((size_t(gd) * gd + 3) & ~size_t(3)) * 4 + size_t(WV32) * 2 * TW32 * (Mk + 8) * 2 +
+                              size_t(WV32) * Mk * 4;
+        if (sh <= 150 * 1024 && sp.M <= 1023) {
+            const int tw = sizeof(T) == 8 ? TW64 : TW32;
+            const int nct = int((L + tw - 1) / tw);
+            const int nkc = int((sp.nK + CQ2_KCH - 1) / CQ2_KCH);
+            const int64_t nwork = int64_t(nct) * nkc;
+            if constexpr (sizeof(T) == 8) {
+                auto kern = k_clique2_f64<TW64, WV64>;
+                HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
+                kern<<<xcd_grid(nwork), 64 * WV64, sh, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd,
+                                                            nct, nkc);
+            } else {
+                auto kern = k_clique2_mixed<TW32, WV32>;
+                HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
+                kern<<<xcd_grid(nwork), 64 * WV32, sh, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd,
+                                                            nct, nkc);
+            }
+            HIPCHK(hipGetLastError());
+            return;
+        }
+    }
     if (sp.M > CQ_MAXM) throw Error(-1, "clique size exceeds kernel limit");
     int nct = int((L + CQ_TW - 1) / CQ_TW), nmt = (sp.M + CQ_MT - 1) / CQ_MT;
     int64_t nwork = int64_t(nct) * sp.nK * nmt;

@@ -526,6 +526,10 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("eigenvalues", &DavidsonLiuSolver::eigenvalues)
         .def("eigenvector", &DavidsonLiuSolver::eigenvector);
 
+    m.def("device_synchronize", [] {
+        py::gil_scoped_release nogil;
+        return fci_device_synchronize() == FCI_OK;
+    });
     m.def("seeded_integrals", [](int n, uint64_t seed) {
         std::vector<double> h(size_t(n) * n), eri(size_t(n) * n * n * n);
         fci_seeded_integrals(n, seed, h.data(), eri.data());

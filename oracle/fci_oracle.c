@@ -423,6 +423,9 @@ static void transpose_into(const double* c, double* m, int64_t na, int64_t nb) {
         for (int64_t b = 0; b < nb; ++b) m[b * na + a] = c[a * nb + b];
 }
 
+static int g_stride = 1;  /* outer-loop sampling stride (1 = full sigma) */
+static double g_done = 0.0, g_total = 0.0;
+
 static void H1(orc_ctx* ctx, const double* C, double* R, int alfa) {
     int64_t na = ctx->sp[0].nstr, nb = ctx->sp[1].nstr;
     const double* Cr;
@@ -444,6 +447,9 @@ static void H1(orc_ctx* ctx, const double* C, double* R, int alfa) {
         for (int q = 0; q < n; ++q) {
             const double Hpq = h[p * n + q];
             const sublist* l = &s->vo[p * n + q];
+            g_total += (double)l->size;
+            if ((p * n + q) % g_stride) continue;
+            g_done += (double)l->size;
             for (int64_t e = 0; e < l->size; ++e)
                 daxpy(maxL, l->sign[e] * Hpq, Cr + (int64_t)l->I[e] * maxL,
                       Cl + (int64_t)l->J[e] * maxL);
@@ -474,6 +480,9 @@ static void H2_aaaa2(orc_ctx* ctx, const double* C, double* R, int alfa) {
         int p = ctx->pair_p[pq], q = ctx->pair_q[pq];
         const double integral = tei[IDX4(p, q, p, q, n)];
         const sublist* l = &s->oo[pq];
+        g_total += (double)l->size;
+        if (pq % g_stride) continue;
+        g_done += (double)l->size;
         for (int64_t e = 0; e < l->size; ++e)
             daxpy(maxL, l->sign[e] * integral, Cr + (int64_t)l->I[e] * maxL,
                   Cl + (int64_t)l->J[e] * maxL);
@@ -484,6 +493,10 @@ static void H2_aaaa2(orc_ctx* ctx, const double* C, double* R, int alfa) {
             int r = ctx->pair_p[rs], ss = ctx->pair_q[rs];
             const double integral = tei[IDX4(p, q, r, ss, n)];
             const sublist* l1 = &s->vvoo[pq * ctx->npair + rs];
+            const sublist* l2s = &s->vvoo[rs * ctx->npair + pq];
+            g_total += (double)(l1->size + l2s->size);
+            if (pq % g_stride) continue;
+            g_done += (double)(l1->size + l2s->size);
             for (int64_t e = 0; e < l1->size; ++e)
                 daxpy(maxL, l1->sign[e] * integral, Cr + (int64_t)l1->I[e] * maxL,
                       Cl + (int64_t)l1->J[e] * maxL);
@@ -508,6 +521,9 @@ static void H2_aabb(orc_ctx* ctx, const double* C, double* R, int rs_begin, int 
             if (rs_lin < rs_begin || rs_lin >= rs_end) continue;
             const sublist* vb = &ctx->sp[1].vo[r * n + s];
             int64_t maxSSb = vb->size;
+            g_total += (double)maxSSb;
+            if (rs_lin % g_stride) continue;
+            g_done += (double)maxSSb;
             if (maxSSb == 0) continue;
             memset(ctx->CR, 0, (size_t)(ld * ld) * sizeof(double));
             memset(ctx->CL, 0, (size_t)(ld * ld) * sizeof(double));
@@ -567,6 +583,31 @@ void orc_sigma(orc_ctx* ctx, const double* C, double* sigma, int mask) {
         H2_aaaa2(ctx, C, sigma, 0);
         ctx->timers[5] = now_s() - t;
     }
+}
+
+void orc_sigma_sample(orc_ctx* ctx, const double* C, double* sigma, int stride, double* t6, double* w6) {
+    static const int masks[6] = {ORC_H0, ORC_H1A, ORC_H1B, ORC_H2AABB, ORC_H2AAAA, ORC_H2BBBB};
+    int64_t nd = orc_ndet(ctx);
+    memset(sigma, 0, (size_t)nd * sizeof(double));
+    g_stride = stride < 1 ? 1 : stride;
+    for (int t = 0; t < 6; ++t) {
+        g_done = g_total = 0.0;
+        double t0 = now_s();
+        switch (masks[t]) {
+            case ORC_H0:
+                for (int64_t i = 0; i < nd; ++i) sigma[i] = C[i] * ctx->e_core;
+                g_done = g_total = 1.0;
+                break;
+            case ORC_H1A: H1(ctx, C, sigma, 1); break;
+            case ORC_H1B: H1(ctx, C, sigma, 0); break;
+            case ORC_H2AABB: H2_aabb(ctx, C, sigma, 0, ctx->n * ctx->n); break;
+            case ORC_H2AAAA: H2_aaaa2(ctx, C, sigma, 1); break;
+            case ORC_H2BBBB: H2_aaaa2(ctx, C, sigma, 0); break;
+        }
+        t6[t] = now_s() - t0;
+        w6[t] = g_total > 0.0 ? g_done / g_total : 1.0;
+    }
+    g_stride = 1;
 }
 
 void orc_sigma_timers(const orc_ctx* ctx, double* t6) { memcpy(t6, ctx->timers, 6 * sizeof(double)); }

@@ -91,6 +91,11 @@ void orc_sigma(orc_ctx* ctx, const double* C, double* sigma, int mask);
 void orc_sigma_timers(const orc_ctx* ctx, double* t6);
 /* H2_aabb restricted to the beta pairs (r,s) with rs_begin <= r*n+s < rs_end (sampling). */
 void orc_sigma_aabb_range(orc_ctx* ctx, const double* C, double* sigma, int rs_begin, int rs_end);
+/* Timing sample of the Forte-structured sigma: every term runs only the outer-loop indices i
+ * with i % stride == 0 (H1: p*n+q, H2_aaaa2: pair pq, H2_aabb: r*n+s). t6 gets the sampled
+ * seconds per term (H0,H1a,H1b,H2aabb,H2aaaa,H2bbbb) and w6 the fraction of that term's list
+ * entries the sample covered, so t6[i]/w6[i] extrapolates the full term. */
+void orc_sigma_sample(orc_ctx* ctx, const double* C, double* sigma, int stride, double* t6, double* w6);
 void orc_hdiag(const orc_ctx* ctx, double* out);
 double orc_spin2(const orc_ctx* ctx, const double* C);
 /* level 1: g1a,g1b (n^2); level 2 adds g2aa,g2ab,g2bb (n^4), <a+p a+q a_s a_r> at [p,q,r,s]. */

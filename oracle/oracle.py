@@ -62,6 +62,7 @@ def lib():
         L.orc_vvoo_get.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [_d, _u32, _u32]
         L.orc_sigma.argtypes = [ctypes.c_void_p, _d, _d, ctypes.c_int]
         L.orc_sigma_timers.argtypes = [ctypes.c_void_p, _d]
+        L.orc_sigma_sample.argtypes = [ctypes.c_void_p, _d, _d, ctypes.c_int, _d, _d]
         L.orc_sigma_aabb_range.argtypes = [ctypes.c_void_p, _d, _d, ctypes.c_int, ctypes.c_int]
         L.orc_hdiag.argtypes = [ctypes.c_void_p, _d]
         L.orc_spin2.restype = ctypes.c_double
@@ -238,6 +239,14 @@ class Oracle:
         t = np.zeros(6)
         lib().orc_sigma_timers(self.h, _p(t))
         return t
+
+    def sigma_sample(self, C, stride):
+        """Strided timing sample of the Forte-structured sigma: (seconds[6], fraction[6])."""
+        C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
+        out = np.zeros(self.ndet)
+        t, w = np.zeros(6), np.zeros(6)
+        lib().orc_sigma_sample(self.h, _p(C), _p(out), int(stride), _p(t), _p(w))
+        return t, w
 
     def sigma_aabb_range(self, C, rs_begin, rs_end, out=None):
         C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
