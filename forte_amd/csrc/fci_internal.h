@@ -86,6 +86,10 @@ struct SpinTables {
     // when the excitation does not exist. P = (r,r): J = I when r occupied.
     int np = 0;
     DBuf<uint32_t> out;   // [np * nstr], pair-major
+    // the same targets transposed to [nstr][npp] (npp = np rounded up to 16) as 16-bit entries
+    // (address | sign << 15, 0x7FFF = none); built only when nstr <= 32767
+    int npp = 0;
+    DBuf<uint16_t> outT;
     // pair cliques for the same-spin operator: hole strings K with k-d electrons (d = 2, or 1
     // when k == 1); members = K plus d electrons in vir(K); packed (I | m<<21 | sign<<31) where m
     // is the pair index a(a-1)/2+b (d = 2) or the orbital (d = 1).
@@ -93,6 +97,9 @@ struct SpinTables {
     int64_t nK = 0;
     int M = 0;
     DBuf<uint32_t> clq;   // [nK * M]
+    // owners: for each string I, the C(k,d) entries K * M + m with clq[K * M + m] addressing I
+    int nown = 0;
+    DBuf<uint32_t> own;   // [nstr * nown]
 };
 
 void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* d_binom, hipStream_t st);

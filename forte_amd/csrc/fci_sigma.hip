@@ -22,6 +22,7 @@
 #include "fci_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace fci {
@@ -244,42 +245,41 @@ __global__ __launch_bounds__(256) void k_clique_mixed(const float* __restrict__ 
         }
 }
 
-// Column-tile-major variant: the whole operator g' lives in LDS, each wave owns one hole string
-// K at a time, and the XCD-contiguous work order keeps one column tile of X and Y resident in
-// that XCD's L2 while every K of the tile passes through it. The MFMA A operand is read from
-// g' through the member pair indices, so no per-K copy of W_K is made.
-constexpr int CQ2_KCH = 64;    // hole strings per block
+// Column-tile-major variant: each wave owns one hole string K at a time (kch hole strings per
+// block) and the XCD-contiguous work order keeps one column tile of X resident in that XCD's L2
+// while every K of the tile passes through it. The MFMA A operand is read from g' through the
+// member pair indices, so no per-K copy of W_K is made (FP64: g' through L1; mixed: g' in LDS).
 
-template <int TW, int CQ2_WAVES>
+// STORE = true: no atomics; E[(K*M + m) * Ls + (col - cbeg)] = s_m sum_m' W s_m' X (pass 1 of the
+// two-pass same-spin product, gathered by k_clique_gather).
+template <int TW, int CQ2_WAVES, bool STORE>
 __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_f64(const double* __restrict__ X, double* __restrict__ Y,
                                                                 int64_t L, const uint32_t* __restrict__ clq, int M,
                                                                 int64_t nK, const double* __restrict__ G, int gdim,
-                                                                int nct, int nkc) {
+                                                                int nct, int nkc, int kch, int64_t cbeg, int64_t Ls) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int NT = TW / 16;
-    double* Gs = reinterpret_cast<double*>(smem);                     // [gdim][gdim]
+    const double* Gs = G;                                             // read through L1/L2
     const int Mp = (M + 15) & ~15;
-    double* Dall = Gs + size_t(gdim) * gdim;                          // per wave [Mp][TW]
+    double* Dall = reinterpret_cast<double*>(smem);                   // per wave [Mp][TW]
     uint32_t* memall = reinterpret_cast<uint32_t*>(Dall + size_t(CQ2_WAVES) * Mp * TW);  // per wave [Mp]
     const int64_t nwork = int64_t(nct) * nkc;
     const int64_t w = xcd_remap(blockIdx.x, nwork);
     if (w >= nwork) return;
     const int kc = int(w % nkc), ct = int(w / nkc);
-    const int64_t c0 = int64_t(ct) * TW;
+    const int64_t c0 = cbeg + int64_t(ct) * TW;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < gdim * gdim; i += 64 * CQ2_WAVES) Gs[i] = G[i];
-    __syncthreads();
     double* Ds = Dall + size_t(wave) * Mp * TW;
     uint32_t* mem = memall + size_t(wave) * Mp;
     const int nmt = Mp / 16;
-    for (int64_t K = int64_t(kc) * CQ2_KCH + wave; K < min(int64_t(kc + 1) * CQ2_KCH, nK); K += CQ2_WAVES) {
+    for (int64_t K = int64_t(kc) * kch + wave; K < min(int64_t(kc + 1) * kch, nK); K += CQ2_WAVES) {
         for (int i = lane; i < Mp; i += 64) mem[i] = i < M ? clq[K * M + i] : 0u;
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
         for (int i = lane; i < Mp * TW; i += 64) {
             int m = i / TW, c = i % TW;
             double v = 0.0;
-            if (m < M && c0 + c < L) {
+            if (m < M && c0 + c < L && c0 + c < cbeg + Ls) {
                 uint32_t e = mem[m];
                 v = X[int64_t(e & kAddrMask) * L + c0 + c];
                 if (e >> 31) v = -v;
@@ -310,9 +310,13 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_f64(const double* __
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = mt * 16 + 4 * v + (lane >> 4);  // f64 output row
-                    if (r < M && col < L) {
+                    if (r < M && col < cbeg + Ls && col < L) {
                         uint32_t e = mem[r];
-                        atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], (e >> 31) ? -acc[nt][v] : acc[nt][v]);
+                        const double y = (e >> 31) ? -acc[nt][v] : acc[nt][v];
+                        if constexpr (STORE)
+                            Y[(K * M + r) * Ls + (col - cbeg)] = y;
+                        else
+                            atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], y);
                     }
                 }
             }
@@ -321,11 +325,11 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_f64(const double* __
     }
 }
 
-template <int TW, int CQ2_WAVES>
+template <int TW, int CQ2_WAVES, bool STORE>
 __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* __restrict__ X, float* __restrict__ Y,
                                                                   int64_t L, const uint32_t* __restrict__ clq, int M,
                                                                   int64_t nK, const float* __restrict__ G, int gdim,
-                                                                  int nct, int nkc) {
+                                                                  int nct, int nkc, int kch, int64_t cbeg, int64_t Ls) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int NT = TW / 16;
     float* Gs = reinterpret_cast<float*>(smem);                       // [gdim][gdim]
@@ -338,7 +342,7 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* _
     const int64_t w = xcd_remap(blockIdx.x, nwork);
     if (w >= nwork) return;
     const int kc = int(w % nkc), ct = int(w / nkc);
-    const int64_t c0 = int64_t(ct) * TW;
+    const int64_t c0 = cbeg + int64_t(ct) * TW;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < gdim * gdim; i += 64 * CQ2_WAVES) Gs[i] = G[i];
     __syncthreads();
@@ -346,14 +350,14 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* _
     __bf16* Dl = Dh + TW * KPP;
     uint32_t* mem = memall + size_t(wave) * Mk;
     const int nmt = Mp / 16;
-    for (int64_t K = int64_t(kc) * CQ2_KCH + wave; K < min(int64_t(kc + 1) * CQ2_KCH, nK); K += CQ2_WAVES) {
+    for (int64_t K = int64_t(kc) * kch + wave; K < min(int64_t(kc + 1) * kch, nK); K += CQ2_WAVES) {
         for (int i = lane; i < Mk; i += 64) mem[i] = i < M ? clq[K * M + i] : 0u;
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
         for (int i = lane; i < Mk * TW; i += 64) {
             int m = i / TW, c = i % TW;
             float v = 0.f;
-            if (m < M && c0 + c < L) {
+            if (m < M && c0 + c < L && c0 + c < cbeg + Ls) {
                 uint32_t e = mem[m];
                 v = X[int64_t(e & kAddrMask) * L + c0 + c];
                 if (e >> 31) v = -v;
@@ -396,15 +400,236 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* _
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = mt * 16 + 4 * (lane >> 4) + v;  // f32-output row
-                    if (r < M && col < L) {
+                    if (r < M && col < cbeg + Ls && col < L) {
                         uint32_t e = mem[r];
-                        atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], (e >> 31) ? -acc[nt][v] : acc[nt][v]);
+                        const float y = (e >> 31) ? -acc[nt][v] : acc[nt][v];
+                        if constexpr (STORE)
+                            Y[(K * M + r) * Ls + (col - cbeg)] = y;
+                        else
+                            atomicAdd(&Y[int64_t(e & kAddrMask) * L + col], y);
                     }
                 }
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Pass 1 of the two-pass same-spin product for cliques of at most 64 members, with no LDS: each
+// wave owns one (hole string K, 64-column chunk) work item, keeps the MFMA A operand
+// W_K[m][m'] = g'[pair(m)][pair(m')] in registers and streams 16-column tiles of
+// X[I_m'][c] straight from global memory into the B operand layout, one tile ahead.
+// E[(K * M + m) * Ls + c - cbeg] = s_m sum_m' W_K[m][m'] s_m' X[I_m'][c].
+constexpr int CQ3_CW = 64;  // columns per work item
+
+template <int NMT>  // row tiles of 16 members; k steps of 32 members: (NMT + 1) / 2
+__global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__ X, float* __restrict__ E, int64_t L,
+                                                       const uint32_t* __restrict__ clq, int M, int64_t nK,
+                                                       const float* __restrict__ G, int gdim, int nchunk,
+                                                       int64_t cbeg, int64_t Ls) {
+    constexpr int NKS = (NMT + 1) / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int64_t nwork = nK * nchunk;
+    const int64_t nblk = (nwork + 3) / 4;
+    const int64_t w = xcd_remap(blockIdx.x, nblk) * 4 + wave;
+    if (w >= nwork) return;
+    const int64_t K = w % nK;
+    const int chunk = int(w / nK);
+    const uint32_t* ent = clq + K * M;
+    uint32_t mb[NKS][8];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const int m = ks * 32 + 8 * g + x;
+            mb[ks][x] = m < M ? ent[m] : kInvalid;
+        }
+    bf8 ah[NMT][NKS], al[NMT][NKS];
+    uint32_t neg[NMT];  // sign bits of this lane's 4 output rows, bit v
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) {
+        const int r = mt * 16 + cl;
+        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t e = mb[ks][x];
+                const float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                __bf16 h, l;
+                split_bf16(v, h, l);
+                ah[mt][ks][x] = h;
+                al[mt][ks][x] = l;
+            }
+        uint32_t nb = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ro = mt * 16 + 4 * g + v;
+            if (ro < M) nb |= (ent[ro] >> 31) << v;
+        }
+        neg[mt] = nb;
+    }
+    const int64_t cend = cbeg + Ls < L ? cbeg + Ls : L;
+    const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
+    float xv[NKS][8], xn[NKS][8];
+    auto load = [&](int64_t col, float (&xx)[NKS][8]) {
+        const bool ok = col < cend;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t e = mb[ks][x];
+                float v = 0.f;
+                if (ok && e != kInvalid) {
+                    v = X[int64_t(e & kAddrMask) * L + col];
+                    if (e >> 31) v = -v;
+                }
+                xx[ks][x] = v;
+            }
+    };
+    load(c0 + cl, xv);
+#pragma unroll 1
+    for (int ct = 0; ct < CQ3_CW / 16; ++ct) {
+        const int64_t col = c0 + ct * 16 + cl;
+        if (ct + 1 < CQ3_CW / 16) load(col + 16, xn);
+        bf8 bh[NKS], bl[NKS];
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                __bf16 h, l;
+                split_bf16(xv[ks][x], h, l);
+                bh[ks][x] = h;
+                bl[ks][x] = l;
+            }
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+            f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt][ks], bh[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bl[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bh[ks], acc, 0, 0, 0);
+            }
+            if (col < cend) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * g + v;
+                    if (r < M) E[(K * M + r) * Ls + (col - cbeg)] = ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) xv[ks][x] = xn[ks][x];
+    }
+}
+
+template <int NK4>  // k steps of 4 members; row tiles of 16: (NK4 + 3) / 4
+__global__ __launch_bounds__(256) void k_clique3_f64(const double* __restrict__ X, double* __restrict__ E, int64_t L,
+                                                     const uint32_t* __restrict__ clq, int M, int64_t nK,
+                                                     const double* __restrict__ G, int gdim, int nchunk,
+                                                     int64_t cbeg, int64_t Ls) {
+    constexpr int NMT = (NK4 + 3) / 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int64_t nwork = nK * nchunk;
+    const int64_t nblk = (nwork + 3) / 4;
+    const int64_t w = xcd_remap(blockIdx.x, nblk) * 4 + wave;
+    if (w >= nwork) return;
+    const int64_t K = w % nK;
+    const int chunk = int(w / nK);
+    const uint32_t* ent = clq + K * M;
+    uint32_t mb[NK4];
+#pragma unroll
+    for (int ks = 0; ks < NK4; ++ks) {
+        const int m = ks * 4 + g;
+        mb[ks] = m < M ? ent[m] : kInvalid;
+    }
+    double va[NMT][NK4];
+    uint32_t neg[NMT];
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) {
+        const int r = mt * 16 + cl;
+        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+#pragma unroll
+        for (int ks = 0; ks < NK4; ++ks) {
+            const uint32_t e = mb[ks];
+            va[mt][ks] = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.0;
+        }
+        uint32_t nb = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ro = mt * 16 + 4 * v + g;  // f64 output row
+            if (ro < M) nb |= (ent[ro] >> 31) << v;
+        }
+        neg[mt] = nb;
+    }
+    const int64_t cend = cbeg + Ls < L ? cbeg + Ls : L;
+    const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
+    double xv[NK4], xn[NK4];
+    auto load = [&](int64_t col, double (&xx)[NK4]) {
+        const bool ok = col < cend;
+#pragma unroll
+        for (int ks = 0; ks < NK4; ++ks) {
+            const uint32_t e = mb[ks];
+            double v = 0.0;
+            if (ok && e != kInvalid) {
+                v = X[int64_t(e & kAddrMask) * L + col];
+                if (e >> 31) v = -v;
+            }
+            xx[ks] = v;
+        }
+    };
+    load(c0 + cl, xv);
+#pragma unroll 1
+    for (int ct = 0; ct < CQ3_CW / 16; ++ct) {
+        const int64_t col = c0 + ct * 16 + cl;
+        if (ct + 1 < CQ3_CW / 16) load(col + 16, xn);
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < NK4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt][ks], xv[ks], acc, 0, 0, 0);
+            if (col < cend) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * v + g;
+                    if (r < M) E[(K * M + r) * Ls + (col - cbeg)] = ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < NK4; ++ks) xv[ks] = xn[ks];
+    }
+}
+
+// Pass 2: Y[I][cbeg + c] (= or +=) sum_j E[own[I][j]][c] for c < ncol, E rows of stride Ls:
+// every hole string K inside I hands I its row of s_m (W_K s X_K)_m. No atomics, and the sum
+// runs in a fixed order, so the result does not depend on scheduling.
+template <class T, bool ACC>
+__global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, T* __restrict__ Y, int64_t nrow,
+                                                       int64_t L, const uint32_t* __restrict__ own, int nown,
+                                                       int64_t cbeg, int64_t ncol, int64_t Ls) {
+    // block: 4 rows x 64 columns
+    const int64_t nct = (ncol + 63) / 64;
+    const int64_t nwork = ((nrow + 3) / 4) * nct;
+    const int64_t w = xcd_remap(blockIdx.x, nwork);
+    if (w >= nwork) return;
+    const int64_t rb = w / nct, ct = w % nct;
+    const int64_t I = rb * 4 + (threadIdx.x >> 6);
+    const int64_t cl = ct * 64 + (threadIdx.x & 63);
+    if (I >= nrow || cl >= ncol) return;
+    const uint32_t* o = own + I * nown;
+    T acc = T(0);
+    for (int j = 0; j < nown; ++j) acc += E[int64_t(o[j]) * Ls + cl];
+    T* y = Y + I * L + cbeg + cl;
+    if constexpr (ACC)
+        *y += acc;
+    else
+        *y = acc;
 }
 
 // ---- alpha-beta: per alpha row gather -> MFMA -> LDS scatter ----------------------------------
@@ -422,12 +647,12 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int KP = KS * 4;
     double* Xs = reinterpret_cast<double*>(smem);       // [KP][AB_TW]
-    double* srow = Xs + KP * AB_TW;                     // [Nb] when use_lds_row
+    double* xd = Xs + KP * AB_TW;                       // [AB_TW] row Ja of C (diagonal term)
+    double* srow = xd + AB_TW;                          // [Nb] when use_lds_row
     __shared__ uint32_t ent[AB_MAXE];
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int E = ns + 1;
     for (int i = tid; i < ns; i += 256) ent[i] = into_a[Ja * ns + i];
     if (use_lds_row)
         for (int64_t i = tid; i < Nb; i += 256) srow[i] = 0.0;
@@ -435,6 +660,7 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
     const uint32_t occ = str_a[Ja];
     // A operand (V) in registers for this wave's row tiles: V[rs][e] = (p_e q_e | rs)
     double va[AB_MTW][KS];
+    double vd[AB_MTW][4];  // diagonal excitations for this lane's output rows
 #pragma unroll
     for (int j = 0; j < AB_MTW; ++j) {
         const int mtile = wave + 4 * j;
@@ -443,19 +669,22 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
         for (int ks = 0; ks < KS; ++ks) {
             const int e = ks * 4 + (lane >> 4);
             double v = 0.0;
-            if (mtile < nmt && rs < np) {
-                if (e < ns) {
-                    v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
-                } else if (e == ns) {
-                    uint32_t o = occ;
-                    while (o) {
-                        int q = __ffs(o) - 1;
-                        o &= o - 1u;
-                        v += vpk[size_t(q * (q + 1) / 2 + q) * np + rs];
-                    }
+            if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
+            va[j][ks] = v;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = mtile * 16 + 4 * v + (lane >> 4);  // f64 output row
+            double d = 0.0;
+            if (mtile < nmt && r < np) {
+                uint32_t o = occ;
+                while (o) {
+                    int q = __ffs(o) - 1;
+                    o &= o - 1u;
+                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
                 }
             }
-            va[j][ks] = v;
+            vd[j][v] = d;
         }
     }
     for (int64_t c0 = 0; c0 < Nb; c0 += AB_TW) {
@@ -474,17 +703,14 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
         for (int i = tid; i < KP * AB_TW; i += 256) {
             int e = i / AB_TW, c = i % AB_TW;
             double v = 0.0;
-            if (c0 + c < Nb) {
-                if (e < ns) {
-                    uint32_t t = ent[e];
-                    v = C[int64_t(t & kAddrMask) * Nb + c0 + c];
-                    if (t >> 31) v = -v;
-                } else if (e == ns) {
-                    v = C[Ja * Nb + c0 + c];
-                }
+            if (c0 + c < Nb && e < ns) {
+                uint32_t t = ent[e];
+                v = C[int64_t(t & kAddrMask) * Nb + c0 + c];
+                if (t >> 31) v = -v;
             }
             Xs[e * AB_TW + c] = v;
         }
+        for (int c = tid; c < AB_TW; c += 256) xd[c] = (c0 + c < Nb) ? C[Ja * Nb + c0 + c] : 0.0;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < AB_MTW; ++j) {
@@ -503,11 +729,13 @@ __global__ __launch_bounds__(256) void k_ab_f64(const double* __restrict__ C, do
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
+                const double cd = xd[nt * 16 + (lane & 15)];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t t = tgt[j][nt][v];
                     if (t == kInvalid) continue;
-                    double y = (t >> 31) ? -acc[nt][v] : acc[nt][v];
+                    const double a = acc[nt][v] + vd[j][v] * cd;
+                    double y = (t >> 31) ? -a : a;
                     if (use_lds_row)
                         atomicAdd(&srow[t & kAddrMask], y);
                     else
@@ -533,7 +761,8 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
     constexpr int KPP = KP + 8;  // padded k stride
     __bf16* Xh = reinterpret_cast<__bf16*>(smem);     // [AB_TW][KPP]
     __bf16* Xl = Xh + AB_TW * KPP;
-    float* srow = reinterpret_cast<float*>(Xl + AB_TW * KPP);
+    float* xd = reinterpret_cast<float*>(Xl + AB_TW * KPP);  // [AB_TW] row Ja of C (diagonal term)
+    float* srow = xd + AB_TW;
     __shared__ uint32_t ent[AB_MAXE];
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
@@ -544,6 +773,7 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
     __syncthreads();
     const uint32_t occ = str_a[Ja];
     bf8 vh[AB_MTW][KS], vl[AB_MTW][KS];
+    float vd[AB_MTW][4];  // diagonal excitations, sum_{q in Ja} (qq|rs), for this lane's output rows
 #pragma unroll
     for (int j = 0; j < AB_MTW; ++j) {
         const int mtile = wave + 4 * j;
@@ -554,23 +784,26 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
             for (int x = 0; x < 8; ++x) {
                 const int e = ks * 32 + 8 * (lane >> 4) + x;
                 float v = 0.f;
-                if (mtile < nmt && rs < np) {
-                    if (e < ns) {
-                        v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
-                    } else if (e == ns) {
-                        uint32_t o = occ;
-                        while (o) {
-                            int q = __ffs(o) - 1;
-                            o &= o - 1u;
-                            v += vpk[size_t(q * (q + 1) / 2 + q) * np + rs];
-                        }
-                    }
-                }
+                if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
                 __bf16 h, l;
                 split_bf16(v, h, l);
                 vh[j][ks][x] = h;
                 vl[j][ks][x] = l;
             }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = mtile * 16 + 4 * (lane >> 4) + v;
+            float d = 0.f;
+            if (mtile < nmt && r < np) {
+                uint32_t o = occ;
+                while (o) {
+                    int q = __ffs(o) - 1;
+                    o &= o - 1u;
+                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+                }
+            }
+            vd[j][v] = d;
         }
     }
     for (int64_t c0 = 0; c0 < Nb; c0 += AB_TW) {
@@ -588,17 +821,14 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
         for (int i = tid; i < KP * AB_TW; i += 256) {
             int e = i / AB_TW, c = i % AB_TW;
             float v = 0.f;
-            if (c0 + c < Nb) {
-                if (e < ns) {
-                    uint32_t t = ent[e];
-                    v = C[int64_t(t & kAddrMask) * Nb + c0 + c];
-                    if (t >> 31) v = -v;
-                } else if (e == ns) {
-                    v = C[Ja * Nb + c0 + c];
-                }
+            if (c0 + c < Nb && e < ns) {
+                uint32_t t = ent[e];
+                v = C[int64_t(t & kAddrMask) * Nb + c0 + c];
+                if (t >> 31) v = -v;
             }
             split_bf16(v, Xh[c * KPP + e], Xl[c * KPP + e]);
         }
+        for (int c = tid; c < AB_TW; c += 256) xd[c] = (c0 + c < Nb) ? C[Ja * Nb + c0 + c] : 0.f;
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < AB_MTW; ++j) {
@@ -621,11 +851,13 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
+                const float cd = xd[nt * 16 + (lane & 15)];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t t = tgt[j][nt][v];
                     if (t == kInvalid) continue;
-                    float y = (t >> 31) ? -acc[nt][v] : acc[nt][v];
+                    const float a = acc[nt][v] + vd[j][v] * cd;
+                    float y = (t >> 31) ? -a : a;
                     if (use_lds_row)
                         atomicAdd(&srow[t & kAddrMask], y);
                     else
@@ -639,10 +871,262 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
         for (int64_t i = tid; i < Nb; i += 256) S[Ja * Nb + i] += srow[i];
 }
 
+// ---- alpha-beta, register-resident variant ----------------------------------------------------
+// One block per alpha string Ja, 4 waves = WM row-tile groups x (4 / WM) column groups. Each wave
+// keeps the A operand V[rs][e] = (p_e q_e | rs) of its row tiles (mtile = wm + WM * j) in
+// registers for the whole block and streams its own 16 beta columns at a time: the B operand
+// X[e][Ib] = s_e C[Ia_e][Ib] is loaded from global memory straight into the MFMA register layout
+// (no LDS staging), and the product plus the diagonal rank-1 term is scattered into the LDS row
+// srow[Jb] with Jb = E_rs Ib from the transposed 16-bit table outT[Ib][npp] (4 consecutive rs per
+// lane -> one 8-byte load). Requires Nb <= 32767.
+constexpr uint16_t kInv16 = 0x7FFFu;
+
+template <int KS, int TPW, int WM>
+__global__ __launch_bounds__(256, 2) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
+                                                      int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
+                                                      const uint32_t* __restrict__ str_a,
+                                                      const uint16_t* __restrict__ outT, int npp, int np,
+                                                      const float* __restrict__ vpk, int nmt) {
+    extern __shared__ __attribute__((aligned(16))) float srow[];  // [Nb]
+    __shared__ uint32_t ent[KS * 32];
+    constexpr int WC = 4 / WM;
+    const int64_t Ja = xcd_remap(blockIdx.x, Na);
+    if (Ja >= Na) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wc = wave / WM;
+    const int g = lane >> 4, cl = lane & 15;
+    for (int i = tid; i < KS * 32; i += 256) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
+    for (int64_t i = tid; i < Nb; i += 256) srow[i] = 0.f;
+    __syncthreads();
+    const uint32_t occ = str_a[Ja];
+    bf8 ah[TPW][KS], al[TPW][KS];
+    float vd[TPW][4];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int mtile = wm + WM * j;
+        const int rs = mtile * 16 + cl;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const int e = ks * 32 + 8 * g + x;
+                float v = 0.f;
+                if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
+                __bf16 h, l;
+                split_bf16(v, h, l);
+                ah[j][ks][x] = h;
+                al[j][ks][x] = l;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = mtile * 16 + 4 * g + v;
+            float d = 0.f;
+            if (mtile < nmt && r < np) {
+                uint32_t o = occ;
+                while (o) {
+                    const int q = __ffs(o) - 1;
+                    o &= o - 1u;
+                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+                }
+            }
+            vd[j][v] = d;
+        }
+    }
+    // gather rows of this lane's k group: e = ks * 32 + 8 g + x
+    uint32_t src[KS][8];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const int e = ks * 32 + 8 * g + x;
+            src[ks][x] = e < ns ? ent[e] : kInvalid;
+        }
+    const float* Crow = C + Ja * Nb;
+    // one column tile of loads in flight while the previous tile is multiplied and scattered
+    float xv[KS][8], xn[KS][8];
+    uint64_t tg[TPW], tn[TPW];
+    float cd, cn;
+    auto load = [&](int64_t c0, float (&x)[KS][8], uint64_t (&t)[TPW], float& d) {
+        const int64_t Ib = c0 + cl;
+        const bool ok = Ib < Nb;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            const int mtile = wm + WM * j;
+            t[j] = (ok && mtile < nmt) ? *reinterpret_cast<const uint64_t*>(outT + Ib * npp + mtile * 16 + 4 * g)
+                                       : ~uint64_t(0);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx) {
+                const uint32_t e = src[ks][xx];
+                float v = 0.f;
+                if (ok && e != kInvalid) {
+                    v = C[int64_t(e & kAddrMask) * Nb + Ib];
+                    if (e >> 31) v = -v;
+                }
+                x[ks][xx] = v;
+            }
+        d = ok ? Crow[Ib] : 0.f;
+    };
+    load(int64_t(wc) * 16, xv, tg, cd);
+    for (int64_t c0 = int64_t(wc) * 16; c0 < Nb; c0 += 16 * WC) {
+        load(c0 + 16 * WC, xn, tn, cn);
+        bf8 bh[KS], bl[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx) {
+                __bf16 h, l;
+                split_bf16(xv[ks][xx], h, l);
+                bh[ks][xx] = h;
+                bl[ks][xx] = l;
+            }
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j][ks], bh[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bl[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bh[ks], acc, 0, 0, 0);
+            }
+            const uint64_t t4 = tg[j];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t t = uint32_t(t4 >> (16 * v)) & 0xFFFFu;
+                if ((t & 0x7FFFu) == kInv16) continue;
+                const float a = acc[v] + vd[j][v] * cd;
+                atomicAdd(&srow[t & 0x7FFFu], (t >> 15) ? -a : a);
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx) xv[ks][xx] = xn[ks][xx];
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) tg[j] = tn[j];
+        cd = cn;
+    }
+    __syncthreads();
+    float* Srow = S + Ja * Nb;
+    for (int64_t i = tid; i < Nb; i += 256) Srow[i] += srow[i];
+}
+
+// FP64: 16x16x4 MFMA, KS k-steps of 4 single excitations. The A operand rows are loaded in the
+// permuted order m -> rs = 4 (m % 4) + m / 4 inside each tile so that the accumulator element v of
+// lane (g = lane / 16) is row rs = 4 g + v, the layout of the 16-bit target table.
+template <int KS, int TPW>
+__global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C, double* __restrict__ S, int64_t Na,
+                                                    int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
+                                                    const uint32_t* __restrict__ str_a,
+                                                    const uint16_t* __restrict__ outT, int npp, int np,
+                                                    const double* __restrict__ vpk, int nmt) {
+    extern __shared__ __attribute__((aligned(16))) double srow64[];  // [Nb]
+    __shared__ uint32_t ent[KS * 4];
+    constexpr int WM = 4, WC = 2, NTH = 64 * WM * WC;
+    const int64_t Ja = xcd_remap(blockIdx.x, Na);
+    if (Ja >= Na) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wc = wave / WM;
+    const int g = lane >> 4, cl = lane & 15;
+    for (int i = tid; i < KS * 4; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
+    for (int64_t i = tid; i < Nb; i += NTH) srow64[i] = 0.0;
+    __syncthreads();
+    const uint32_t occ = str_a[Ja];
+    double va[TPW][KS];
+    double vd[TPW][4];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int mtile = wm + WM * j;
+        const int rs = mtile * 16 + 4 * (cl % 4) + cl / 4;  // permuted A row
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int e = ks * 4 + g;
+            double v = 0.0;
+            if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
+            va[j][ks] = v;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = mtile * 16 + 4 * g + v;
+            double d = 0.0;
+            if (mtile < nmt && r < np) {
+                uint32_t o = occ;
+                while (o) {
+                    const int q = __ffs(o) - 1;
+                    o &= o - 1u;
+                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+                }
+            }
+            vd[j][v] = d;
+        }
+    }
+    uint32_t src[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int e = ks * 4 + g;
+        src[ks] = e < ns ? ent[e] : kInvalid;
+    }
+    const double* Crow = C + Ja * Nb;
+    double xv[KS], xn[KS];
+    uint64_t tg[TPW], tn[TPW];
+    double cd, cn;
+    auto load = [&](int64_t c0, double (&x)[KS], uint64_t (&t)[TPW], double& d) {
+        const int64_t Ib = c0 + cl;
+        const bool ok = Ib < Nb;
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            const int mtile = wm + WM * j;
+            t[j] = (ok && mtile < nmt) ? *reinterpret_cast<const uint64_t*>(outT + Ib * npp + mtile * 16 + 4 * g)
+                                       : ~uint64_t(0);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const uint32_t e = src[ks];
+            double v = 0.0;
+            if (ok && e != kInvalid) {
+                v = C[int64_t(e & kAddrMask) * Nb + Ib];
+                if (e >> 31) v = -v;
+            }
+            x[ks] = v;
+        }
+        d = ok ? Crow[Ib] : 0.0;
+    };
+    load(int64_t(wc) * 16, xv, tg, cd);
+    for (int64_t c0 = int64_t(wc) * 16; c0 < Nb; c0 += 16 * WC) {
+        load(c0 + 16 * WC, xn, tn, cn);
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks], acc, 0, 0, 0);
+            const uint64_t t4 = tg[j];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t t = uint32_t(t4 >> (16 * v)) & 0xFFFFu;
+                if ((t & 0x7FFFu) == kInv16) continue;
+                const double a = acc[v] + vd[j][v] * cd;
+                atomicAdd(&srow64[t & 0x7FFFu], (t >> 15) ? -a : a);
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) xv[ks] = xn[ks];
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) tg[j] = tn[j];
+        cd = cn;
+    }
+    __syncthreads();
+    double* Srow = S + Ja * Nb;
+    for (int64_t i = tid; i < Nb; i += NTH) Srow[i] += srow64[i];
+}
+
 // ---- launchers -----------------------------------------------------------------------------
 struct SigmaWork {
     DBuf<double> t64a, t64b;
     DBuf<float> t32a, t32b;
+    DBuf<unsigned char> clique;  // two-pass same-spin scratch
     hipEvent_t ev[4];
     bool ev_ok = false;
 };
@@ -665,56 +1149,122 @@ template <class T> static DBuf<T>& scratch(SigmaWork& w, int which);
 template <> DBuf<double>& scratch<double>(SigmaWork& w, int which) { return which ? w.t64b : w.t64a; }
 template <> DBuf<float>& scratch<float>(SigmaWork& w, int which) { return which ? w.t32b : w.t32a; }
 
+// Bytes of scratch for the two-pass same-spin product (E holds nK*M rows of a column chunk).
+static size_t clique_scratch_bytes() {
+    static size_t v = [] {
+        size_t mb = 4096;
+        if (const char* e = std::getenv("FCI_CLIQUE_SCRATCH_MB")) {
+            long x = std::atol(e);
+            if (x >= 16) mb = size_t(x);
+        }
+        return mb << 20;
+    }();
+    return v;
+}
+
+// Same-spin product Y (+)= sum_K P_K^T S_K W_K S_K P_K X over the pair cliques of one spin, X and
+// Y with L columns. Pass 1 (k_clique2_*<STORE>) writes E_K = S_K W_K S_K X_K for a column chunk,
+// pass 2 (k_clique_gather) sums, for every string, the C(k,d) rows of E that address it.
 template <class T>
 static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, const DevInts& ints, int spin,
-                          hipStream_t st) {
-    if (sp.d == 0 || sp.nK == 0) return;
-    {
-        // operator-in-LDS variant when it fits
-        const int gd = ints.gdim[spin];
-        const int Mp = (sp.M + 15) & ~15, Mk = (sp.M + 31) & ~31;
-        constexpr int TW64 = 16, TW32 = 32, WV64 = 4, WV32 = 8;
-        size_t sh = sizeof(T) == 8
-                        ? size_t(gd) * gd * 8 + size_t(WV64) * Mp * TW64 * 8 + size_t(WV64) * Mp * 4
-                        : ((size_t(gd) * gd + 3) & ~size_t(3)) * 4 + size_t(WV32) * 2 * TW32 * (Mk + 8) * 2 +
-                              size_t(WV32) * Mk * 4;
-        if (sh <= 150 * 1024 && sp.M <= 1023) {
-            const int tw = sizeof(T) == 8 ? TW64 : TW32;
-            const int nct = int((L + tw - 1) / tw);
-            const int nkc = int((sp.nK + CQ2_KCH - 1) / CQ2_KCH);
-            const int64_t nwork = int64_t(nct) * nkc;
-            if constexpr (sizeof(T) == 8) {
-                auto kern = k_clique2_f64<TW64, WV64>;
-                HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
-                kern<<<xcd_grid(nwork), 64 * WV64, sh, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd,
-                                                            nct, nkc);
-            } else {
-                auto kern = k_clique2_mixed<TW32, WV32>;
-                HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
-                kern<<<xcd_grid(nwork), 64 * WV32, sh, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd,
-                                                            nct, nkc);
-            }
-            HIPCHK(hipGetLastError());
-            return;
-        }
+                          bool accumulate, SigmaWork& w, hipStream_t st) {
+    if (sp.d == 0 || sp.nK == 0) {
+        if (!accumulate) k_fill<T><<<nblk(sp.nstr * L, 256), 256, 0, st>>>(Y, sp.nstr * L, T(0));
+        return;
     }
-    if (sp.M > CQ_MAXM) throw Error(-1, "clique size exceeds kernel limit");
-    int nct = int((L + CQ_TW - 1) / CQ_TW), nmt = (sp.M + CQ_MT - 1) / CQ_MT;
-    int64_t nwork = int64_t(nct) * sp.nK * nmt;
+    const int gd = ints.gdim[spin];
+    const int Mp = (sp.M + 15) & ~15, Mk = (sp.M + 31) & ~31;
+    constexpr int TW64 = 16, TW32 = 32, WV64 = 8, WV32 = 8;
+    const size_t sh = sizeof(T) == 8 ? size_t(WV64) * Mp * TW64 * 8 + size_t(WV64) * Mp * 4
+                                     : ((size_t(gd) * gd + 3) & ~size_t(3)) * 4 +
+                                           size_t(WV32) * 2 * TW32 * (Mk + 8) * 2 + size_t(WV32) * Mk * 4;
+    if (sh > 150 * 1024 || sp.M > 1023) {
+        // large cliques: one-pass kernel with global atomics
+        if (sp.M > CQ_MAXM) throw Error(-1, "clique size exceeds kernel limit");
+        if (!accumulate) k_fill<T><<<nblk(sp.nstr * L, 256), 256, 0, st>>>(Y, sp.nstr * L, T(0));
+        int nct = int((L + CQ_TW - 1) / CQ_TW), nmt = (sp.M + CQ_MT - 1) / CQ_MT;
+        int64_t nwork = int64_t(nct) * sp.nK * nmt;
+        if constexpr (sizeof(T) == 8)
+            k_clique_f64<<<xcd_grid(nwork), 256, 0, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p,
+                                                          ints.gdim[spin], nct, nmt);
+        else
+            k_clique_mixed<<<xcd_grid(nwork), 256, 0, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p,
+                                                            ints.gdim[spin], nct, nmt);
+        HIPCHK(hipGetLastError());
+        return;
+    }
+    const bool reg = sp.M <= 64;  // register-resident pass 1
+    const int tw = reg ? CQ3_CW : (sizeof(T) == 8 ? TW64 : TW32);
+    const int wv = sizeof(T) == 8 ? WV64 : WV32;
+    const int64_t rowsE = sp.nK * sp.M;
+    const int64_t Lr = (L + tw - 1) / tw * tw;
+    int64_t Ls = int64_t(clique_scratch_bytes() / (size_t(rowsE) * sizeof(T))) / tw * tw;
+    Ls = std::max<int64_t>(tw, std::min<int64_t>(Ls, Lr));
+    DBuf<unsigned char>& eb = w.clique;
+    const size_t need = size_t(rowsE) * size_t(Ls) * sizeof(T);
+    if (eb.n < need) {
+        HIPCHK(hipStreamSynchronize(st));
+        eb.alloc(need);
+    }
+    T* E = reinterpret_cast<T*>(eb.p);
+    const int nct = int(Ls / tw);
+    int kch = 64;
+    while (kch > wv && int64_t(nct) * ((sp.nK + kch - 1) / kch) < 2048) kch /= 2;
+    const int nkc = int((sp.nK + kch - 1) / kch);
+    const int64_t nwork = int64_t(nct) * nkc;
     if constexpr (sizeof(T) == 8)
-        k_clique_f64<<<xcd_grid(nwork), 256, 0, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p,
-                                                      ints.gdim[spin], nct, nmt);
+        HIPCHK(hipFuncSetAttribute((const void*)k_clique2_f64<TW64, WV64, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     else
-        k_clique_mixed<<<xcd_grid(nwork), 256, 0, st>>>(X, Y, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p,
-                                                        ints.gdim[spin], nct, nmt);
-    HIPCHK(hipGetLastError());
+        HIPCHK(hipFuncSetAttribute((const void*)k_clique2_mixed<TW32, WV32, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
+    for (int64_t cbeg = 0; cbeg < L; cbeg += Ls) {
+        const int64_t ncol = std::min<int64_t>(Ls, L - cbeg);
+        if (reg) {
+            const int nchunk = int((ncol + CQ3_CW - 1) / CQ3_CW);
+            const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
+            if constexpr (sizeof(T) == 8) {
+                const int nk4 = (sp.M + 3) / 4;
+                switch (nk4) {
+#define C3(K) \
+    case K: k_clique3_f64<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd, nchunk, cbeg, Ls); break;
+                    C3(1) C3(2) C3(3) C3(4) C3(5) C3(6) C3(7) C3(8) C3(9) C3(10) C3(11) C3(12) C3(13) C3(14) C3(15) C3(16)
+#undef C3
+                    default: throw Error(-1, "clique kernel: no instance for this size");
+                }
+            } else {
+                const int nmt = (sp.M + 15) / 16;
+                switch (nmt) {
+#define C3(K) \
+    case K: k_clique3_mixed<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd, nchunk, cbeg, Ls); break;
+                    C3(1) C3(2) C3(3) C3(4)
+#undef C3
+                    default: throw Error(-1, "clique kernel: no instance for this size");
+                }
+            }
+        } else if constexpr (sizeof(T) == 8)
+            k_clique2_f64<TW64, WV64, true><<<xcd_grid(nwork), 64 * WV64, sh, st>>>(
+                X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd, nct, nkc, kch, cbeg, Ls);
+        else
+            k_clique2_mixed<TW32, WV32, true><<<xcd_grid(nwork), 64 * WV32, sh, st>>>(
+                X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd, nct, nkc, kch, cbeg, Ls);
+        HIPCHK(hipGetLastError());
+        const int64_t gw = ((sp.nstr + 3) / 4) * ((ncol + 63) / 64);
+        if (accumulate)
+            k_clique_gather<T, true><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
+                                                                   ncol, Ls);
+        else
+            k_clique_gather<T, false><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
+                                                                    ncol, Ls);
+        HIPCHK(hipGetLastError());
+    }
 }
 
 template <int KS>
 static void ab_f64_launch(const double* C, double* S, const SpinTables& a, const SpinTables& b,
                           const DevInts& ints, hipStream_t st) {
     int nmt = (b.np + 15) / 16;
-    size_t xs = size_t(KS * 4) * AB_TW * sizeof(double);
+    size_t xs = size_t(KS * 4) * AB_TW * sizeof(double) + AB_TW * sizeof(double);
     size_t row = size_t(b.nstr) * sizeof(double);
     int use_row = (xs + row) <= 150 * 1024;
     size_t sh = xs + (use_row ? row : 0);
@@ -728,7 +1278,7 @@ template <int KS>
 static void ab_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                             const DevInts& ints, hipStream_t st) {
     int nmt = (b.np + 15) / 16;
-    size_t xs = size_t(AB_TW) * (KS * 32 + 8) * 2 * sizeof(__bf16);
+    size_t xs = size_t(AB_TW) * (KS * 32 + 8) * 2 * sizeof(__bf16) + AB_TW * sizeof(float);
     size_t row = size_t(b.nstr) * sizeof(float);
     int use_row = (xs + row) <= 150 * 1024;
     size_t sh = xs + (use_row ? row : 0);
@@ -739,9 +1289,46 @@ static void ab_mixed_launch(const float* C, float* S, const SpinTables& a, const
     HIPCHK(hipGetLastError());
 }
 
+template <int KS, int TPW>
+static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, const SpinTables& b,
+                           const DevInts& ints, hipStream_t st) {
+    const size_t sh = size_t(b.nstr) * sizeof(double);
+    auto kern = k_ab2_f64<KS, TPW>;
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
+    kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
+                                            ints.vpk.p, (b.np + 15) / 16);
+    HIPCHK(hipGetLastError());
+}
+template <int KS, int TPW>
+static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
+                             const DevInts& ints, hipStream_t st) {
+    const size_t sh = size_t(b.nstr) * sizeof(float);
+    auto kern = k_ab2_mixed<KS, TPW, 2>;
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
+    kern<<<xcd_grid(a.nstr), 256, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
+                                            ints.vpk32.p, (b.np + 15) / 16);
+    HIPCHK(hipGetLastError());
+}
+
+// register-resident kernels: 16-bit targets, LDS row, E <= 96, <= 12 row tiles
+static bool ab2_ok(const SpinTables& a, const SpinTables& b, size_t w) {
+    return b.outT.p != nullptr && a.ns <= 96 && (b.np + 15) / 16 <= 12 && size_t(b.nstr) * w <= 150 * 1024;
+}
+
 static void launch_ab(const double* C, double* S, const SpinTables& a, const SpinTables& b,
                       const DevInts& ints, hipStream_t st) {
-    int ks = (a.ns + 1 + 3) / 4;
+    if (ab2_ok(a, b, sizeof(double))) {
+        const int ks = std::max(1, (a.ns + 15) / 16) * 4;  // k-steps of 4, in buckets of 16 excitations
+        const int tpw = ((b.np + 15) / 16 + 3) / 4;
+#define F64CASE(K, T) \
+    if (ks == K && tpw == T) return ab2_f64_launch<K, T>(C, S, a, b, ints, st);
+#define F64ROW(K) F64CASE(K, 1) F64CASE(K, 2) F64CASE(K, 3)
+        F64ROW(4) F64ROW(8) F64ROW(12) F64ROW(16) F64ROW(20) F64ROW(24)
+#undef F64ROW
+#undef F64CASE
+        throw Error(-1, "alpha-beta kernel: no instance for this shape");
+    }
+    int ks = std::max(1, (a.ns + 3) / 4);
     if ((b.np + 15) / 16 > 4 * AB_MTW) throw Error(-1, "too many orbitals for the alpha-beta kernel");
     switch (ks) {
 #define CASE(K) \
@@ -756,7 +1343,18 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
 }
 static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                       const DevInts& ints, hipStream_t st) {
-    int ks = (a.ns + 1 + 31) / 32;
+    if (ab2_ok(a, b, sizeof(float))) {
+        const int ks = std::max(1, (a.ns + 31) / 32);
+        const int tpw = ((b.np + 15) / 16 + 1) / 2;
+#define MXCASE(K, T) \
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T>(C, S, a, b, ints, st);
+#define MXROW(K) MXCASE(K, 1) MXCASE(K, 2) MXCASE(K, 3) MXCASE(K, 4) MXCASE(K, 5) MXCASE(K, 6)
+        MXROW(1) MXROW(2) MXROW(3)
+#undef MXROW
+#undef MXCASE
+        throw Error(-1, "alpha-beta kernel: no instance for this shape");
+    }
+    int ks = std::max(1, (a.ns + 31) / 32);
     if ((b.np + 15) / 16 > 4 * AB_MTW) throw Error(-1, "too many orbitals for the alpha-beta kernel");
     switch (ks) {
         case 1: ab_mixed_launch<1>(C, S, a, b, ints, st); break;
@@ -781,14 +1379,13 @@ template <class T> void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st) {
         if (Ct.n < size_t(nd)) Ct.alloc(nd);
         if (Yt.n < size_t(nd)) Yt.alloc(nd);
         transpose_device<T>(C, Ct.p, Na, Nb, st);
-        k_fill<T><<<nblk(nd, 256), 256, 0, st>>>(Yt.p, nd, T(0));
-        launch_clique<T>(Ct.p, Yt.p, Na, b, *ints_, 1, st);
+        launch_clique<T>(Ct.p, Yt.p, Na, b, *ints_, 1, false, *w_, st);
         transpose_add_device<T>(Yt.p, S, Nb, Na, C, ecore, st);
     } else {
         k_scale_copy<T><<<nblk(nd, 256), 256, 0, st>>>(C, S, nd, ecore);
     }
     if (timing_) HIPCHK(hipEventRecord(w_->ev[1], st));
-    if (a.d > 0) launch_clique<T>(C, S, Nb, a, *ints_, 0, st);
+    if (a.d > 0) launch_clique<T>(C, S, Nb, a, *ints_, 0, true, *w_, st);
     if (timing_) HIPCHK(hipEventRecord(w_->ev[2], st));
     if (a.k > 0 && b.k > 0) launch_ab(C, S, a, b, *ints_, st);
     if (timing_) {

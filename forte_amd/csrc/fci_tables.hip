@@ -146,7 +146,47 @@ __global__ void k_clique(uint32_t* out, int64_t nK, int n, int k, int d, int M, 
     }
 }
 
+// owners: own[I][j] = K * M + m for the C(k,d) hole strings K of string I (the d-subsets of
+// occ(I), pairs a > b in ascending (a, b) order) and the member index m of I in clique K.
+__global__ void k_clique_owner(const uint32_t* __restrict__ str, int64_t nstr, int n, int k, int d, int M,
+                               const uint32_t* __restrict__ B, uint32_t* __restrict__ own, int nown) {
+    int64_t I = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (I >= nstr) return;
+    const uint32_t s = str[I];
+    uint32_t* o = own + I * nown;
+    int j = 0;
+    for (int a = 0; a < n; ++a) {
+        if (!((s >> a) & 1u)) continue;
+        if (d == 1) {
+            o[j++] = uint32_t(a);  // K is the empty string (rank 0); member index = orbital
+            continue;
+        }
+        for (int b = 0; b < a; ++b) {
+            if (!((s >> b) & 1u)) continue;
+            const uint32_t K = s & ~((1u << a) | (1u << b));
+            // position of a and b among the unoccupied orbitals of K
+            const int pa = a - __popc(K & ((1u << a) - 1u));
+            const int pb = b - __popc(K & ((1u << b) - 1u));
+            o[j++] = d_rank(K, n, B) * uint32_t(M) + uint32_t(pa * (pa - 1) / 2 + pb);
+        }
+    }
+    (void)k;
+}
+
 static inline unsigned nblk(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+__global__ void k_out_t16(const uint32_t* __restrict__ out, int64_t nstr, int np, int npp, uint16_t* __restrict__ o16) {
+    const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= nstr * npp) return;
+    const int64_t I = x / npp;
+    const int P = int(x % npp);
+    uint16_t v = 0x7FFFu;
+    if (P < np) {
+        const uint32_t t = out[int64_t(P) * nstr + I];
+        if (t != kInvalid) v = uint16_t((t & kAddrMask) | ((t >> 31) << 15));
+    }
+    o16[x] = v;
+}
 
 void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream_t st) {
     t.n = n;
@@ -163,12 +203,21 @@ void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream
     t.np = n * (n + 1) / 2;
     t.out.alloc(size_t(t.nstr) * t.np);
     k_out<<<nblk(t.nstr * t.np, 256), 256, 0, st>>>(t.str.p, t.out.p, t.nstr, n, t.np, B);
+    if (t.nstr <= 32767) {
+        t.npp = (t.np + 15) / 16 * 16;
+        t.outT.alloc(size_t(t.nstr) * t.npp);
+        k_out_t16<<<nblk(t.nstr * t.npp, 256), 256, 0, st>>>(t.out.p, t.nstr, t.np, t.npp, t.outT.p);
+    }
     t.d = k >= 2 ? 2 : (k == 1 ? 1 : 0);
     if (t.d > 0) {
         t.nK = int64_t(binom_host(n, k - t.d));
         t.M = int(binom_host(n - k + t.d, t.d));
         t.clq.alloc(size_t(t.nK) * t.M);
         k_clique<<<nblk(t.nK, 64), 64, 0, st>>>(t.clq.p, t.nK, n, k, t.d, t.M, B);
+        t.nown = int(binom_host(k, t.d));
+        if (uint64_t(t.nK) * uint64_t(t.M) > 0xFFFFFFFFull) throw Error(-1, "clique table too large");
+        t.own.alloc(size_t(t.nstr) * t.nown);
+        k_clique_owner<<<nblk(t.nstr, 128), 128, 0, st>>>(t.str.p, t.nstr, n, k, t.d, t.M, B, t.own.p, t.nown);
     }
     HIPCHK(hipGetLastError());
 }
