@@ -78,6 +78,7 @@ def main():
 
     import numpy as np
     import forte_amd
+    import forte_amd.learned
 
     n = args.n
     na = nb = n // 2
@@ -149,34 +150,43 @@ def main():
 
     # ---- time to solution ------------------------------------------------------------------------
     if args.solve:
+        legs = [("fp64", "diagonal"), ("mixed", "learned"), ("mixed", "diagonal")]
+        lp_w = None
+        if os.path.exists(forte_amd.learned.WEIGHTS):
+            lp_w = forte_amd.learned.load_weights()
+        else:
+            legs = [leg for leg in legs if leg[1] != "learned"]
         tts = {}
-        energies = {}
-        for prec in ("fp64", "mixed"):
+        for prec, pc in legs:
             s = forte_amd.FCISolver(na, nb, 1, 1, ints)
             s.set_e_convergence(1e-9)
             s.set_r_convergence(1e-4)
             s.set_maxiter(args.maxiter)
             s.set_die_if_not_converged(False)
             s.set_precision(prec)
+            if pc == "learned":
+                s.set_lp_weights(lp_w)
+                s.set_preconditioner("learned")
             forte_amd.device_synchronize()
             t0 = time.perf_counter()
             e = s.compute_energy()
             forte_amd.device_synchronize()
-            tts[prec] = time.perf_counter() - t0
-            energies[prec] = e
+            dt = time.perf_counter() - t0
             st = s.stats()
-            print(f"# solve {prec}: E={e:.12f} {tts[prec]:.2f}s {st}", file=sys.stderr, flush=True)
-            tts[prec + "_stats"] = {k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
-                                                        "converged")}
+            print(f"# solve {prec}+{pc}: E={e:.12f} {dt:.2f}s {st}", file=sys.stderr, flush=True)
+            tts[f"{prec}+{pc}"] = dict(seconds=round(dt, 3), energy=e,
+                                       **{k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
+                                                             "converged", "fell_back")})
             del s
+        ref = tts["fp64+diagonal"]
+        best = tts.get("mixed+learned", tts["mixed+diagonal"])
         result["time_to_solution"] = {
-            "fp64_s": round(tts["fp64"], 3), "mixed_s": round(tts["mixed"], 3),
-            "speedup": round(tts["fp64"] / tts["mixed"], 3),
-            "abs_dE_mixed_vs_fp64": abs(energies["mixed"] - energies["fp64"]),
-            "energy_fp64": energies["fp64"],
+            "runs": tts,
+            "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
+            "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
+            "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
             "thresholds": {"e_conv": 1e-9, "r_conv": 1e-4, "maxiter": args.maxiter},
-            "fp64_stats": tts["fp64_stats"], "mixed_stats": tts["mixed_stats"],
-            "preconditioner": "diagonal"}
+            "learned_weights": forte_amd.learned.WEIGHTS if lp_w is not None else None}
 
     if args.cpu:
         result["cpu_baseline"] = cpu_baseline(n, h, eri, args.cpu_stride)

@@ -26,6 +26,10 @@ RDMsType = _ext.RDMsType
 seeded_integrals = _ext.seeded_integrals
 DeviceArray = _ext.DeviceArray
 device_synchronize = _ext.device_synchronize
+lp_forward = _ext.lp_forward
+lp_loss_grad = _ext.lp_loss_grad
+lp_train = _ext.lp_train
+LP_NPARAM = _ext.LP_NPARAM
 
 LIBRARY_PATH = os.path.join(_HERE, "_lib", "libforte_fci_gfx950.so")
 
@@ -60,11 +64,19 @@ def make_active_space_method(method, na, nb, multiplicity, nroot, as_ints, optio
     s = FCISolver(na, nb, multiplicity, nroot, as_ints)
     if options:
         s.set_options(dict(options))
+        if str(options.get("FCI_PRECONDITIONER", "diagonal")).lower() == "learned":
+            from .learned import load_weights
+            s.set_lp_weights(load_weights(options.get("FCI_LP_WEIGHTS", None) or _default_lp_weights()))
     return s
+
+
+def _default_lp_weights():
+    return os.path.join(_HERE, "data", "lp_weights.npy")
 
 
 __all__ = [
     "ActiveSpaceIntegrals", "FCISolver", "DavidsonLiuSolver", "RDMs", "RDMsType",
     "seeded_integrals", "DeviceArray", "device_synchronize", "make_active_space_integrals", "integrals_from_fcidump",
-    "make_active_space_method", "read_fcidump", "chem_to_forte", "LIBRARY_PATH",
+    "make_active_space_method", "read_fcidump", "chem_to_forte", "LIBRARY_PATH", "lp_forward", "lp_loss_grad",
+    "lp_train", "LP_NPARAM",
 ]

@@ -48,6 +48,8 @@ class DavidsonLiu {
     int iterations() const { return iters_; }
     int sigma_builds() const { return nsigma_; }
     const std::vector<double>& residual_norms() const { return resid_; }
+    // Ritz vector of root 0 of the current iteration (valid inside the preconditioner callback)
+    void ritz_vector0(double* out) { rotate(b_.p, out, 1); }
     int basis_size() const { return basis_size_; }
 
   private:

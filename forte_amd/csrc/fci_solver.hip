@@ -7,6 +7,7 @@
 #include "../../include/forte_fci.h"
 #include "fci_dl.h"
 #include "fci_internal.h"
+#include "fci_lp.h"
 
 #include <chrono>
 #include <cmath>
@@ -255,6 +256,7 @@ static void check_device() {
     });
     if (g_dev_status) throw Error(g_dev_status, g_dev_msg);
 }
+void check_device_public() { check_device(); }
 
 class Solver {
   public:
@@ -365,6 +367,7 @@ class Solver {
     bool have_ints_ = false;
     std::unique_ptr<SigmaEngine> eng_;
     std::unique_ptr<DavidsonLiu> dl_;
+    LearnedPrecond lp_;
     DBuf<double> hdiag_;
     DBuf<float> c32_, s32_;
     std::vector<double> energies_, spin2_;
@@ -383,6 +386,7 @@ class Solver {
     }
     std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess(int num_guess_states);
     double compute_energy();
+    int64_t lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, float* X, float* d0, float* e);
 };
 
 __global__ void k_f64_to_f32(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
@@ -489,8 +493,29 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(
     return {guesses, bad};
 }
 
+// Learned preconditioner inside the Davidson-Liu loop, with the diagonal one taking over for the
+// rest of the solve once the largest residual norm has failed to improve on its best value for
+// two consecutive iterations.
+struct LPRun {
+    bool fell_back = false;
+    int stalled = 0;
+    double best = 1e300;
+    void observe(const std::vector<double>& r) {
+        double mx = 0.0;
+        for (double x : r) mx = std::max(mx, x);
+        if (mx < best) {
+            best = mx;
+            stalled = 0;
+        } else if (++stalled >= 2) {
+            fell_back = true;
+        }
+    }
+};
+
 double Solver::compute_energy() {
     if (!have_ints_) throw Error(-2, "integrals have not been set");
+    const bool learned = opt_.preconditioner == FCI_PRECOND_LEARNED;
+    if (learned && !lp_.ready()) throw Error(-2, "learned preconditioner weights have not been set");
     auto t0 = std::chrono::steady_clock::now();
     const int nroot = opt_.nroot;
     bool first_run = false;
@@ -511,6 +536,21 @@ double Solver::compute_energy() {
     }
     int nsig64 = 0, nsig = 0, iters = 0;
     bool converged = false;
+    auto lpr = std::make_shared<LPRun>();
+    if (learned) {
+        dl_->set_preconditioner([this, lpr](double* r, const double* lam, int nr, int) {
+            if (lpr->fell_back) return false;
+            const auto& rn = dl_->residual_norms();
+            for (int k = 0; k < nr; ++k) lp_.apply(r + size_t(k) * ndet_, hdiag_.p, lam[k], rn[k], sa_, sb_, 0);
+            return true;
+        });
+    } else {
+        dl_->set_preconditioner(nullptr);
+    }
+    auto lp_observer = [lpr, learned](int, const std::vector<double>& r, const std::vector<double>&) {
+        if (learned) lpr->observe(r);
+        return false;
+    };
     if (opt_.precision == FCI_PREC_MIXED) {
         // mixed phase: FP32 sigma until the residual reaches mixed_switch or stalls
         dl_->add_sigma_builder([this](const double* b, double* s) { sigma_mixed(b, s); });
@@ -519,7 +559,9 @@ double Solver::compute_energy() {
         dl_->set_maxiter(opt_.maxiter);
         // end the mixed phase when the largest residual has not dropped by 20% in 2 iterations
         auto hist = std::make_shared<std::vector<double>>();
-        dl_->set_iteration_observer([hist](int, const std::vector<double>& r, const std::vector<double>&) {
+        dl_->set_iteration_observer([hist, lp_observer](int it, const std::vector<double>& r,
+                                                        const std::vector<double>& l) {
+            lp_observer(it, r, l);
             double mx = 0.0;
             for (double x : r) mx = std::max(mx, x);
             hist->push_back(mx);
@@ -527,7 +569,6 @@ double Solver::compute_energy() {
             return k >= 3 && (*hist)[k - 1] > 0.8 * (*hist)[k - 3];
         });
         dl_->solve();
-        dl_->set_iteration_observer(nullptr);
         nsig += dl_->sigma_builds();
         iters += dl_->iterations();
     }
@@ -536,7 +577,11 @@ double Solver::compute_energy() {
     dl_->set_e_convergence(opt_.e_convergence);
     dl_->set_r_convergence(opt_.r_convergence);
     dl_->set_maxiter(opt_.maxiter);
+    dl_->set_iteration_observer(lp_observer);
     converged = dl_->solve();
+    dl_->set_iteration_observer(nullptr);
+    dl_->set_preconditioner(nullptr);
+    stat_fellback_ = learned && lpr->fell_back;
     nsig64 = dl_->sigma_builds();
     nsig += nsig64;
     iters += dl_->iterations();
@@ -560,6 +605,79 @@ double Solver::compute_energy() {
     stat_sec_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (opt_.root < 0 || opt_.root >= nroot) throw Error(-1, "root index out of range");
     return energies_[opt_.root];
+}
+
+// Training data for the learned preconditioner from an FP64 solve with the diagonal
+// preconditioner: at each of the first max_groups iterations, per_iter random determinants of
+// root 0 give (features, r / (lambda - H), exact correction c / (c.x) - x), where x is the Ritz
+// vector of that iteration and c the converged eigenvector.
+int64_t Solver::lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, float* X, float* d0, float* e) {
+    if (!have_ints_) throw Error(-2, "integrals have not been set");
+    if (per_iter < 1 || max_groups < 1) throw Error(-1, "per_iter and max_groups must be positive");
+    struct Rec {
+        DBuf<double> x, xs;
+        DBuf<int64_t> idx;
+        DBuf<float> X, d0;
+    };
+    auto recs = std::make_shared<std::vector<std::unique_ptr<Rec>>>();
+    const int nroot = opt_.nroot;
+    dl_.reset(new DavidsonLiu(ndet_, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
+    dl_->set_print_level(opt_.print);
+    compute_hdiag();
+    dl_->add_h_diag_device(hdiag_.p, ndet_);
+    dl_->reset();
+    {
+        const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, ndet_));
+        auto [g, bad] = initial_guess(num_guess_states);
+        dl_->add_guesses(g);
+        dl_->add_project_out_vectors(bad);
+    }
+    uint64_t rng = seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+    auto next = [&rng]() {
+        uint64_t z = (rng += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    dl_->set_preconditioner([&, recs](double* r, const double* lam, int, int) {
+        if (int64_t(recs->size()) >= max_groups) return false;
+        auto rec = std::make_unique<Rec>();
+        rec->x.alloc(ndet_);
+        dl_->ritz_vector0(rec->x.p);
+        std::vector<int64_t> idx(per_iter);
+        for (auto& v : idx) v = int64_t(next() % uint64_t(ndet_));
+        rec->idx.alloc(per_iter);
+        rec->idx.upload(idx.data(), per_iter);
+        rec->X.alloc(size_t(per_iter) * LP_NIN);
+        rec->d0.alloc(per_iter);
+        rec->xs.alloc(per_iter);
+        lp_features_device(sa_, sb_, rec->idx.p, per_iter, hdiag_.p, lam[0], r, dl_->residual_norms()[0], rec->X.p, 0);
+        lp_sample_device(rec->idx.p, per_iter, hdiag_.p, lam[0], r, rec->x.p, rec->d0.p, rec->xs.p, 0);
+        recs->push_back(std::move(rec));
+        return false;  // the diagonal preconditioner is applied by the solver
+    });
+    dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
+    dl_->set_e_convergence(opt_.e_convergence);
+    dl_->set_r_convergence(opt_.r_convergence);
+    dl_->set_maxiter(opt_.maxiter);
+    const bool conv = dl_->solve();
+    dl_->set_preconditioner(nullptr);
+    if (!conv) throw Error(-4, "Davidson-Liu solver did not converge while harvesting");
+    const double* c = dl_->eigenvector_device(0);
+    DBuf<float> de(per_iter);
+    int64_t ng = 0;
+    for (auto& rec : *recs) {
+        const double cx = dot_device(c, rec->x.p, ndet_, 0);
+        if (std::fabs(cx) < 1e-8) continue;  // iterate orthogonal to the solution: no usable target
+        lp_target_device(rec->idx.p, per_iter, c, cx, rec->xs.p, de.p, 0);
+        rec->X.download(X + size_t(ng) * per_iter * LP_NIN, size_t(per_iter) * LP_NIN);
+        rec->d0.download(d0 + size_t(ng) * per_iter, per_iter);
+        de.download(e + size_t(ng) * per_iter, per_iter);
+        ++ng;
+    }
+    HIPCHK(hipDeviceSynchronize());
+    dl_.reset();  // the next compute_energy starts from a fresh guess
+    return ng;
 }
 
 }  // namespace fci
@@ -1018,6 +1136,121 @@ int fci_dl_eigenvector(const fci_dl* d, int k, double* out) {
 // SplitMix64(seed); u = ((x >> 11) + 0.5) 2^-53; N = sqrt(-2 ln u1) cos(2 pi u2);
 // h_pp = -2 + 1.5 p/(n-1); h_pq = 0.05 N (p<q, row-major); L^K_pq = 0.1 N (p<=q), L^K_pp += 0.5,
 // K = 0..2n-1; (pq|rs) = sum_K L^K_pq L^K_rs.
+int fci_set_lp_weights(fci_solver* s, const float* params) {
+    if (!s || !params) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] { s->s->lp_.set_weights(params); });
+}
+
+int fci_lp_precondition(fci_solver* s, const double* r, double lambda, double* out) {
+    if (!s || !r || !out) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (!S.have_ints_) throw Error(FCI_ERR_STATE, "integrals have not been set");
+        S.compute_hdiag();
+        fci::DBuf<double> d(S.ndet_);
+        d.upload(r, S.ndet_);
+        double nr = 0.0;
+        for (int64_t i = 0; i < S.ndet_; ++i) nr += r[i] * r[i];
+        S.lp_.apply(d.p, S.hdiag_.p, lambda, std::sqrt(nr), S.sa_, S.sb_, 0);
+        HIPCHK(hipDeviceSynchronize());
+        d.download(out, S.ndet_);
+    });
+}
+
+int fci_lp_features(fci_solver* s, const int64_t* idx, int64_t m, double lambda, const double* r, float* X) {
+    if (!s || !idx || !r || !X || m < 0) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (!S.have_ints_) throw Error(FCI_ERR_STATE, "integrals have not been set");
+        for (int64_t i = 0; i < m; ++i)
+            if (idx[i] < 0 || idx[i] >= S.ndet_) throw Error(FCI_ERR_ARG, "determinant index out of range");
+        if (m == 0) return;
+        S.compute_hdiag();
+        fci::DBuf<double> d(S.ndet_);
+        d.upload(r, S.ndet_);
+        double nr = 0.0;
+        for (int64_t i = 0; i < S.ndet_; ++i) nr += r[i] * r[i];
+        fci::DBuf<int64_t> di(m);
+        di.upload(idx, m);
+        fci::DBuf<float> dx(size_t(m) * fci::LP_NIN);
+        fci::lp_features_device(S.sa_, S.sb_, di.p, m, S.hdiag_.p, lambda, d.p, std::sqrt(nr), dx.p, 0);
+        HIPCHK(hipDeviceSynchronize());
+        dx.download(X, size_t(m) * fci::LP_NIN);
+    });
+}
+
+int fci_lp_harvest(fci_solver* s, int64_t per_iter, int64_t max_groups, uint64_t seed, float* X, float* d0,
+                   float* e, int64_t* ngroups) {
+    if (!s || !X || !d0 || !e || !ngroups) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] { *ngroups = s->s->lp_harvest(per_iter, max_groups, seed, X, d0, e); });
+}
+
+int fci_lp_forward(const float* params, const float* X, int64_t m, float* y) {
+    if (!params || !X || !y || m < 1) return FCI_ERR_ARG;
+    return guarded(nullptr, [&] {
+        fci::check_device_public();
+        fci::LPTrainer t(m);
+        t.set_params(params);
+        fci::DBuf<float> dx(size_t(m) * fci::LP_NIN);
+        dx.upload(X, size_t(m) * fci::LP_NIN);
+        t.forward(dx.p, m, 0);
+        HIPCHK(hipDeviceSynchronize());
+        t.get_y(y, m);
+    });
+}
+
+int fci_lp_loss_grad(const float* params, const float* X, const float* d0, const float* e, int64_t m, float reg,
+                     float* loss, float* grad) {
+    if (!params || !X || !d0 || !e || !loss || !grad || m < 1) return FCI_ERR_ARG;
+    return guarded(nullptr, [&] {
+        fci::check_device_public();
+        fci::LPTrainer t(m);
+        t.set_params(params);
+        fci::DBuf<float> dx(size_t(m) * fci::LP_NIN), dd(m), de(m);
+        dx.upload(X, size_t(m) * fci::LP_NIN);
+        dd.upload(d0, m);
+        de.upload(e, m);
+        *loss = t.loss_grad(dx.p, dd.p, de.p, m, reg, 0);
+        t.get_grad(grad);
+    });
+}
+
+int fci_lp_train(float* params, const float* X, const float* d0, const float* e, int64_t ngroups, int64_t group,
+                 int epochs, float lr, float reg, uint64_t seed, float* epoch_loss) {
+    if (!params || !X || !d0 || !e || ngroups < 1 || group < 1 || epochs < 0) return FCI_ERR_ARG;
+    return guarded(nullptr, [&] {
+        fci::check_device_public();
+        const size_t ns = size_t(ngroups) * group;
+        fci::LPTrainer t(group);
+        t.set_params(params);
+        fci::DBuf<float> dx(ns * fci::LP_NIN), dd(ns), de(ns);
+        dx.upload(X, ns * fci::LP_NIN);
+        dd.upload(d0, ns);
+        de.upload(e, ns);
+        std::vector<int64_t> order(ngroups);
+        for (int64_t i = 0; i < ngroups; ++i) order[i] = i;
+        uint64_t st = seed;
+        auto next = [&st]() {
+            uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            return z ^ (z >> 31);
+        };
+        for (int ep = 0; ep < epochs; ++ep) {
+            for (int64_t i = ngroups - 1; i > 0; --i) std::swap(order[i], order[next() % uint64_t(i + 1)]);
+            double tot = 0.0;
+            for (int64_t gi : order) {
+                const size_t o = size_t(gi) * group;
+                tot += t.loss_grad(dx.p + o * fci::LP_NIN, dd.p + o, de.p + o, group, reg, 0);
+                t.adam(lr, 0);
+            }
+            if (epoch_loss) epoch_loss[ep] = float(tot / double(ngroups));
+        }
+        HIPCHK(hipDeviceSynchronize());
+        t.get_params(params);
+    });
+}
+
 void fci_seeded_integrals(int n, uint64_t seed, double* h, double* eri) {
     uint64_t st = seed;
     auto next = [&st]() {

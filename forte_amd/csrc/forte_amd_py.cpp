@@ -10,6 +10,7 @@
 // C++ exceptions do through pybind11.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+#include <cstring>
 #include <pybind11/stl.h>
 
 #include <memory>
@@ -161,6 +162,12 @@ class FCISolver {
         push();
     }
     void set_mixed_switch(double v) { o_.mixed_switch = v; push(); }
+    void set_preconditioner(const std::string& p) {
+        if (p == "diagonal") o_.preconditioner = FCI_PRECOND_DIAGONAL;
+        else if (p == "learned") o_.preconditioner = FCI_PRECOND_LEARNED;
+        else throw std::runtime_error("preconditioner must be 'diagonal' or 'learned'");
+        push();
+    }
     // FCISolver::set_options (fci_solver.cc:156-172) from a dict of Forte option names
     void set_options(const py::dict& d) {
         for (auto item : d) {
@@ -180,6 +187,10 @@ class FCISolver {
             } else if (k == "FCI_TEST_RDMS") {
                 // the brute-force RDM self test is the test suite's job here
             } else if (k == "FCI_PRECISION") set_precision(v.cast<std::string>());
+            else if (k == "FCI_PRECONDITIONER") set_preconditioner(v.cast<std::string>());
+            else if (k == "FCI_LP_WEIGHTS") {
+                // a weights file path, loaded by forte_amd.make_active_space_method
+            }
             else throw std::runtime_error("unknown FCI option " + k);
         }
         push();
@@ -311,6 +322,45 @@ class FCISolver {
         if (C.size() != fci_ndet(h_)) throw std::runtime_error("spin2_of: wrong vector size");
         check(fci_spin2_of(h_, C.data(), &s), fci_last_error(h_));
         return s;
+    }
+    // ---- learned preconditioner ----
+    void set_lp_weights(py::array_t<float, py::array::c_style | py::array::forcecast> p) {
+        if (p.size() != FCI_LP_NPARAM) throw std::runtime_error("expected FCI_LP_NPARAM parameters");
+        check(fci_set_lp_weights(h_, p.data()), fci_last_error(h_));
+    }
+    py::array_t<double> lp_precondition(py::array_t<double, py::array::c_style | py::array::forcecast> r,
+                                        double lambda) {
+        int64_t nd = fci_ndet(h_);
+        if (r.size() != nd) throw std::runtime_error("residual has the wrong length");
+        py::array_t<double> out(nd);
+        check(fci_lp_precondition(h_, r.data(), lambda, out.mutable_data()), fci_last_error(h_));
+        return out;
+    }
+    py::array_t<float> lp_features(py::array_t<int64_t, py::array::c_style | py::array::forcecast> idx,
+                                   double lambda, py::array_t<double, py::array::c_style | py::array::forcecast> r) {
+        if (r.size() != fci_ndet(h_)) throw std::runtime_error("residual has the wrong length");
+        int64_t m = idx.size();
+        py::array_t<float> X({m, int64_t(FCI_LP_NFEAT)});
+        check(fci_lp_features(h_, idx.data(), m, lambda, r.data(), X.mutable_data()), fci_last_error(h_));
+        return X;
+    }
+    py::tuple lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed) {
+        if (per_iter < 1 || max_groups < 1) throw std::runtime_error("per_iter and max_groups must be positive");
+        const int64_t ns = per_iter * max_groups;
+        std::vector<float> X(size_t(ns) * FCI_LP_NFEAT), d0(ns), e(ns);
+        int64_t ng = 0;
+        int st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_lp_harvest(h_, per_iter, max_groups, seed, X.data(), d0.data(), e.data(), &ng);
+        }
+        check(st, fci_last_error(h_));
+        const int64_t m = ng * per_iter;
+        py::array_t<float> aX({m, int64_t(FCI_LP_NFEAT)}), ad(m), ae(m);
+        std::memcpy(aX.mutable_data(), X.data(), size_t(m) * FCI_LP_NFEAT * sizeof(float));
+        std::memcpy(ad.mutable_data(), d0.data(), size_t(m) * sizeof(float));
+        std::memcpy(ae.mutable_data(), e.data(), size_t(m) * sizeof(float));
+        return py::make_tuple(aX, ad, ae);
     }
     py::array_t<double> hdiag() {
         py::array_t<double> a(fci_ndet(h_));
@@ -484,6 +534,10 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("sigma_arrays", &FCISolver::sigma_arrays)
         .def("spin2_of", &FCISolver::spin2_of)
         .def("hdiag", &FCISolver::hdiag)
+        .def("set_lp_weights", &FCISolver::set_lp_weights)
+        .def("lp_precondition", &FCISolver::lp_precondition)
+        .def("lp_features", &FCISolver::lp_features)
+        .def("lp_harvest", &FCISolver::lp_harvest, py::arg("per_iter"), py::arg("max_groups"), py::arg("seed") = 0)
         .def("strings", &FCISolver::strings)
         .def("vo_list", &FCISolver::vo_list)
         .def("oo_list", &FCISolver::oo_list)
@@ -501,7 +555,8 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("set_collapse_per_root", &FCISolver::set_collapse_per_root)
         .def("set_subspace_per_root", &FCISolver::set_subspace_per_root)
         .def("set_precision", &FCISolver::set_precision)
-        .def("set_mixed_switch", &FCISolver::set_mixed_switch);
+        .def("set_mixed_switch", &FCISolver::set_mixed_switch)
+        .def("set_preconditioner", &FCISolver::set_preconditioner);
 
     py::class_<DeviceArray, std::shared_ptr<DeviceArray>>(m, "DeviceArray")
         .def(py::init<int64_t, const std::string&>(), py::arg("n"), py::arg("dtype") = "float64")
@@ -530,6 +585,44 @@ PYBIND11_MODULE(_forte_amd, m) {
         py::gil_scoped_release nogil;
         return fci_device_synchronize() == FCI_OK;
     });
+    using farr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+    m.attr("LP_NPARAM") = FCI_LP_NPARAM;
+    m.def("lp_forward", [](farr p, farr X) {
+        if (p.size() != FCI_LP_NPARAM || X.ndim() != 2 || X.shape(1) != FCI_LP_NFEAT)
+            throw std::runtime_error("lp_forward: bad shapes");
+        int64_t m = X.shape(0);
+        py::array_t<float> y(m);
+        check(fci_lp_forward(p.data(), X.data(), m, y.mutable_data()), fci_last_error(nullptr));
+        return y;
+    });
+    m.def("lp_loss_grad", [](farr p, farr X, farr d0, farr e, float reg) {
+        if (p.size() != FCI_LP_NPARAM || X.ndim() != 2 || X.shape(1) != FCI_LP_NFEAT || d0.size() != X.shape(0) ||
+            e.size() != X.shape(0))
+            throw std::runtime_error("lp_loss_grad: bad shapes");
+        float loss = 0.f;
+        py::array_t<float> g(FCI_LP_NPARAM);
+        check(fci_lp_loss_grad(p.data(), X.data(), d0.data(), e.data(), X.shape(0), reg, &loss, g.mutable_data()),
+              fci_last_error(nullptr));
+        return py::make_tuple(loss, g);
+    }, py::arg("params"), py::arg("X"), py::arg("d0"), py::arg("e"), py::arg("reg") = 0.f);
+    m.def("lp_train", [](farr p, farr X, farr d0, farr e, int64_t group, int epochs, float lr, float reg,
+                         uint64_t seed) {
+        if (p.size() != FCI_LP_NPARAM || X.ndim() != 2 || X.shape(1) != FCI_LP_NFEAT || d0.size() != X.shape(0) ||
+            e.size() != X.shape(0) || group < 1 || X.shape(0) % group != 0)
+            throw std::runtime_error("lp_train: bad shapes (samples must be a multiple of the group size)");
+        py::array_t<float> out(FCI_LP_NPARAM);
+        std::memcpy(out.mutable_data(), p.data(), sizeof(float) * FCI_LP_NPARAM);
+        py::array_t<float> losses(std::max(epochs, 0));
+        int st;
+        {
+            py::gil_scoped_release nogil;
+            st = fci_lp_train(out.mutable_data(), X.data(), d0.data(), e.data(), X.shape(0) / group, group, epochs, lr,
+                              reg, seed, losses.mutable_data());
+        }
+        check(st, fci_last_error(nullptr));
+        return py::make_tuple(out, losses);
+    }, py::arg("params"), py::arg("X"), py::arg("d0"), py::arg("e"), py::arg("group"), py::arg("epochs"),
+       py::arg("lr") = 1e-3f, py::arg("reg") = 1e-4f, py::arg("seed") = 0);
     m.def("seeded_integrals", [](int n, uint64_t seed) {
         std::vector<double> h(size_t(n) * n), eri(size_t(n) * n * n * n);
         fci_seeded_integrals(n, seed, h.data(), eri.data());

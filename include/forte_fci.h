@@ -165,6 +165,39 @@ int fci_device_synchronize(void);
 int fci_sigma_bench(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, int nrep,
                     double* ms_per_sigma, double* ms3);
 
+/* ---- learned preconditioner ---------------------------------------------------------------
+ * No counterpart in the reference: an MLP (34 -> 128 -> 128 -> 1, GELU) that scales Forte's
+ * diagonal Davidson correction r / (lambda - H_II) (davidson_liu_solver.cc:432-448) element-wise
+ * by exp(MLP(x)); y = 0 reproduces the reference. Parameters are FCI_LP_NPARAM floats in the
+ * order W1[128][34], b1[128], W2[128][128], b2[128], w3[128], b3. Used when fci_options
+ * .preconditioner = FCI_PRECOND_LEARNED; the diagonal preconditioner takes over for the rest of a
+ * solve after two iterations without a new lowest residual norm (fci_solve_stats: fell_back). */
+#define FCI_LP_NPARAM 21121
+#define FCI_LP_NFEAT 34
+int fci_set_lp_weights(fci_solver* s, const float* params);
+/* out = learned correction of the raw residual r (host, ndet) at eigenvalue estimate lambda,
+ * computed by the BF16 inference kernel with the solver's H diagonal. */
+int fci_lp_precondition(fci_solver* s, const double* r, double lambda, double* out);
+/* features X[m][34] of determinants idx[m] for residual r (host, ndet) and lambda */
+int fci_lp_features(fci_solver* s, const int64_t* idx, int64_t m, double lambda, const double* r, float* X);
+/* Training data from an FP64 solve with the diagonal preconditioner (current options): for each of
+ * the first max_groups iterations, per_iter random determinants (SplitMix64 stream from seed) give
+ * X (per_iter x 34), d0 = r / (lambda - H) and e = c / (c.x) - x (x: that iteration's Ritz vector
+ * of root 0, c: the converged eigenvector). Buffers hold max_groups * per_iter samples; *ngroups
+ * receives the number written. */
+int fci_lp_harvest(fci_solver* s, int64_t per_iter, int64_t max_groups, uint64_t seed, float* X, float* d0,
+                   float* e, int64_t* ngroups);
+/* FP32 device MLP of the training path: y[m] = MLP(X) */
+int fci_lp_forward(const float* params, const float* X, int64_t m, float* y);
+/* loss = 1 - (d.e)^2 / ((d.d)(e.e)) + reg mean(y^2) with d = d0 exp(y), and d loss / d params */
+int fci_lp_loss_grad(const float* params, const float* X, const float* d0, const float* e, int64_t m, float reg,
+                     float* loss, float* grad);
+/* Adam training on the device over ngroups groups of `group` samples (one group per step, groups
+ * shuffled each epoch from seed); params is read and overwritten; epoch_loss[epochs] receives the
+ * mean loss of each epoch. */
+int fci_lp_train(float* params, const float* X, const float* d0, const float* e, int64_t ngroups, int64_t group,
+                 int epochs, float lr, float reg, uint64_t seed, float* epoch_loss);
+
 /* ---- seeded integrals of the benchmark configurations (host) ----------------------------- */
 /* h (n*n) and chemist ERIs (pq|rs) (n^4) from a SplitMix64 stream; see fci_solver.hip. */
 void fci_seeded_integrals(int n, uint64_t seed, double* h, double* eri);

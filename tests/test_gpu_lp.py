@@ -1,0 +1,191 @@
+"""GPU checks of the learned preconditioner against the CPU model (oracle/lp_oracle.py): FP32
+training forward pass and gradients, features, one Adam step, the BF16 inference kernel, the
+harvest of training data, and solves with the learned preconditioner."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import seeded_problem
+from oracle import lp_oracle as L
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _params(seed, scale=0.2):
+    rng = np.random.default_rng(seed)
+    p = L.init_params(seed)
+    p["w3"] = rng.standard_normal(L.NH) * scale
+    p["b3"] = np.array([0.05])
+    p["b1"] = rng.standard_normal(L.NH) * 0.1
+    p["b2"] = rng.standard_normal(L.NH) * 0.1
+    return p
+
+
+def _data(m, seed):
+    rng = np.random.default_rng(seed)
+    X = np.concatenate([rng.uniform(0, 3, (m, 1)), rng.standard_normal((m, 1)),
+                        rng.integers(0, 2, (m, 32)).astype(float)], axis=1)
+    return X, rng.standard_normal(m), rng.uniform(0.5, 4.0, m), rng.standard_normal(m)
+
+
+def _solver(n, seed, nroot=1):
+    import forte_amd
+    p = seeded_problem(n, n // 2, n // 2, seed)
+    ints = forte_amd.ActiveSpaceIntegrals(n, p["h"], p["h"], p["aa"], p["ab"], p["aa"], 0.0)
+    return forte_amd.FCISolver(n // 2, n // 2, 1, nroot, ints), p
+
+
+def test_forward_matches_oracle():
+    import forte_amd
+    say("lp forward")
+    p = _params(1)
+    X, *_ = _data(1000, 2)
+    y = forte_amd.lp_forward(L.flatten(p).astype(np.float32), X.astype(np.float32))
+    yo, _ = L.forward(p, X.astype(np.float32).astype(np.float64))
+    assert np.abs(y - yo).max() < 2e-5 * (1 + np.abs(yo).max())
+
+
+@pytest.mark.parametrize("m", [1, 63, 3001])
+def test_loss_and_gradients_match_oracle(m):
+    import forte_amd
+    say("lp grad", m)
+    p = _params(3)
+    X, r, hd, e = _data(m, 4)
+    lam = 0.2
+    d0 = np.where(np.abs(lam - hd) > 1e-6, r / (lam - hd), 0.0)
+    f32 = lambda a: np.asarray(a, dtype=np.float32)
+    loss, g = forte_amd.lp_loss_grad(f32(L.flatten(p)), f32(X), f32(d0), f32(e), 1e-3)
+    # oracle on the same FP32-rounded inputs
+    P = L.unflatten(f32(L.flatten(p)).astype(np.float64))
+    lo, go = L.loss_and_grads(P, f32(X).astype(np.float64), f32(d0).astype(np.float64) * (lam - hd), hd, lam,
+                              f32(e).astype(np.float64), reg=1e-3)
+    assert abs(loss - lo) < 1e-5 + 1e-4 * abs(lo)
+    gv = L.flatten(go)
+    scale = np.abs(gv).max()
+    assert np.abs(g - gv).max() < 2e-4 * scale, np.abs(g - gv).max() / scale
+
+
+def test_adam_step_matches_oracle():
+    import forte_amd
+    say("lp adam")
+    p = _params(5)
+    G = 256
+    X, r, hd, e = _data(G, 6)
+    d0 = r / (0.1 - hd)
+    f32 = lambda a: np.asarray(a, dtype=np.float32)
+    # reg > 0: with reg = 0 the b3 gradient vanishes identically (the loss ignores a constant y)
+    # and Adam's first step on it would only see rounding noise
+    out, losses = forte_amd.lp_train(f32(L.flatten(p)), f32(X), f32(d0), f32(e), group=G, epochs=1, lr=1e-3,
+                                     reg=1e-2, seed=1)
+    P = L.unflatten(f32(L.flatten(p)).astype(np.float64))
+    lo, go = L.loss_and_grads(P, f32(X).astype(np.float64), f32(d0).astype(np.float64) * (0.1 - hd), hd, 0.1,
+                              f32(e).astype(np.float64), reg=1e-2)
+    q = L.adam_step(P, go, {}, lr=1e-3)
+    assert abs(losses[0] - lo) < 1e-5
+    # first Adam step: |delta| = lr wherever |g| >> eps; compare the updated parameters
+    assert np.abs(out - L.flatten(q)).max() < 2e-6
+
+
+def test_features_match_oracle():
+    say("lp features")
+    s, p = _solver(6, 11)
+    n = 6
+    rng = np.random.default_rng(0)
+    nd = s.ndet()
+    r = rng.standard_normal(nd)
+    hd = s.hdiag()
+    lam = hd.min() - 0.3
+    idx = rng.integers(0, nd, 500)
+    X = s.lp_features(idx, lam, r)
+    Xo = L.features(s.strings(0), s.strings(1), n, 3, 3, hd, lam, r)[idx]
+    assert np.array_equal(X[:, 2:], Xo[:, 2:].astype(np.float32))
+    assert np.abs(X[:, :2] - Xo[:, :2]).max() < 1e-6 * (1 + np.abs(Xo[:, :2]).max())
+
+
+def test_inference_zero_model_is_diagonal():
+    say("lp inference zero")
+    s, _ = _solver(8, 12)
+    s.set_lp_weights(L.flatten(L.init_params(0)).astype(np.float32))  # y = 0
+    nd = s.ndet()
+    r = np.random.default_rng(1).standard_normal(nd)
+    hd = s.hdiag()
+    lam = float(hd[17])  # one element hits the |lambda - H| <= 1e-6 guard
+    d = s.lp_precondition(r, lam)
+    den = lam - hd
+    ref = np.where(np.abs(den) > 1e-6, r / np.where(np.abs(den) > 1e-6, den, 1.0), 0.0)
+    assert np.array_equal(d, ref)
+
+
+def test_inference_matches_oracle():
+    say("lp inference")
+    s, _ = _solver(8, 13)
+    p = _params(7, scale=0.1)
+    s.set_lp_weights(L.flatten(p).astype(np.float32))
+    nd = s.ndet()
+    r = np.random.default_rng(2).standard_normal(nd)
+    hd = s.hdiag()
+    lam = hd.min() - 1.0
+    d = s.lp_precondition(r, lam)
+    X = L.features(s.strings(0), s.strings(1), 8, 4, 4, hd, lam, r)
+    yo, _ = L.forward(p, X)
+    y = np.log(d * (lam - hd) / r)
+    # BF16 weights and activations, FP32 accumulation
+    assert np.abs(y - yo).max() < 0.03 * (1 + np.abs(yo).max()), np.abs(y - yo).max()
+
+
+def test_harvest_targets():
+    say("lp harvest")
+    s, p = _solver(6, 21)
+    s.set_e_convergence(1e-12)
+    s.set_r_convergence(1e-8)
+    s.set_maxiter(300)
+    X, d0, e = s.lp_harvest(per_iter=200, max_groups=400, seed=3)
+    ng = len(d0) // 200
+    assert X.shape == (ng * 200, 34) and ng >= 3
+    assert set(np.unique(X[:, 2:])) <= {0.0, 1.0}
+    assert np.isfinite(e).all() and np.isfinite(d0).all()
+    # the exact correction shrinks as the iterates converge
+    first = np.linalg.norm(e[:200])
+    last = np.linalg.norm(e[-200:])
+    assert last < 1e-3 * first
+    # the solve after a harvest starts fresh and still converges to the oracle energy
+    e0 = s.compute_energy()
+    from oracle import oracle as O
+    o = O.Oracle.from_chem(p["h"], p["eri"], 3, 3)
+    assert abs(e0 - o.solve(e_conv=1e-12, maxiter=300)["energies"][0]) < 1e-9
+
+
+@pytest.mark.parametrize("scale", [0.0, 0.3, 5.0])
+def test_learned_solve_energy_and_fallback(scale):
+    say("lp solve", scale)
+    s, _ = _solver(8, 14)
+    s.set_e_convergence(1e-11)
+    s.set_r_convergence(1e-7)
+    s.set_maxiter(400)
+    e_diag = s.compute_energy()
+    it_diag = s.stats()["iterations"]
+    p = _params(8, scale=scale)
+    s2, _ = _solver(8, 14)
+    s2.set_e_convergence(1e-11)
+    s2.set_r_convergence(1e-7)
+    s2.set_maxiter(400)
+    s2.set_lp_weights(L.flatten(p).astype(np.float32))
+    s2.set_preconditioner("learned")
+    e_l = s2.compute_energy()
+    st = s2.stats()
+    say("  diag", it_diag, "learned", st)
+    assert abs(e_l - e_diag) < 1e-9
+    assert st["converged"]
+
+
+def test_learned_requires_weights():
+    s, _ = _solver(6, 1)
+    s.set_preconditioner("learned")
+    with pytest.raises(RuntimeError, match="weights"):
+        s.compute_energy()
