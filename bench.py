@@ -8,13 +8,16 @@ seeded integrals, seed 3) with the mixed-precision kernels (FP32 C and sigma, BF
 contractions accumulated in FP32). `value` is the sigma throughput in GB/s, defined as
 9 * N_det * 4 bytes / t_sigma (C read by the two same-spin and the alpha-beta passes, sigma
 written and re-read), inputs resident in HBM. The line also carries:
-  roofline      MFMA roofline of the dominant kernel (the alpha-beta contraction), timed with
-                HIP events on its own stream; algorithmic flops = 3 * 2 * N_det * P * E with
-                P = n(n+1)/2 packed beta pairs and E = k(n-k)+1 alpha excitations (BF16x3 = 3
-                BF16 products), against the 2.5 PFLOP/s dense BF16 peak;
+  roofline      MFMA roofline of the dominant kernel (k_ab2_mixed, the alpha-beta contraction),
+                timed with HIP events on the stream it runs on; algorithmic flops per launch =
+                3 * 2 * N_det * P * E with P = n(n+1)/2 packed beta pairs and E = k(n-k) alpha
+                single excitations (BF16x3 = 3 BF16 products; the diagonal alpha term is a
+                rank-1 VALU update and is not counted), against the 2.5 PFLOP/s dense BF16 peak;
+                traffic = HBM bytes per launch from the committed rocprofv3 counter summary;
   fp64_sigma    the FP64 sigma of the plain solver on the same vector;
-  time_to_solution  full Davidson-Liu solves (FP64 vs mixed + FP64 refinement) to the same
-                    thresholds, and |E_mixed - E_fp64|;
+  time_to_solution  full Davidson-Liu solves to the same thresholds: FP64 with the diagonal
+                    preconditioner, mixed precision (FP32/BF16x3 sigma + FP64 refinement) with
+                    the learned preconditioner (shipped weights) and with the diagonal one;
   cpu_baseline  the CPU restatement of Forte's sigma (oracle/fci_oracle.c, one core) on a
                 strided sample of its loops, extrapolated to one full sigma.
 """
@@ -70,6 +73,9 @@ def main():
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--maxiter", type=int, default=400)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
+                         "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this FCI solver runs on one GPU; multi-GPU runs are not part of this build "
@@ -108,10 +114,19 @@ def main():
     print(f"# mixed sigma {ms_per_step:.2f} ms/step (events {ms_event:.2f}); beta-same {t3[0]:.2f} "
           f"alpha-same {t3[1]:.2f} alpha-beta {t3[2]:.2f} ms", file=sys.stderr, flush=True)
     np_ = n * (n + 1) // 2
-    E = na * (n - na) + 1
+    E = na * (n - na)
     ab_flops = 3 * 2 * nd * np_ * E
     achieved = ab_flops / (t3[2] * 1e-3) / 1e12
     del C, S
+
+    if args.calib_copy_mb > 0:
+        nc = args.calib_copy_mb * (1 << 20) // 4
+        A = forte_amd.DeviceArray(nc, "float32")
+        B = forte_amd.DeviceArray(nc, "float32")
+        B.copy_from(A)
+        forte_amd.device_synchronize()
+        print(f"# calibration copy of {nc * 4} bytes", file=sys.stderr, flush=True)
+        del A, B
 
     # ---- FP64 sigma on the same vector -----------------------------------------------------------
     C64 = forte_amd.DeviceArray(nd, "float64")

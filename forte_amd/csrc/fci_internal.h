@@ -132,6 +132,11 @@ struct DevInts {
     // same-spin clique operator G'[m][m'] over members (pairs or orbitals), per spin
     DBuf<double> g2[2];
     DBuf<float> g2f[2];
+    // mixed precision: g' without its diagonal (pass 1 on the MFMA) and, per string, the diagonal
+    // sum_{pairs in occ(I)} g'[pair][pair] applied in FP32 by the gather pass (the diagonal
+    // dominates |H C|, so keeping it off the BF16x3 products lowers the FP32 noise floor)
+    DBuf<float> g2f_off[2];
+    DBuf<float> gdiag[2];
     int gdim[2] = {0, 0};
 };
 
@@ -159,6 +164,10 @@ class SigmaEngine {
     bool timing_ = false;
     double ms_[3] = {0, 0, 0};
 };
+
+// per-string diagonal of the same-spin clique operator: out[I] = sum over the d-subsets P of
+// occ(I) of g[P][P] (d = 2: pairs a > b, index a(a-1)/2 + b; d = 1: orbitals)
+void clique_diagonal_device(const SpinTables& sp, const double* d_g, int gdim, float* d_out, hipStream_t st);
 
 // transposes: out[c*R + r] = in[r*Cc + c] (+ optional accumulate of beta * base)
 template <class T>

@@ -389,10 +389,12 @@ __global__ __launch_bounds__(1024) void k_lp_loss(const float* __restrict__ y, c
     }
     __syncthreads();
     const float Sde = tot[0], Sdd = tot[1], See = tot[2];
-    const float inv = 1.f / (Sdd * See);
-    if (threadIdx.x == 0) *loss_out = 1.f - Sde * Sde * inv + reg * tot[3] / float(m);
+    // a group with a vanishing correction or target carries no direction: no loss, no gradient
+    const bool ok = Sdd > 1e-30f && See > 1e-30f && isfinite(Sdd * See) && isfinite(Sde);
+    const float inv = ok ? 1.f / (Sdd * See) : 0.f;
+    if (threadIdx.x == 0) *loss_out = ok ? 1.f - Sde * Sde * inv + reg * tot[3] / float(m) : 0.f;
     float sg = 0.f;
-    for (int64_t i = threadIdx.x; i < m; i += 1024) {
+    for (int64_t i = ok ? int64_t(threadIdx.x) : m; i < m; i += 1024) {
         const float yc = fminf(fmaxf(y[i], -30.f), 30.f);
         const float d = d0[i] * __expf(yc);
         const float gd = -2.f * Sde * e[i] * inv + 2.f * Sde * Sde * d * inv / Sdd;
@@ -400,6 +402,8 @@ __global__ __launch_bounds__(1024) void k_lp_loss(const float* __restrict__ y, c
         gy[i] = gyi;
         sg += gyi;
     }
+    if (!ok)
+        for (int64_t i = threadIdx.x; i < m; i += 1024) gy[i] = 0.f;
     for (int off = 32; off > 0; off >>= 1) sg += __shfl_down(sg, off, 64);
     __syncthreads();
     if (lane == 0) red[0][wave] = sg;

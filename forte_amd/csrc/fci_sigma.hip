@@ -422,12 +422,19 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* _
 // E[(K * M + m) * Ls + c - cbeg] = s_m sum_m' W_K[m][m'] s_m' X[I_m'][c].
 constexpr int CQ3_CW = 64;  // columns per work item
 
-template <int NMT>  // row tiles of 16 members; k steps of 32 members: (NMT + 1) / 2
+template <int NMT, bool GL>  // row tiles of 16 members; k steps of 32 members: (NMT + 1) / 2; GL: g' in LDS
 __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__ X, float* __restrict__ E, int64_t L,
                                                        const uint32_t* __restrict__ clq, int M, int64_t nK,
-                                                       const float* __restrict__ G, int gdim, int nchunk,
+                                                       const float* __restrict__ Gg, int gdim, int nchunk,
                                                        int64_t cbeg, int64_t Ls) {
     constexpr int NKS = (NMT + 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) float gl_f32[];
+    const float* G = Gg;
+    if constexpr (GL) {
+        for (int i = threadIdx.x; i < gdim * gdim; i += 256) gl_f32[i] = Gg[i];
+        __syncthreads();
+        G = gl_f32;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int64_t nwork = nK * nchunk;
@@ -527,17 +534,24 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
     }
 }
 
-template <int NK4>  // k steps of 4 members; row tiles of 16: (NK4 + 3) / 4
-__global__ __launch_bounds__(256) void k_clique3_f64(const double* __restrict__ X, double* __restrict__ E, int64_t L,
+template <int NK4, bool GL>  // k steps of 4 members; row tiles of 16: (NK4 + 3) / 4; GL: g' in LDS
+__global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ X, double* __restrict__ E, int64_t L,
                                                      const uint32_t* __restrict__ clq, int M, int64_t nK,
-                                                     const double* __restrict__ G, int gdim, int nchunk,
+                                                     const double* __restrict__ Gg, int gdim, int nchunk,
                                                      int64_t cbeg, int64_t Ls) {
     constexpr int NMT = (NK4 + 3) / 4;
+    extern __shared__ __attribute__((aligned(16))) double gl_f64[];
+    const double* G = Gg;
+    if constexpr (GL) {
+        for (int i = threadIdx.x; i < gdim * gdim; i += 512) gl_f64[i] = Gg[i];
+        __syncthreads();
+        G = gl_f64;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int64_t nwork = nK * nchunk;
-    const int64_t nblk = (nwork + 3) / 4;
-    const int64_t w = xcd_remap(blockIdx.x, nblk) * 4 + wave;
+    const int64_t nblk = (nwork + 7) / 8;
+    const int64_t w = xcd_remap(blockIdx.x, nblk) * 8 + wave;
     if (w >= nwork) return;
     const int64_t K = w % nK;
     const int chunk = int(w / nK);
@@ -612,19 +626,30 @@ __global__ __launch_bounds__(256) void k_clique3_f64(const double* __restrict__ 
 template <class T, bool ACC>
 __global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, T* __restrict__ Y, int64_t nrow,
                                                        int64_t L, const uint32_t* __restrict__ own, int nown,
-                                                       int64_t cbeg, int64_t ncol, int64_t Ls) {
+                                                       int64_t cbeg, int64_t ncol, int64_t Ls,
+                                                       const T* __restrict__ X, const float* __restrict__ gdiag) {
     // block: 4 rows x 64 columns
     const int64_t nct = (ncol + 63) / 64;
     const int64_t nwork = ((nrow + 3) / 4) * nct;
     const int64_t w = xcd_remap(blockIdx.x, nwork);
     if (w >= nwork) return;
     const int64_t rb = w / nct, ct = w % nct;
-    const int64_t I = rb * 4 + (threadIdx.x >> 6);
+    const int64_t I = rb * 4 + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int64_t cl = ct * 64 + (threadIdx.x & 63);
     if (I >= nrow || cl >= ncol) return;
     const uint32_t* o = own + I * nown;
     T acc = T(0);
-    for (int j = 0; j < nown; ++j) acc += E[int64_t(o[j]) * Ls + cl];
+    int j = 0;
+    for (; j + 4 <= nown; j += 4) {  // four independent loads in flight
+        const T a0 = E[int64_t(o[j]) * Ls + cl], a1 = E[int64_t(o[j + 1]) * Ls + cl];
+        const T a2 = E[int64_t(o[j + 2]) * Ls + cl], a3 = E[int64_t(o[j + 3]) * Ls + cl];
+        acc += a0;
+        acc += a1;
+        acc += a2;
+        acc += a3;
+    }
+    for (; j < nown; ++j) acc += E[int64_t(o[j]) * Ls + cl];
+    if (gdiag) acc += T(gdiag[I]) * X[I * L + cbeg + cl];
     T* y = Y + I * L + cbeg + cl;
     if constexpr (ACC)
         *y += acc;
@@ -872,7 +897,7 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
 }
 
 // ---- alpha-beta, register-resident variant ----------------------------------------------------
-// One block per alpha string Ja, 4 waves = WM row-tile groups x (4 / WM) column groups. Each wave
+// One block per alpha string Ja, 8 waves = WM row-tile groups x (8 / WM) column groups. Each wave
 // keeps the A operand V[rs][e] = (p_e q_e | rs) of its row tiles (mtile = wm + WM * j) in
 // registers for the whole block and streams its own 16 beta columns at a time: the B operand
 // X[e][Ib] = s_e C[Ia_e][Ib] is loaded from global memory straight into the MFMA register layout
@@ -881,22 +906,25 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
 // lane -> one 8-byte load). Requires Nb <= 32767.
 constexpr uint16_t kInv16 = 0x7FFFu;
 
+// The LDS row accumulates in FP64: on gfx950 an LDS FP64 atomic add of a full wave costs about a
+// seventh of the FP32 one, which is serialized per lane.
 template <int KS, int TPW, int WM>
-__global__ __launch_bounds__(256, 2) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
+__global__ __launch_bounds__(512, 2) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
                                                       int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
                                                       const uint32_t* __restrict__ str_a,
                                                       const uint16_t* __restrict__ outT, int npp, int np,
                                                       const float* __restrict__ vpk, int nmt) {
-    extern __shared__ __attribute__((aligned(16))) float srow[];  // [Nb]
+    extern __shared__ __attribute__((aligned(16))) double srow_m[];  // [Nb]
+    double* srow = srow_m;
     __shared__ uint32_t ent[KS * 32];
-    constexpr int WC = 4 / WM;
+    constexpr int WC = 8 / WM, NTH = 512;
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wc = wave / WM;
     const int g = lane >> 4, cl = lane & 15;
-    for (int i = tid; i < KS * 32; i += 256) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
-    for (int64_t i = tid; i < Nb; i += 256) srow[i] = 0.f;
+    for (int i = tid; i < KS * 32; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
+    for (int64_t i = tid; i < Nb; i += NTH) srow[i] = 0.0;
     __syncthreads();
     const uint32_t occ = str_a[Ja];
     bf8 ah[TPW][KS], al[TPW][KS];
@@ -998,7 +1026,7 @@ __global__ __launch_bounds__(256, 2) void k_ab2_mixed(const float* __restrict__ 
                 const uint32_t t = uint32_t(t4 >> (16 * v)) & 0xFFFFu;
                 if ((t & 0x7FFFu) == kInv16) continue;
                 const float a = acc[v] + vd[j][v] * cd;
-                atomicAdd(&srow[t & 0x7FFFu], (t >> 15) ? -a : a);
+                atomicAdd(&srow[t & 0x7FFFu], double((t >> 15) ? -a : a));
             }
         }
 #pragma unroll
@@ -1011,7 +1039,7 @@ __global__ __launch_bounds__(256, 2) void k_ab2_mixed(const float* __restrict__ 
     }
     __syncthreads();
     float* Srow = S + Ja * Nb;
-    for (int64_t i = tid; i < Nb; i += 256) Srow[i] += srow[i];
+    for (int64_t i = tid; i < Nb; i += NTH) Srow[i] = float(double(Srow[i]) + srow[i]);
 }
 
 // FP64: 16x16x4 MFMA, KS k-steps of 4 single excitations. The A operand rows are loaded in the
@@ -1122,6 +1150,32 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
     for (int64_t i = tid; i < Nb; i += NTH) Srow[i] += srow64[i];
 }
 
+__global__ void k_clique_diag(const uint32_t* __restrict__ str, int64_t nstr, int n, int d,
+                              const double* __restrict__ g, int gdim, float* __restrict__ out) {
+    const int64_t I = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (I >= nstr) return;
+    const uint32_t s = str[I];
+    double acc = 0.0;
+    for (int a = 0; a < n; ++a) {
+        if (!((s >> a) & 1u)) continue;
+        if (d == 1) {
+            acc += g[size_t(a) * gdim + a];
+            continue;
+        }
+        for (int b = 0; b < a; ++b) {
+            if (!((s >> b) & 1u)) continue;
+            const int P = a * (a - 1) / 2 + b;
+            acc += g[size_t(P) * gdim + P];
+        }
+    }
+    out[I] = float(acc);
+}
+
+void clique_diagonal_device(const SpinTables& sp, const double* d_g, int gdim, float* d_out, hipStream_t st) {
+    k_clique_diag<<<nblk(sp.nstr, 128), 128, 0, st>>>(sp.str.p, sp.nstr, sp.n, sp.d, d_g, gdim, d_out);
+    HIPCHK(hipGetLastError());
+}
+
 // ---- launchers -----------------------------------------------------------------------------
 struct SigmaWork {
     DBuf<double> t64a, t64b;
@@ -1222,21 +1276,44 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
         const int64_t ncol = std::min<int64_t>(Ls, L - cbeg);
         if (reg) {
             const int nchunk = int((ncol + CQ3_CW - 1) / CQ3_CW);
-            const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
+            const size_t gsh = size_t(gd) * gd * sizeof(T);
+            const bool gl = gsh <= 150 * 1024;
             if constexpr (sizeof(T) == 8) {
+                const int64_t nblk3 = (sp.nK * nchunk + 7) / 8;
                 const int nk4 = (sp.M + 3) / 4;
                 switch (nk4) {
-#define C3(K) \
-    case K: k_clique3_f64<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd, nchunk, cbeg, Ls); break;
+#define C3(K)                                                                                                   \
+    case K:                                                                                                     \
+        if (gl) {                                                                                               \
+            HIPCHK(hipFuncSetAttribute((const void*)k_clique3_f64<K, true>,                                    \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(gsh)));                  \
+            k_clique3_f64<K, true><<<xcd_grid(nblk3), 512, gsh, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,           \
+                                                                     ints.g2[spin].p, gd, nchunk, cbeg, Ls);   \
+        } else {                                                                                                \
+            k_clique3_f64<K, false><<<xcd_grid(nblk3), 512, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,            \
+                                                                    ints.g2[spin].p, gd, nchunk, cbeg, Ls);    \
+        }                                                                                                       \
+        break;
                     C3(1) C3(2) C3(3) C3(4) C3(5) C3(6) C3(7) C3(8) C3(9) C3(10) C3(11) C3(12) C3(13) C3(14) C3(15) C3(16)
 #undef C3
                     default: throw Error(-1, "clique kernel: no instance for this size");
                 }
             } else {
+                const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
                 const int nmt = (sp.M + 15) / 16;
                 switch (nmt) {
-#define C3(K) \
-    case K: k_clique3_mixed<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd, nchunk, cbeg, Ls); break;
+#define C3(K)                                                                                                   \
+    case K:                                                                                                     \
+        if (gl) {                                                                                               \
+            HIPCHK(hipFuncSetAttribute((const void*)k_clique3_mixed<K, true>,                                  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(gsh)));                  \
+            k_clique3_mixed<K, true><<<xcd_grid(nblk3), 256, gsh, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,         \
+                                                                       ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls); \
+        } else {                                                                                                \
+            k_clique3_mixed<K, false><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,          \
+                                                                      ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls);  \
+        }                                                                                                       \
+        break;
                     C3(1) C3(2) C3(3) C3(4)
 #undef C3
                     default: throw Error(-1, "clique kernel: no instance for this size");
@@ -1247,15 +1324,17 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
                 X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, gd, nct, nkc, kch, cbeg, Ls);
         else
             k_clique2_mixed<TW32, WV32, true><<<xcd_grid(nwork), 64 * WV32, sh, st>>>(
-                X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2f[spin].p, gd, nct, nkc, kch, cbeg, Ls);
+                X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2f_off[spin].p, gd, nct, nkc, kch, cbeg, Ls);
         HIPCHK(hipGetLastError());
         const int64_t gw = ((sp.nstr + 3) / 4) * ((ncol + 63) / 64);
+        // mixed: the operator diagonal was left out of pass 1 and is applied here in FP32
+        const float* gdiag = sizeof(T) == 4 ? ints.gdiag[spin].p : nullptr;
         if (accumulate)
             k_clique_gather<T, true><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
-                                                                   ncol, Ls);
+                                                                   ncol, Ls, X, gdiag);
         else
             k_clique_gather<T, false><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
-                                                                    ncol, Ls);
+                                                                    ncol, Ls, X, gdiag);
         HIPCHK(hipGetLastError());
     }
 }
@@ -1302,10 +1381,10 @@ static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, cons
 template <int KS, int TPW>
 static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                              const DevInts& ints, hipStream_t st) {
-    const size_t sh = size_t(b.nstr) * sizeof(float);
+    const size_t sh = size_t(b.nstr) * sizeof(double);
     auto kern = k_ab2_mixed<KS, TPW, 2>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
-    kern<<<xcd_grid(a.nstr), 256, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
+    kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
                                             ints.vpk32.p, (b.np + 15) / 16);
     HIPCHK(hipGetLastError());
 }
@@ -1343,7 +1422,7 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
 }
 static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                       const DevInts& ints, hipStream_t st) {
-    if (ab2_ok(a, b, sizeof(float))) {
+    if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 31) / 32);
         const int tpw = ((b.np + 15) / 16 + 1) / 2;
 #define MXCASE(K, T) \

@@ -352,6 +352,12 @@ class Solver {
                 di_.g2[spin].upload(g.data(), g.size());
                 di_.g2f[spin].alloc(g32.size());
                 di_.g2f[spin].upload(g32.data(), g32.size());
+                for (int i = 0; i < gd; ++i) g32[size_t(i) * gd + i] = 0.f;
+                di_.g2f_off[spin].alloc(g32.size());
+                di_.g2f_off[spin].upload(g32.data(), g32.size());
+                di_.gdiag[spin].alloc(sp.nstr);
+                clique_diagonal_device(sp, di_.g2[spin].p, gd, di_.gdiag[spin].p, 0);
+                HIPCHK(hipDeviceSynchronize());
             }
         }
         have_ints_ = true;
@@ -557,7 +563,8 @@ double Solver::compute_energy() {
         dl_->set_e_convergence(std::max(opt_.e_convergence, 1e-7));
         dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
         dl_->set_maxiter(opt_.maxiter);
-        // end the mixed phase when the largest residual has not dropped by 20% in 2 iterations
+        // end the mixed phase at the FP32 noise floor: the largest residual has not improved on
+        // its best earlier value by 1% within the last 6 iterations
         auto hist = std::make_shared<std::vector<double>>();
         dl_->set_iteration_observer([hist, lp_observer](int it, const std::vector<double>& r,
                                                         const std::vector<double>& l) {
@@ -565,8 +572,12 @@ double Solver::compute_energy() {
             double mx = 0.0;
             for (double x : r) mx = std::max(mx, x);
             hist->push_back(mx);
-            size_t k = hist->size();
-            return k >= 3 && (*hist)[k - 1] > 0.8 * (*hist)[k - 3];
+            const size_t k = hist->size(), w = 6;
+            if (k <= w) return false;
+            double before = 1e300, recent = 1e300;
+            for (size_t i = 0; i < k - w; ++i) before = std::min(before, (*hist)[i]);
+            for (size_t i = k - w; i < k; ++i) recent = std::min(recent, (*hist)[i]);
+            return recent > 0.99 * before;
         });
         dl_->solve();
         nsig += dl_->sigma_builds();
@@ -1004,6 +1015,12 @@ int fci_device_upload(void* dst, const void* src, size_t bytes) {
 }
 int fci_device_download(void* dst, const void* src, size_t bytes) {
     return guarded(nullptr, [&] { HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int fci_device_copy(void* dst, const void* src, size_t bytes) {
+    return guarded(nullptr, [&] {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, 0));
+        HIPCHK(hipStreamSynchronize(0));
+    });
 }
 int fci_device_synchronize(void) {
     return guarded(nullptr, [&] { HIPCHK(hipDeviceSynchronize()); });

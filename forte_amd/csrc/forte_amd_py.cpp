@@ -116,6 +116,10 @@ class DeviceArray {
         check(fci_device_download(a.mutable_data(), p_, size_t(n_) * es_), fci_last_error(nullptr));
         return a;
     }
+    void copy_from(const DeviceArray& o) {
+        if (o.n_ != n_ || o.es_ != es_) throw std::runtime_error("DeviceArray.copy_from: shape mismatch");
+        check(fci_device_copy(p_, o.p_, size_t(n_) * es_), fci_last_error(nullptr));
+    }
     uintptr_t ptr() const { return reinterpret_cast<uintptr_t>(p_); }
     int64_t size() const { return n_; }
     int dtype_code() const { return es_ == 8 ? 0 : 1; }
@@ -562,6 +566,7 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def(py::init<int64_t, const std::string&>(), py::arg("n"), py::arg("dtype") = "float64")
         .def("upload", &DeviceArray::upload)
         .def("download", &DeviceArray::download)
+        .def("copy_from", &DeviceArray::copy_from)
         .def("ptr", &DeviceArray::ptr)
         .def("size", &DeviceArray::size);
 

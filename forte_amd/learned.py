@@ -69,12 +69,25 @@ def harvest(solvers, per_iter=1024, max_groups=64, seed=0, e_conv=1e-9, r_conv=1
         s.set_r_convergence(r_conv)
         s.set_maxiter(maxiter)
         t0 = time.time()
-        X, d0, e = s.lp_harvest(per_iter, max_groups, seed + j)
-        Xs.append(X)
-        ds.append(d0)
-        es.append(e)
+        try:
+            X, d0, e = s.lp_harvest(per_iter, max_groups, seed + j)
+        except RuntimeError as err:  # no converged eigenvector: no targets from this problem
+            if log:
+                log(f"harvest {j}: skipped ({str(err).splitlines()[0]})")
+            continue
+        ng = len(d0) // per_iter
+        keep = []
+        for g in range(ng):  # groups whose correction or target vanish carry no direction
+            sl = slice(g * per_iter, (g + 1) * per_iter)
+            fin = np.isfinite(X[sl]).all() and np.isfinite(d0[sl]).all() and np.isfinite(e[sl]).all()
+            if fin and np.linalg.norm(e[sl]) > 1e-10 and np.linalg.norm(d0[sl]) > 0:
+                keep.append(g)
+        idx = (np.asarray(keep)[:, None] * per_iter + np.arange(per_iter)[None, :]).ravel()
+        Xs.append(X[idx])
+        ds.append(d0[idx])
+        es.append(e[idx])
         if log:
-            log(f"harvest {j}: {len(d0) // per_iter} groups in {time.time() - t0:.2f}s")
+            log(f"harvest {j}: {len(keep)} of {ng} groups in {time.time() - t0:.2f}s")
     return np.concatenate(Xs), np.concatenate(ds), np.concatenate(es)
 
 
@@ -93,6 +106,30 @@ def group_losses(params, X, d0, e, group):
         sl = slice(g * group, (g + 1) * group)
         out.append(_C.lp_loss_grad(params, X[sl], d0[sl], e[sl], 0.0)[0])
     return np.array(out)
+
+
+def compare_solves(params, seeds, n, e_conv=1e-9, r_conv=1e-5, maxiter=400, precision="fp64"):
+    """Davidson-Liu iterations and wall time with the diagonal and the learned preconditioner."""
+    out = []
+    for sd in seeds:
+        row = {"seed": sd}
+        for pc in ("diagonal", "learned"):
+            s = seeded_solver(n, sd)
+            s.set_e_convergence(e_conv)
+            s.set_r_convergence(r_conv)
+            s.set_maxiter(maxiter)
+            s.set_die_if_not_converged(False)
+            s.set_precision(precision)
+            if pc == "learned":
+                s.set_lp_weights(params)
+                s.set_preconditioner("learned")
+            t0 = time.time()
+            e = s.compute_energy()
+            st = s.stats()
+            row[pc] = dict(energy=e, iterations=st["iterations"], seconds=round(time.time() - t0, 4),
+                           converged=st["converged"], fell_back=st["fell_back"])
+        out.append(row)
+    return out
 
 
 def main(argv=None):
@@ -124,10 +161,14 @@ def main(argv=None):
     base = float(group_losses(p0, Xh, dh, eh, a.per_iter).mean())
     held = float(group_losses(params, Xh, dh, eh, a.per_iter).mean())
     log(f"epoch losses {np.round(losses, 5).tolist()}; held-out loss diagonal {base:.5f} learned {held:.5f}")
+    cmp = compare_solves(params, seeds[ntrain:], a.n)
+    for row in cmp:
+        log(f"held-out seed {row['seed']}: diagonal {row['diagonal']['iterations']} iterations, "
+            f"learned {row['learned']['iterations']} (fell back: {row['learned']['fell_back']})")
     meta = dict(n=a.n, problems=a.problems, first_seed=a.first_seed, holdout=a.holdout, per_iter=a.per_iter,
                 max_groups=a.max_groups, epochs=a.epochs, lr=a.lr, reg=a.reg, train_samples=int(len(d0)),
                 epoch_loss=[float(x) for x in losses], heldout_loss_diagonal=base, heldout_loss_learned=held,
-                harvest_seconds=t_h, train_seconds=t_t)
+                harvest_seconds=t_h, train_seconds=t_t, heldout_solves=cmp)
     save_weights(params, a.out, meta)
     print(json.dumps(meta))
 

@@ -160,6 +160,8 @@ int fci_device_free(void* p);
 int fci_device_upload(void* dst_dev, const void* src_host, size_t bytes);
 int fci_device_download(void* dst_host, const void* src_dev, size_t bytes);
 int fci_device_synchronize(void);
+/* device-to-device copy of `bytes` (a plain copy kernel; used to calibrate traffic counters) */
+int fci_device_copy(void* dst_dev, const void* src_dev, size_t bytes);
 /* Time `nrep` sigma builds on device buffers with HIP events on one stream: ms_per_sigma is the
  * mean wall time per build; ms3 the mean per-kernel-group device time (as fci_sigma_timings). */
 int fci_sigma_bench(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, int nrep,
