@@ -564,7 +564,8 @@ double Solver::compute_energy() {
         dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
         dl_->set_maxiter(opt_.maxiter);
         // end the mixed phase at the FP32 noise floor: the largest residual has not improved on
-        // its best earlier value by 1% within the last 6 iterations
+        // its best earlier value by 1% within the last 20 iterations (Davidson residuals of
+        // strongly correlated problems plateau for several iterations long before that floor)
         auto hist = std::make_shared<std::vector<double>>();
         dl_->set_iteration_observer([hist, lp_observer](int it, const std::vector<double>& r,
                                                         const std::vector<double>& l) {
@@ -572,7 +573,7 @@ double Solver::compute_energy() {
             double mx = 0.0;
             for (double x : r) mx = std::max(mx, x);
             hist->push_back(mx);
-            const size_t k = hist->size(), w = 6;
+            const size_t k = hist->size(), w = 20;
             if (k <= w) return false;
             double before = 1e300, recent = 1e300;
             for (size_t i = 0; i < k - w; ++i) before = std::min(before, (*hist)[i]);
