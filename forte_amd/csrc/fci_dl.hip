@@ -93,6 +93,89 @@ __global__ __launch_bounds__(DL_BS) void k_dots(const double* __restrict__ x, co
     }
 }
 
+// part[blk][i] for the vectors i of chunk blockIdx.y (8 per chunk), double2 loads (x and every
+// Y_i 16-byte aligned: n even)
+constexpr int DOT_MC = 8;
+__global__ __launch_bounds__(DL_BS) void k_dots8(const double* __restrict__ x, const double* __restrict__ Y, int m,
+                                                 int64_t n, double* __restrict__ part) {
+    __shared__ double red[DOT_MC][DL_BS / 64];
+    const int i0 = blockIdx.y * DOT_MC;
+    const int mc = min(DOT_MC, m - i0);
+    double acc[DOT_MC];
+#pragma unroll
+    for (int i = 0; i < DOT_MC; ++i) acc[i] = 0.0;
+    const int64_t n2 = n / 2;
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n2; e += int64_t(gridDim.x) * DL_BS) {
+        const double2 xv = x2[e];
+#pragma unroll
+        for (int i = 0; i < DOT_MC; ++i)
+            if (i < mc) {
+                const double2 y = reinterpret_cast<const double2*>(Y + size_t(i0 + i) * n)[e];
+                acc[i] += xv.x * y.x + xv.y * y.y;
+            }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < DOT_MC; ++i) {
+        double v = acc[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[i][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < mc) {
+        double s = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) s += red[threadIdx.x][w];
+        part[size_t(blockIdx.x) * m + i0 + threadIdx.x] = s;
+    }
+}
+
+// Fused classical Gram-Schmidt step: v <- a v - sum_i d_i A_i (i < m), then the partial sums of
+// ||v||^2 (slot m) and v . A_i (slots i < m) of the updated v, in one pass over A.
+constexpr int GS_MAXM = 32;
+__global__ __launch_bounds__(DL_BS) void k_gs_fused(double* __restrict__ v, const double* __restrict__ A, int m,
+                                                    const double* __restrict__ d, double a, int64_t n,
+                                                    double* __restrict__ part) {
+    __shared__ double dd[GS_MAXM];
+    __shared__ double red[GS_MAXM + 1][DL_BS / 64];
+    for (int i = threadIdx.x; i < m; i += DL_BS) dd[i] = d[i];
+    __syncthreads();
+    double acc[GS_MAXM + 1];
+#pragma unroll
+    for (int i = 0; i <= GS_MAXM; ++i) acc[i] = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double av[GS_MAXM];
+        double x = a * v[e];
+#pragma unroll
+        for (int i = 0; i < GS_MAXM; ++i)
+            if (i < m) {
+                av[i] = A[size_t(i) * n + e];
+                x -= dd[i] * av[i];
+            }
+        v[e] = x;
+        acc[GS_MAXM] += x * x;
+#pragma unroll
+        for (int i = 0; i < GS_MAXM; ++i)
+            if (i < m) acc[i] += x * av[i];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i <= GS_MAXM; ++i) {
+        if (i < m || i == GS_MAXM) {
+            double t = acc[i];
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            if (lane == 0) red[i][wave] = t;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= m; i += DL_BS) {
+        const int src = i < m ? i : GS_MAXM;
+        double t = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) t += red[src][w];
+        part[size_t(blockIdx.x) * (m + 1) + i] = t;
+    }
+}
+
 __global__ void k_reduce_parts(const double* __restrict__ part, int nparts, int m, double* __restrict__ out) {
     int i = blockIdx.x;
     __shared__ double red[256];
@@ -309,13 +392,37 @@ void DavidsonLiu::eigenvector_host(int k, double* out) const {
 
 void DavidsonLiu::dots(const double* x, const double* Y, int m, double* out) {
     if (m <= 0) return;
-    int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
     double* part = part_.p;
     double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
-    k_dots<<<nb, DL_BS>>>(x, Y, m, size_, part);
+    int nb;
+    // double2 loads need every vector 16-byte aligned: even lengths only
+    if (m <= DL_MAXM && size_ % 2 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(Y) % 16 == 0) {
+        nb = int(std::min<int64_t>(DL_NBLK, (size_ / 2 + DL_BS - 1) / DL_BS));
+        nb = std::max(nb, 1);
+        k_dots8<<<dim3(nb, (m + DOT_MC - 1) / DOT_MC), DL_BS>>>(x, Y, m, size_, part);
+    } else {
+        nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+        k_dots<<<nb, DL_BS>>>(x, Y, m, size_, part);
+    }
     k_reduce_parts<<<m, 256>>>(part, nb, m, res);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, res, m * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+// v <- a v - sum_i d_i A_i; returns ||v||^2 and ov[i] = v . A_i of the updated v
+double DavidsonLiu::gs_fused(double* v, const double* A, int m, const std::vector<double>& d, double a,
+                             std::vector<double>& ov) {
+    const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    small_.upload(d.data(), std::max(m, 1));
+    double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
+    k_gs_fused<<<nb, DL_BS>>>(v, A, m, small_.p, a, size_, part_.p);
+    k_reduce_parts<<<m + 1, 256>>>(part_.p, nb, m + 1, res);
+    HIPCHK(hipGetLastError());
+    std::vector<double> out(m + 1);
+    HIPCHK(hipMemcpy(out.data(), res, (m + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    ov.assign(out.begin(), out.begin() + m);
+    return out[m];
 }
 
 void DavidsonLiu::lincomb(const double* X, int m, const double* coef, int k, double* out) {
@@ -347,6 +454,44 @@ double DavidsonLiu::add_row_and_orthonormalize(double* A, int rowsA, const doubl
     int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
     std::vector<double> d(std::max(rowsA, 1)), parts(nb);
     *ok = false;
+    if (rowsA <= GS_MAXM) {
+        // Fused passes: each cycle subtracts the projections found by the previous pass and, in the
+        // same sweep, measures the norm and the overlaps of the result; the normalization of a
+        // cycle is folded into the next update (v <- v / |v| - sum_i (ov_i / |v|) A_i).
+        if (rowsA > 0) dots(v, A, rowsA, d.data());
+        double a = 1.0;
+        std::vector<double> ov;
+        for (int cycle = 0; cycle < 10; ++cycle) {
+            const double s2 = gs_fused(v, A, rowsA, d, a, ov);
+            const double normval = std::sqrt(s2);
+            if (normval < discard_) return 0.0;
+            // after scaling by 1/normval: overlaps ov_i / normval, norm 1 up to rounding
+            double mo = 0.0;
+            for (int i = 0; i < rowsA; ++i) mo = std::max(mo, std::fabs(ov[i] / normval));
+            if (mo < orth_) {
+                k_scale<<<nblk(size_, 256), 256>>>(v, size_, 1. / normval);
+                HIPCHK(hipGetLastError());
+                double nn = 0.0;
+                dots(v, v, 1, &nn);
+                if (std::fabs(nn - 1.0) >= orth_) {
+                    a = 1.0;
+                    std::fill(d.begin(), d.end(), 0.0);
+                    continue;
+                }
+                *ok = true;
+                int m = subspace_size_;
+                if (A == b_.p) {
+                    for (int i = 0; i < rowsA; ++i)
+                        S_[size_t(i) * m + rowsA] = S_[size_t(rowsA) * m + i] = ov[i] / normval;
+                    S_[size_t(rowsA) * m + rowsA] = nn;
+                }
+                return 1.0;
+            }
+            a = 1.0 / normval;
+            for (int i = 0; i < rowsA; ++i) d[i] = ov[i] / normval;
+        }
+        return 0.0;
+    }
     for (int cycle = 0; cycle < 10; ++cycle) {
         if (rowsA > 0) dots(v, A, rowsA, d.data());
         small_.upload(d.data(), std::max(rowsA, 1));
