@@ -908,8 +908,8 @@ constexpr uint16_t kInv16 = 0x7FFFu;
 
 // The LDS row accumulates in FP64: on gfx950 an LDS FP64 atomic add of a full wave costs about a
 // seventh of the FP32 one, which is serialized per lane.
-template <int KS, int TPW, int WM>
-__global__ __launch_bounds__(512, 2) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
+template <int KS, int TPW, int WM, int WC>
+__global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
                                                       int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
                                                       const uint32_t* __restrict__ str_a,
                                                       const uint16_t* __restrict__ outT, int npp, int np,
@@ -917,7 +917,7 @@ __global__ __launch_bounds__(512, 2) void k_ab2_mixed(const float* __restrict__ 
     extern __shared__ __attribute__((aligned(16))) double srow_m[];  // [Nb]
     double* srow = srow_m;
     __shared__ uint32_t ent[KS * 32];
-    constexpr int WC = 8 / WM, NTH = 512;
+    constexpr int NTH = 64 * WM * WC;
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1276,8 +1276,10 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
         const int64_t ncol = std::min<int64_t>(Ls, L - cbeg);
         if (reg) {
             const int nchunk = int((ncol + CQ3_CW - 1) / CQ3_CW);
+            // g' stays in global memory (L1/L2): staging it in LDS caps the resident blocks per CU
+            // and measured slower at CAS(16,16) in both precisions
             const size_t gsh = size_t(gd) * gd * sizeof(T);
-            const bool gl = gsh <= 150 * 1024;
+            const bool gl = false;
             if constexpr (sizeof(T) == 8) {
                 const int64_t nblk3 = (sp.nK * nchunk + 7) / 8;
                 const int nk4 = (sp.M + 3) / 4;
@@ -1378,13 +1380,13 @@ static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, cons
                                             ints.vpk.p, (b.np + 15) / 16);
     HIPCHK(hipGetLastError());
 }
-template <int KS, int TPW>
+template <int KS, int TPW, int WM, int WC>
 static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                              const DevInts& ints, hipStream_t st) {
     const size_t sh = size_t(b.nstr) * sizeof(double);
-    auto kern = k_ab2_mixed<KS, TPW, 2>;
+    auto kern = k_ab2_mixed<KS, TPW, WM, WC>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
-    kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
+    kern<<<xcd_grid(a.nstr), 64 * WM * WC, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
                                             ints.vpk32.p, (b.np + 15) / 16);
     HIPCHK(hipGetLastError());
 }
@@ -1424,9 +1426,24 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
                       const DevInts& ints, hipStream_t st) {
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 31) / 32);
-        const int tpw = ((b.np + 15) / 16 + 1) / 2;
+        const int nmt = (b.np + 15) / 16;
+        // 8 waves (2 row-tile groups x 4 column groups); FCI_AB_LAYOUT=12 selects 4 x 3 (more
+        // waves, fewer registers, but each X tile is loaded by four waves: slower at CAS(16,16))
+        static const int layout = [] {
+            const char* e = std::getenv("FCI_AB_LAYOUT");
+            return e ? std::atoi(e) : 8;
+        }();
+        if (layout == 12 && (nmt + 3) / 4 * ks <= 6) {
+            const int tpw = (nmt + 3) / 4;
 #define MXCASE(K, T) \
-    if (ks == K && tpw == T) return ab2_mixed_launch<K, T>(C, S, a, b, ints, st);
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 4, 3>(C, S, a, b, ints, st);
+            MXCASE(1, 1) MXCASE(1, 2) MXCASE(1, 3) MXCASE(2, 1) MXCASE(2, 2) MXCASE(2, 3)
+            MXCASE(3, 1) MXCASE(3, 2)
+#undef MXCASE
+        }
+        const int tpw = (nmt + 1) / 2;
+#define MXCASE(K, T) \
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 2, 4>(C, S, a, b, ints, st);
 #define MXROW(K) MXCASE(K, 1) MXCASE(K, 2) MXCASE(K, 3) MXCASE(K, 4) MXCASE(K, 5) MXCASE(K, 6)
         MXROW(1) MXROW(2) MXROW(3)
 #undef MXROW
