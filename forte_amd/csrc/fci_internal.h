@@ -86,8 +86,10 @@ struct SpinTables {
     // when the excitation does not exist. P = (r,r): J = I when r occupied.
     int np = 0;
     DBuf<uint32_t> out;   // [np * nstr], pair-major
-    // the same targets transposed to [nstr][npp] (npp = np rounded up to 16) as 16-bit entries
-    // (address | sign << 15, 0x7FFF = none); built only when nstr <= 32767
+    // the same targets as 16-bit entries (address | sign << 15, 0x7FFF = none) in the paired-column
+    // MFMA tile layout [ceil(nstr/32)][npp/16][64 lanes][2][4] (npp = np rounded up to 16): the 8
+    // targets of lane (g, c) for strings 32 it + 2c + p and pairs 16 tile + 4 g + v are one 16-byte
+    // word; built only when nstr <= 32767
     int npp = 0;
     DBuf<uint16_t> outT;
     // pair cliques for the same-spin operator: hole strings K with k-d electrons (d = 2, or 1

@@ -523,7 +523,9 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = mt * 16 + 4 * g + v;
-                    if (r < M) E[(K * M + r) * Ls + (col - cbeg)] = ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
+                    if (r < M)
+                        E[((int64_t(chunk) * nK + K) * M + r) * CQ3_CW + (col - c0)] =
+                            ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
                 }
             }
         }
@@ -532,6 +534,121 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 #pragma unroll
             for (int x = 0; x < 8; ++x) xv[ks][x] = xn[ks][x];
     }
+}
+
+// Paired-column variant for even row lengths: each lane loads two adjacent columns with one 8-byte
+// load, which feeds two MFMA column tiles (columns c0 + 2j and c0 + 2j + 1, j = lane % 16), and
+// stores both results with one 8-byte store: half the memory instructions of k_clique3_mixed.
+template <int NMT>
+__global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict__ X, float* __restrict__ E, int64_t L,
+                                                        const uint32_t* __restrict__ clq, int M, int64_t nK,
+                                                        const float* __restrict__ G, int gdim, int nchunk,
+                                                        int64_t cbeg, int64_t Ls) {
+    constexpr int NKS = (NMT + 1) / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int64_t nwork = nK * nchunk;
+    const int64_t nblk = (nwork + 3) / 4;
+    const int64_t w = xcd_remap(blockIdx.x, nblk) * 4 + wave;
+    if (w >= nwork) return;
+    const int64_t K = w % nK;
+    const int chunk = int(w / nK);
+    const uint32_t* ent = clq + K * M;
+    uint32_t mb[NKS][8];
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const int m = ks * 32 + 8 * g + x;
+            mb[ks][x] = m < M ? ent[m] : kInvalid;
+        }
+    bf8 ah[NMT][NKS], al[NMT][NKS];
+    uint32_t neg[NMT];
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) {
+        const int r = mt * 16 + cl;
+        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t e = mb[ks][x];
+                const float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                __bf16 h, l;
+                split_bf16(v, h, l);
+                ah[mt][ks][x] = h;
+                al[mt][ks][x] = l;
+            }
+        uint32_t nb = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ro = mt * 16 + 4 * g + v;
+            if (ro < M) nb |= (ent[ro] >> 31) << v;
+        }
+        neg[mt] = nb;
+    }
+    const int64_t cend = cbeg + Ls < L ? cbeg + Ls : L;  // even
+    const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
+    float2 xv[NKS][8], xn[NKS][8];
+    auto load = [&](int64_t col, float2 (&xx)[NKS][8]) {
+        const bool ok = col < cend;  // col even, so col + 1 < cend as well
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t e = mb[ks][x];
+                float2 v = make_float2(0.f, 0.f);
+                if (ok && e != kInvalid) {
+                    v = *reinterpret_cast<const float2*>(X + int64_t(e & kAddrMask) * L + col);
+                    if (e >> 31) v = make_float2(-v.x, -v.y);
+                }
+                xx[ks][x] = v;
+            }
+    };
+    auto compute = [&](const float2 (&x)[NKS][8], int64_t col) {
+        bf8 bh0[NKS], bl0[NKS], bh1[NKS], bl1[NKS];
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx) {
+                __bf16 h, l;
+                split_bf16(x[ks][xx].x, h, l);
+                bh0[ks][xx] = h;
+                bl0[ks][xx] = l;
+                split_bf16(x[ks][xx].y, h, l);
+                bh1[ks][xx] = h;
+                bl1[ks][xx] = l;
+            }
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+            f4 a0 = f4{0.f, 0.f, 0.f, 0.f}, a1 = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt][ks], bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[mt][ks], bh1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bl0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bl1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bh1[ks], a1, 0, 0, 0);
+            }
+            if (col < cend) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * g + v;
+                    if (r < M) {
+                        const bool ng = (neg[mt] >> v) & 1u;
+                        *reinterpret_cast<float2*>(E + ((int64_t(chunk) * nK + K) * M + r) * CQ3_CW + (col - c0)) =
+                            ng ? make_float2(-a0[v], -a1[v]) : make_float2(a0[v], a1[v]);
+                    }
+                }
+            }
+        }
+    };
+    // the two 32-column halves of the chunk, the second one's loads in flight during the first
+    load(c0 + 2 * cl, xv);
+    load(c0 + 32 + 2 * cl, xn);
+    compute(xv, c0 + 2 * cl);
+    compute(xn, c0 + 32 + 2 * cl);
 }
 
 template <int NK4, bool GL>  // k steps of 4 members; row tiles of 16: (NK4 + 3) / 4; GL: g' in LDS
@@ -597,37 +714,46 @@ __global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ 
             xx[ks] = v;
         }
     };
-    load(c0 + cl, xv);
-#pragma unroll 1
-    for (int ct = 0; ct < CQ3_CW / 16; ++ct) {
-        const int64_t col = c0 + ct * 16 + cl;
-        if (ct + 1 < CQ3_CW / 16) load(col + 16, xn);
+    auto compute = [&](const double (&xx)[NK4], int64_t col) {
 #pragma unroll
         for (int mt = 0; mt < NMT; ++mt) {
             d4 acc = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int ks = 0; ks < NK4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt][ks], xv[ks], acc, 0, 0, 0);
+            for (int ks = 0; ks < NK4; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt][ks], xx[ks], acc, 0, 0, 0);
             if (col < cend) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = mt * 16 + 4 * v + g;
-                    if (r < M) E[(K * M + r) * Ls + (col - cbeg)] = ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
+                    if (r < M)
+                        E[((int64_t(chunk) * nK + K) * M + r) * CQ3_CW + (col - c0)] =
+                            ((neg[mt] >> v) & 1u) ? -acc[v] : acc[v];
                 }
             }
         }
-#pragma unroll
-        for (int ks = 0; ks < NK4; ++ks) xv[ks] = xn[ks];
-    }
+    };
+    // four 16-column tiles, two buffers used alternately (no register copies)
+    static_assert(CQ3_CW == 64, "four tiles per work item");
+    load(c0 + cl, xv);
+    load(c0 + 16 + cl, xn);
+    compute(xv, c0 + cl);
+    load(c0 + 32 + cl, xv);
+    compute(xn, c0 + 16 + cl);
+    load(c0 + 48 + cl, xn);
+    compute(xv, c0 + 32 + cl);
+    compute(xn, c0 + 48 + cl);
 }
 
-// Pass 2: Y[I][cbeg + c] (= or +=) sum_j E[own[I][j]][c] for c < ncol, E rows of stride Ls:
-// every hole string K inside I hands I its row of s_m (W_K s X_K)_m. No atomics, and the sum
+// Pass 2: Y[I][cbeg + c] (= or +=) sum_j E[own[I][j]][c] for c < ncol: every hole string K
+// inside I hands I its row of s_m (W_K s X_K)_m. E is either row-major with rows of stride Ls
+// (CHUNKED = false) or blocked by 64-column chunk, E[(chunk * nrowE + row) * 64 + c % 64]
+// (CHUNKED = true: each pass-1 work item writes one contiguous block). No atomics, and the sum
 // runs in a fixed order, so the result does not depend on scheduling.
-template <class T, bool ACC>
+template <class T, bool ACC, bool CHUNKED>
 __global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, T* __restrict__ Y, int64_t nrow,
                                                        int64_t L, const uint32_t* __restrict__ own, int nown,
                                                        int64_t cbeg, int64_t ncol, int64_t Ls,
-                                                       const T* __restrict__ X, const float* __restrict__ gdiag) {
+                                                       const T* __restrict__ X, const float* __restrict__ gdiag,
+                                                       int64_t nrowE) {
     // block: 4 rows x 64 columns
     const int64_t nct = (ncol + 63) / 64;
     const int64_t nwork = ((nrow + 3) / 4) * nct;
@@ -638,17 +764,20 @@ __global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, 
     const int64_t cl = ct * 64 + (threadIdx.x & 63);
     if (I >= nrow || cl >= ncol) return;
     const uint32_t* o = own + I * nown;
+    // element (row, cl) of E
+    const T* Eb = CHUNKED ? E + (cl / 64) * nrowE * 64 + (cl % 64) : E + cl;
+    const int64_t rs = CHUNKED ? 64 : Ls;
     T acc = T(0);
     int j = 0;
     for (; j + 4 <= nown; j += 4) {  // four independent loads in flight
-        const T a0 = E[int64_t(o[j]) * Ls + cl], a1 = E[int64_t(o[j + 1]) * Ls + cl];
-        const T a2 = E[int64_t(o[j + 2]) * Ls + cl], a3 = E[int64_t(o[j + 3]) * Ls + cl];
+        const T a0 = Eb[int64_t(o[j]) * rs], a1 = Eb[int64_t(o[j + 1]) * rs];
+        const T a2 = Eb[int64_t(o[j + 2]) * rs], a3 = Eb[int64_t(o[j + 3]) * rs];
         acc += a0;
         acc += a1;
         acc += a2;
         acc += a3;
     }
-    for (; j < nown; ++j) acc += E[int64_t(o[j]) * Ls + cl];
+    for (; j < nown; ++j) acc += Eb[int64_t(o[j]) * rs];
     if (gdiag) acc += T(gdiag[I]) * X[I * L + cbeg + cl];
     T* y = Y + I * L + cbeg + cl;
     if constexpr (ACC)
@@ -899,15 +1028,16 @@ __global__ __launch_bounds__(256) void k_ab_mixed(const float* __restrict__ C, f
 // ---- alpha-beta, register-resident variant ----------------------------------------------------
 // One block per alpha string Ja, 8 waves = WM row-tile groups x (8 / WM) column groups. Each wave
 // keeps the A operand V[rs][e] = (p_e q_e | rs) of its row tiles (mtile = wm + WM * j) in
-// registers for the whole block and streams its own 16 beta columns at a time: the B operand
-// X[e][Ib] = s_e C[Ia_e][Ib] is loaded from global memory straight into the MFMA register layout
-// (no LDS staging), and the product plus the diagonal rank-1 term is scattered into the LDS row
-// srow[Jb] with Jb = E_rs Ib from the transposed 16-bit table outT[Ib][npp] (4 consecutive rs per
-// lane -> one 8-byte load). Requires Nb <= 32767.
+// registers for the whole block and streams its own 32 beta columns at a time: lane (g, c) loads
+// columns 2c and 2c + 1 of every gathered row X[e][Ib] = s_e C[Ia_e][Ib] with one 8- (FP32) or
+// 16-byte (FP64) load, so each load instruction reads whole 128-byte lines of 4 rows, straight into
+// the MFMA B-operand layout of two column tiles (no LDS staging). The product plus the diagonal
+// rank-1 term is scattered into the LDS row srow[Jb], Jb = E_rs Ib, from the paired 16-bit target
+// table (8 targets per lane and tile = one 16-byte load; a wave's loads are contiguous).
+// The LDS row accumulates in FP64: on gfx950 an LDS FP64 atomic add of a full wave costs about a
+// seventh of the FP32 one, which is serialized per lane. Requires Nb even and Nb <= 32767.
 constexpr uint16_t kInv16 = 0x7FFFu;
 
-// The LDS row accumulates in FP64: on gfx950 an LDS FP64 atomic add of a full wave costs about a
-// seventh of the FP32 one, which is serialized per lane.
 template <int KS, int TPW, int WM, int WC>
 __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
                                                       int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
@@ -923,7 +1053,7 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wc = wave / WM;
     const int g = lane >> 4, cl = lane & 15;
-    for (int i = tid; i < KS * 32; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
+    for (int i = tid; i < KS * 32; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : kInvalid;
     for (int64_t i = tid; i < Nb; i += NTH) srow[i] = 0.0;
     __syncthreads();
     const uint32_t occ = str_a[Ja];
@@ -961,81 +1091,69 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
             vd[j][v] = d;
         }
     }
-    // gather rows of this lane's k group: e = ks * 32 + 8 g + x
-    uint32_t src[KS][8];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-        for (int x = 0; x < 8; ++x) {
-            const int e = ks * 32 + 8 * g + x;
-            src[ks][x] = e < ns ? ent[e] : kInvalid;
-        }
     const float* Crow = C + Ja * Nb;
-    // one column tile of loads in flight while the previous tile is multiplied and scattered
-    float xv[KS][8], xn[KS][8];
-    uint64_t tg[TPW], tn[TPW];
-    float cd, cn;
-    auto load = [&](int64_t c0, float (&x)[KS][8], uint64_t (&t)[TPW], float& d) {
-        const int64_t Ib = c0 + cl;
+    const int ntile = npp >> 4;
+    for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 32 * WC) {
+        const int64_t Ib = c0 + 2 * cl;  // and Ib + 1 (Nb even)
         const bool ok = Ib < Nb;
+        uint4 tg[TPW];
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
             const int mtile = wm + WM * j;
-            t[j] = (ok && mtile < nmt) ? *reinterpret_cast<const uint64_t*>(outT + Ib * npp + mtile * 16 + 4 * g)
-                                       : ~uint64_t(0);
+            tg[j] = (ok && mtile < nmt)
+                        ? *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtile) * 64 + lane) * 8)
+                        : make_uint4(~0u, ~0u, ~0u, ~0u);
         }
+        float2 xv[KS][8];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
             for (int xx = 0; xx < 8; ++xx) {
-                const uint32_t e = src[ks][xx];
-                float v = 0.f;
+                const uint32_t e = ent[ks * 32 + 8 * g + xx];
+                float2 v = make_float2(0.f, 0.f);
                 if (ok && e != kInvalid) {
-                    v = C[int64_t(e & kAddrMask) * Nb + Ib];
-                    if (e >> 31) v = -v;
+                    v = *reinterpret_cast<const float2*>(C + int64_t(e & kAddrMask) * Nb + Ib);
+                    if (e >> 31) v = make_float2(-v.x, -v.y);
                 }
-                x[ks][xx] = v;
+                xv[ks][xx] = v;
             }
-        d = ok ? Crow[Ib] : 0.f;
-    };
-    load(int64_t(wc) * 16, xv, tg, cd);
-    for (int64_t c0 = int64_t(wc) * 16; c0 < Nb; c0 += 16 * WC) {
-        load(c0 + 16 * WC, xn, tn, cn);
-        bf8 bh[KS], bl[KS];
+        const float2 cd = ok ? *reinterpret_cast<const float2*>(Crow + Ib) : make_float2(0.f, 0.f);
+        bf8 bh0[KS], bl0[KS], bh1[KS], bl1[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
             for (int xx = 0; xx < 8; ++xx) {
                 __bf16 h, l;
-                split_bf16(xv[ks][xx], h, l);
-                bh[ks][xx] = h;
-                bl[ks][xx] = l;
+                split_bf16(xv[ks][xx].x, h, l);
+                bh0[ks][xx] = h;
+                bl0[ks][xx] = l;
+                split_bf16(xv[ks][xx].y, h, l);
+                bh1[ks][xx] = h;
+                bl1[ks][xx] = l;
             }
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+            f4 a0 = f4{0.f, 0.f, 0.f, 0.f}, a1 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j][ks], bh[ks], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bl[ks], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bh[ks], acc, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j][ks], bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j][ks], bh1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bl0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bl1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j][ks], bh1[ks], a1, 0, 0, 0);
             }
-            const uint64_t t4 = tg[j];
+            const uint32_t tw[4] = {tg[j].x, tg[j].y, tg[j].z, tg[j].w};  // [p][v] pairs of u16
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const uint32_t t = uint32_t(t4 >> (16 * v)) & 0xFFFFu;
-                if ((t & 0x7FFFu) == kInv16) continue;
-                const float a = acc[v] + vd[j][v] * cd;
-                atomicAdd(&srow[t & 0x7FFFu], double((t >> 15) ? -a : a));
-            }
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
+                    if ((tt & 0x7FFFu) == kInv16) continue;
+                    const float a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
+                    atomicAdd(&srow[tt & 0x7FFFu], double((tt >> 15) ? -a : a));
+                }
         }
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-            for (int xx = 0; xx < 8; ++xx) xv[ks][xx] = xn[ks][xx];
-#pragma unroll
-        for (int j = 0; j < TPW; ++j) tg[j] = tn[j];
-        cd = cn;
     }
     __syncthreads();
     float* Srow = S + Ja * Nb;
@@ -1059,7 +1177,7 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wc = wave / WM;
     const int g = lane >> 4, cl = lane & 15;
-    for (int i = tid; i < KS * 4; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : 0u;
+    for (int i = tid; i < KS * 4; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : kInvalid;
     for (int64_t i = tid; i < Nb; i += NTH) srow64[i] = 0.0;
     __syncthreads();
     const uint32_t occ = str_a[Ja];
@@ -1091,59 +1209,50 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
             vd[j][v] = d;
         }
     }
-    uint32_t src[KS];
-#pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-        const int e = ks * 4 + g;
-        src[ks] = e < ns ? ent[e] : kInvalid;
-    }
     const double* Crow = C + Ja * Nb;
-    double xv[KS], xn[KS];
-    uint64_t tg[TPW], tn[TPW];
-    double cd, cn;
-    auto load = [&](int64_t c0, double (&x)[KS], uint64_t (&t)[TPW], double& d) {
-        const int64_t Ib = c0 + cl;
+    const int ntile = npp >> 4;
+    for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 32 * WC) {
+        const int64_t Ib = c0 + 2 * cl;
         const bool ok = Ib < Nb;
+        uint4 tg[TPW];
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
             const int mtile = wm + WM * j;
-            t[j] = (ok && mtile < nmt) ? *reinterpret_cast<const uint64_t*>(outT + Ib * npp + mtile * 16 + 4 * g)
-                                       : ~uint64_t(0);
+            tg[j] = (ok && mtile < nmt)
+                        ? *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtile) * 64 + lane) * 8)
+                        : make_uint4(~0u, ~0u, ~0u, ~0u);
         }
+        double2 xv[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            const uint32_t e = src[ks];
-            double v = 0.0;
+            const uint32_t e = ent[ks * 4 + g];
+            double2 v = make_double2(0.0, 0.0);
             if (ok && e != kInvalid) {
-                v = C[int64_t(e & kAddrMask) * Nb + Ib];
-                if (e >> 31) v = -v;
+                v = *reinterpret_cast<const double2*>(C + int64_t(e & kAddrMask) * Nb + Ib);
+                if (e >> 31) v = make_double2(-v.x, -v.y);
             }
-            x[ks] = v;
+            xv[ks] = v;
         }
-        d = ok ? Crow[Ib] : 0.0;
-    };
-    load(int64_t(wc) * 16, xv, tg, cd);
-    for (int64_t c0 = int64_t(wc) * 16; c0 < Nb; c0 += 16 * WC) {
-        load(c0 + 16 * WC, xn, tn, cn);
+        const double2 cd = ok ? *reinterpret_cast<const double2*>(Crow + Ib) : make_double2(0.0, 0.0);
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            d4 acc = d4{0.0, 0.0, 0.0, 0.0};
+            d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks], acc, 0, 0, 0);
-            const uint64_t t4 = tg[j];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const uint32_t t = uint32_t(t4 >> (16 * v)) & 0xFFFFu;
-                if ((t & 0x7FFFu) == kInv16) continue;
-                const double a = acc[v] + vd[j][v] * cd;
-                atomicAdd(&srow64[t & 0x7FFFu], (t >> 15) ? -a : a);
+            for (int ks = 0; ks < KS; ++ks) {
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].x, a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].y, a1, 0, 0, 0);
             }
+            const uint32_t tw[4] = {tg[j].x, tg[j].y, tg[j].z, tg[j].w};
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
+                    if ((tt & 0x7FFFu) == kInv16) continue;
+                    const double a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
+                    atomicAdd(&srow64[tt & 0x7FFFu], (tt >> 15) ? -a : a);
+                }
         }
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) xv[ks] = xn[ks];
-#pragma unroll
-        for (int j = 0; j < TPW; ++j) tg[j] = tn[j];
-        cd = cn;
     }
     __syncthreads();
     double* Srow = S + Ja * Nb;
@@ -1300,6 +1409,19 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
 #undef C3
                     default: throw Error(-1, "clique kernel: no instance for this size");
                 }
+            } else if (L % 2 == 0) {
+                const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
+                const int nmt = (sp.M + 15) / 16;
+                switch (nmt) {
+#define C3(K)                                                                                                 \
+    case K:                                                                                                   \
+        k_clique3p_mixed<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,                 \
+                                                             ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls);     \
+        break;
+                    C3(1) C3(2) C3(3) C3(4)
+#undef C3
+                    default: throw Error(-1, "clique kernel: no instance for this size");
+                }
             } else {
                 const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
                 const int nmt = (sp.M + 15) / 16;
@@ -1331,12 +1453,19 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
         const int64_t gw = ((sp.nstr + 3) / 4) * ((ncol + 63) / 64);
         // mixed: the operator diagonal was left out of pass 1 and is applied here in FP32
         const float* gdiag = sizeof(T) == 4 ? ints.gdiag[spin].p : nullptr;
-        if (accumulate)
-            k_clique_gather<T, true><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
-                                                                   ncol, Ls, X, gdiag);
+        const int64_t nrowE = sp.nK * sp.M;
+        if (accumulate && reg)
+            k_clique_gather<T, true, true><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown,
+                                                                         cbeg, ncol, Ls, X, gdiag, nrowE);
+        else if (reg)
+            k_clique_gather<T, false, true><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown,
+                                                                          cbeg, ncol, Ls, X, gdiag, nrowE);
+        else if (accumulate)
+            k_clique_gather<T, true, false><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown,
+                                                                          cbeg, ncol, Ls, X, gdiag, nrowE);
         else
-            k_clique_gather<T, false><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown, cbeg,
-                                                                    ncol, Ls, X, gdiag);
+            k_clique_gather<T, false, false><<<xcd_grid(gw), 256, 0, st>>>(E, Y, sp.nstr, L, sp.own.p, sp.nown,
+                                                                           cbeg, ncol, Ls, X, gdiag, nrowE);
         HIPCHK(hipGetLastError());
     }
 }
@@ -1393,7 +1522,8 @@ static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, cons
 
 // register-resident kernels: 16-bit targets, LDS row, E <= 96, <= 12 row tiles
 static bool ab2_ok(const SpinTables& a, const SpinTables& b, size_t w) {
-    return b.outT.p != nullptr && a.ns <= 96 && (b.np + 15) / 16 <= 12 && size_t(b.nstr) * w <= 150 * 1024;
+    return b.outT.p != nullptr && b.nstr % 2 == 0 && a.ns <= 96 && (b.np + 15) / 16 <= 12 &&
+           size_t(b.nstr) * w <= 150 * 1024;
 }
 
 static void launch_ab(const double* C, double* S, const SpinTables& a, const SpinTables& b,

@@ -175,17 +175,31 @@ __global__ void k_clique_owner(const uint32_t* __restrict__ str, int64_t nstr, i
 
 static inline unsigned nblk(int64_t n, int bs) { return unsigned((n + bs - 1) / bs); }
 
+// o16 in the paired-column MFMA tile layout used by the alpha-beta kernels: strings in groups of
+// 32, where lane (g, c) of a wave handles strings 32 it + 2c and 32 it + 2c + 1 and pairs
+// 16 tile + 4 g + v (v = 0..3):
+//   element (((it * ntile + tile) * 64 + 16 g + c) * 2 + p) * 4 + v,  string 32 it + 2c + p,
+// so the 8 targets a lane needs for one tile are one 16-byte word and a wave's 64 words are
+// contiguous. Strings beyond nstr and pairs beyond np hold 0x7FFF (none).
 __global__ void k_out_t16(const uint32_t* __restrict__ out, int64_t nstr, int np, int npp, uint16_t* __restrict__ o16) {
+    const int ntile = npp / 16;
+    const int64_t nrow = (nstr + 31) / 32 * 32;
     const int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (x >= nstr * npp) return;
-    const int64_t I = x / npp;
-    const int P = int(x % npp);
-    uint16_t v = 0x7FFFu;
-    if (P < np) {
+    if (x >= nrow * npp) return;
+    const int v = int(x % 4);
+    const int pp = int((x / 4) % 2);
+    const int lane = int((x / 8) % 64);
+    const int64_t q = x / 512;  // it * ntile + tile
+    const int tile = int(q % ntile);
+    const int64_t it = q / ntile;
+    const int64_t I = it * 32 + 2 * (lane % 16) + pp;
+    const int P = tile * 16 + 4 * (lane / 16) + v;
+    uint16_t val = 0x7FFFu;
+    if (P < np && I < nstr) {
         const uint32_t t = out[int64_t(P) * nstr + I];
-        if (t != kInvalid) v = uint16_t((t & kAddrMask) | ((t >> 31) << 15));
+        if (t != kInvalid) val = uint16_t((t & kAddrMask) | ((t >> 31) << 15));
     }
-    o16[x] = v;
+    o16[x] = val;
 }
 
 void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream_t st) {
@@ -205,8 +219,9 @@ void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream
     k_out<<<nblk(t.nstr * t.np, 256), 256, 0, st>>>(t.str.p, t.out.p, t.nstr, n, t.np, B);
     if (t.nstr <= 32767) {
         t.npp = (t.np + 15) / 16 * 16;
-        t.outT.alloc(size_t(t.nstr) * t.npp);
-        k_out_t16<<<nblk(t.nstr * t.npp, 256), 256, 0, st>>>(t.out.p, t.nstr, t.np, t.npp, t.outT.p);
+        const int64_t nrow = (t.nstr + 31) / 32 * 32;
+        t.outT.alloc(size_t(nrow) * t.npp);
+        k_out_t16<<<nblk(nrow * t.npp, 256), 256, 0, st>>>(t.out.p, t.nstr, t.np, t.npp, t.outT.p);
     }
     t.d = k >= 2 ? 2 : (k == 1 ? 1 : 0);
     if (t.d > 0) {
