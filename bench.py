@@ -73,13 +73,15 @@ def main():
     ap.add_argument("--cpu", type=int, default=1)
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--maxiter", type=int, default=400)
+    ap.add_argument("--pcie", type=int, default=1, help="time one FP64 sigma on host arrays (PCIe both ways)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this FCI solver runs on one GPU; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the sigma build would shard)", file=sys.stderr)
+              "(the sigma build would shard by alpha-row blocks with one all-gather of C per build)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -136,6 +138,15 @@ def main():
         print(f"# calibration copy of {nc * 4} bytes", file=sys.stderr, flush=True)
         del A, B
 
+    # ---- PCIe-inclusive FP64 sigma (host arrays in and out; reported, never the value) ---------
+    pcie_ms = None
+    if args.pcie:
+        forte_amd.device_synchronize()
+        t0 = time.perf_counter()
+        solver.sigma(x)
+        pcie_ms = 1e3 * (time.perf_counter() - t0)
+        print(f"# fp64 sigma with host buffers (PCIe both ways) {pcie_ms:.1f} ms", file=sys.stderr, flush=True)
+
     # ---- FP64 sigma on the same vector -----------------------------------------------------------
     C64 = forte_amd.DeviceArray(nd, "float64")
     S64 = forte_amd.DeviceArray(nd, "float64")
@@ -168,7 +179,8 @@ def main():
         "sigma_split_ms": {"beta_same_spin_with_transposes": round(t3[0], 3),
                            "alpha_same_spin": round(t3[1], 3), "alpha_beta": round(t3[2], 3)},
         "fp64_sigma": {"ms": round(ms64, 3), "GBps": round(9 * nd * 8 / (ms64 * 1e-3) / 1e9, 3),
-                       "split_ms": [round(t, 3) for t in t3_64]},
+                       "split_ms": [round(t, 3) for t in t3_64],
+                       "host_buffers_pcie_inclusive_ms": None if pcie_ms is None else round(pcie_ms, 1)},
     }
 
     # ---- time to solution ------------------------------------------------------------------------
