@@ -63,6 +63,40 @@ def cpu_baseline(n, h, eri, sample_stride):
                         f"of its list entries covered"))
 
 
+def batch_cpu_baseline(probs, gpu_results):
+    """Oracle (one core) on the batched workload: full FP64 solves of two problems per size for
+    n = 6, 8, 10; for n = 12 one sampled sigma (stride 8) times 2 x the GPU iteration count.
+    Extrapolated to the whole batch by the problem counts per size."""
+    import numpy as np
+    from oracle import oracle as O
+    per_n_time = {}
+    for nn in (6, 8, 10):
+        sel = [p for p in probs if p[0] == nn][:2]
+        t0 = time.time()
+        for n_, seed in sel:
+            h_, eri_ = O.seeded_integrals(n_, seed)
+            O.Oracle.from_chem(h_, eri_, n_ // 2, n_ // 2).solve(nroot=2, maxiter=600, e_conv=1e-9, r_conv=1e-5)
+        per_n_time[nn] = (time.time() - t0) / len(sel)
+    sel = [p for p in probs if p[0] == 12][:1]
+    if sel:
+        n_, seed = sel[0]
+        h_, eri_ = O.seeded_integrals(n_, seed)
+        o = O.Oracle.from_chem(h_, eri_, 6, 6)
+        C = np.random.default_rng(1).standard_normal(o.ndet)
+        t, w = o.sigma_sample(C, 8)
+        sig = float(sum(ti / wi for ti, wi in zip(t, w) if wi > 0))
+        its = [r["iterations"] for r in gpu_results if r["n"] == 12]
+        per_n_time[12] = sig * 2 * (sum(its) / len(its))
+    counts = {}
+    for n_, _ in probs:
+        counts[n_] = counts.get(n_, 0) + 1
+    total = sum(counts[k] * per_n_time[k] for k in counts if k in per_n_time)
+    return dict(value=round(len(probs) / total, 4), unit="problems/s", cores=1, kind="port",
+                seconds_extrapolated=round(total, 1),
+                sample="oracle FP64 solves of 2 problems each for n = 6, 8, 10; n = 12 from one strided "
+                       "sigma sample x 2 x the GPU iteration count; scaled by the problem counts")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +108,10 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--maxiter", type=int, default=400)
     ap.add_argument("--pcie", type=int, default=1, help="time one FP64 sigma on host arrays (PCIe both ways)")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="also solve this many seeded small problems (n in {6,8,10,12}, 2 roots, FP64) "
+                         "with concurrent solvers; 0 to skip")
+    ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
@@ -222,6 +260,31 @@ def main():
             "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
             "thresholds": {"e_conv": 1e-9, "r_conv": 1e-4, "maxiter": args.maxiter},
             "learned_weights": forte_amd.learned.WEIGHTS if lp_w is not None else None}
+
+    # ---- batched small problems -----------------------------------------------------------------
+    if args.batch > 0:
+        from forte_amd.batch import seeded_batch, solve_batch
+        probs = seeded_batch(args.batch)
+        bopt = {"E_CONVERGENCE": 1e-9, "R_CONVERGENCE": 1e-5}
+        solve_batch(probs[:8], nroot=2, threads=args.batch_threads, options=bopt)  # warm-up
+        forte_amd.device_synchronize()
+        res, wall = solve_batch(probs, nroot=2, threads=args.batch_threads, options=bopt)
+        forte_amd.device_synchronize()
+        by_n = {}
+        for r in res:
+            d = by_n.setdefault(r["n"], {"count": 0, "iterations": 0})
+            d["count"] += 1
+            d["iterations"] += r["iterations"]
+        result["batched"] = {"workload": f"{args.batch} seeded CAS(n,n), n in {{6,8,10,12}}, 2 roots, FP64, "
+                                         f"e_conv 1e-9, r_conv 1e-5",
+                             "problems_per_s": round(len(res) / wall, 2), "seconds": round(wall, 3),
+                             "threads": args.batch_threads, "all_converged": all(r["converged"] for r in res),
+                             "per_n": by_n}
+        print(f"# batch: {len(res)} problems in {wall:.2f}s ({len(res) / wall:.1f}/s), "
+              f"converged {result['batched']['all_converged']}", file=sys.stderr, flush=True)
+        if args.cpu:
+            result["batched"]["cpu_baseline"] = batch_cpu_baseline(probs, res)
+            print(f"# batch cpu baseline {result['batched']['cpu_baseline']}", file=sys.stderr, flush=True)
 
     if args.cpu:
         result["cpu_baseline"] = cpu_baseline(n, h, eri, args.cpu_stride)

@@ -23,6 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # stream 0 is each host thread's own stream: solvers driven from different threads overlap
+    "-fgpu-default-stream=per-thread",
     "-Wno-unused-result",
 ]
 

@@ -67,6 +67,8 @@ class DavidsonLiu {
     DBuf<double> part_;  // reduction partials
     DBuf<double> small_; // coefficients uploaded per call
     DBuf<int64_t> pidx_;
+    DBuf<int64_t> poff_dev_;  // device copy of poff_ (bad-root offsets)
+    DBuf<double> ak_;         // Ritz coefficients of one root
     DBuf<double> pval_;
     std::vector<int64_t> poff_;
     std::vector<SparseVec> guesses_;

@@ -359,6 +359,7 @@ void DavidsonLiu::add_h_diag_device(const double* d, int64_t n) {
 }
 
 void DavidsonLiu::add_project_out_vectors(const std::vector<SparseVec>& v) {
+    poff_dev_.release();  // re-uploaded on the next projection
     poff_.assign(1, 0);
     std::vector<int64_t> idx;
     std::vector<double> val;
@@ -540,11 +541,14 @@ int DavidsonLiu::add_rows_and_orthonormalize(double* A, int rowsA, const double*
 void DavidsonLiu::project_out(double* v, int nvec) {
     int nbad = int(poff_.size()) - 1;
     if (nbad <= 0) return;
-    DBuf<int64_t> off(poff_.size());
-    off.upload(poff_.data(), poff_.size());
-    for (int k = 0; k < nvec; ++k) k_project_out<<<1, 256>>>(v + size_t(k) * size_, off.p, nbad, pidx_.p, pval_.p);
+    if (poff_dev_.n < poff_.size()) {
+        poff_dev_.alloc(poff_.size());
+        poff_dev_.upload(poff_.data(), poff_.size());
+    }
+    for (int k = 0; k < nvec; ++k)
+        k_project_out<<<1, 256>>>(v + size_t(k) * size_, poff_dev_.p, nbad, pidx_.p, pval_.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(0));
 }
 
 void DavidsonLiu::fill_random(double* v, int nvec) {
@@ -678,7 +682,8 @@ bool DavidsonLiu::solve() {
             for (int i = 0; i < bs; ++i) alpha_[size_t(i) * m + k] = V[size_t(k) * bs + i];
         }
         // form_residual_vectors + compute_residual_norm + form_correction_vectors, :412-455
-        DBuf<double> ak(bs);
+        if (ak_.n < size_t(m)) ak_.alloc(m);
+        DBuf<double>& ak = ak_;
         for (int k = 0; k < nroot_; ++k) {
             std::vector<double> a(bs);
             for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];

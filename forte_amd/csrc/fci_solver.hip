@@ -269,7 +269,7 @@ class Solver {
         binom_.upload(bt.data(), bt.size());
         build_spin_tables(sa_, n, na, binom_.p, 0);
         build_spin_tables(sb_, n, nb, binom_.p, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         ndet_ = sa_.nstr * sb_.nstr;
         eng_.reset(new SigmaEngine(&sa_, &sb_, &di_));
         fci_default_options(&opt_);
@@ -357,7 +357,7 @@ class Solver {
                 di_.g2f_off[spin].upload(g32.data(), g32.size());
                 di_.gdiag[spin].alloc(sp.nstr);
                 clique_diagonal_device(sp, di_.g2[spin].p, gd, di_.gdiag[spin].p, 0);
-                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipStreamSynchronize(0));
             }
         }
         have_ints_ = true;
@@ -388,7 +388,7 @@ class Solver {
         if (!hdiag_.p) hdiag_.alloc(ndet_);
         hdiag_device(sa_, sb_, di_.oei_a.p, di_.oei_b.p, di_.tei_aa.p, di_.tei_ab.p, di_.tei_bb.p, di_.e_core,
                      hdiag_.p, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
     }
     std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess(int num_guess_states);
     double compute_energy();
@@ -687,7 +687,7 @@ int64_t Solver::lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, 
         de.download(e + size_t(ng) * per_iter, per_iter);
         ++ng;
     }
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipStreamSynchronize(0));
     dl_.reset();  // the next compute_energy starts from a fresh guess
     return ng;
 }
@@ -841,7 +841,7 @@ int fci_rdms(fci_solver* s, int rl, int rr, int level, double* g1a, double* g1b,
         const double* R = S.dl_->eigenvector_device(rr);
         fci::DBuf<double> d1a(n2), d1b(n2);
         fci::rdm1_device(S.sa_, S.sb_, L, R, d1a.p, d1b.p, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         if (g1a) d1a.download(g1a, n2);
         if (g1b) d1b.download(g1b, n2);
         if (level >= 2) {
@@ -849,7 +849,7 @@ int fci_rdms(fci_solver* s, int rl, int rr, int level, double* g1a, double* g1b,
             fci::DBuf<double> aa(size_t(np > 0 ? np : 1) * (np > 0 ? np : 1)), bb(aa.n), ab(n4);
             fci::rdm2_same_device(S.sa_, S.sb_, L, R, aa.p, bb.p, 0);
             fci::rdm2_ab_device(S.sa_, S.sb_, L, R, ab.p, 0);
-            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipStreamSynchronize(0));
             if (g2ab) ab.download(g2ab, n4);
             // expand the pair matrices with the reference's fill (fci_vector_rdm.cc:240-285): the
             // element of the larger pair index is copied to its transpose
@@ -895,7 +895,7 @@ int fci_sigma(fci_solver* s, const double* C, double* sigma) {
         fci::DBuf<double> dc(S.ndet_), ds(S.ndet_);
         dc.upload(C, S.ndet_);
         S.sigma64(dc.p, ds.p);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         ds.download(sigma, S.ndet_);
     });
 }
@@ -1037,7 +1037,7 @@ int fci_sigma_bench(fci_solver* s, const void* C, void* sigma, int dtype, int nr
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         double acc3[3] = {0, 0, 0};
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         HIPCHK(hipEventRecord(e0, 0));
         for (int i = 0; i < nrep; ++i) {
             if (dtype == 0)
@@ -1170,7 +1170,7 @@ int fci_lp_precondition(fci_solver* s, const double* r, double lambda, double* o
         double nr = 0.0;
         for (int64_t i = 0; i < S.ndet_; ++i) nr += r[i] * r[i];
         S.lp_.apply(d.p, S.hdiag_.p, lambda, std::sqrt(nr), S.sa_, S.sb_, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         d.download(out, S.ndet_);
     });
 }
@@ -1192,7 +1192,7 @@ int fci_lp_features(fci_solver* s, const int64_t* idx, int64_t m, double lambda,
         di.upload(idx, m);
         fci::DBuf<float> dx(size_t(m) * fci::LP_NIN);
         fci::lp_features_device(S.sa_, S.sb_, di.p, m, S.hdiag_.p, lambda, d.p, std::sqrt(nr), dx.p, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         dx.download(X, size_t(m) * fci::LP_NIN);
     });
 }
@@ -1212,7 +1212,7 @@ int fci_lp_forward(const float* params, const float* X, int64_t m, float* y) {
         fci::DBuf<float> dx(size_t(m) * fci::LP_NIN);
         dx.upload(X, size_t(m) * fci::LP_NIN);
         t.forward(dx.p, m, 0);
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         t.get_y(y, m);
     });
 }
@@ -1264,7 +1264,7 @@ int fci_lp_train(float* params, const float* X, const float* d0, const float* e,
             }
             if (epoch_loss) epoch_loss[ep] = float(tot / double(ngroups));
         }
-        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipStreamSynchronize(0));
         t.get_params(params);
     });
 }

@@ -221,3 +221,18 @@ def test_errors_are_runtime_errors():
         g.rdms([(0, 0)], 3)
     with pytest.raises(RuntimeError):
         forte_amd.FCISolver(5, 2, 1, 1, forte_amd.ActiveSpaceIntegrals(4, p["h"], p["h"], p["aa"], p["ab"], p["aa"]))
+
+
+def test_batched_solves_match_oracle(oracle):
+    """Concurrent solvers on per-thread streams give the same energies as the oracle."""
+    from forte_amd.batch import seeded_batch, solve_batch
+    say("batch")
+    probs = [p for p in seeded_batch(24) if p[0] <= 8][:8]
+    res, _ = solve_batch(probs, nroot=2, threads=4, options={"E_CONVERGENCE": 1e-11, "R_CONVERGENCE": 1e-7,
+                                                              "DL_MAXITER": 300})
+    for (n, seed), r in zip(probs, res):
+        assert r["converged"]
+        h, eri = oracle.seeded_integrals(n, seed)
+        o = oracle.Oracle.from_chem(h, eri, n // 2, n // 2)
+        ref = o.solve(nroot=2, e_conv=1e-11, r_conv=1e-7, maxiter=300)["energies"]
+        assert np.abs(np.array(r["energies"]) - ref).max() < 1e-9, (n, seed)
