@@ -83,7 +83,8 @@ class DavidsonLiu {
     void lincomb(const double* X, int m, const double* coef /*m x k, col-major per output*/, int k,
                  double* out);
     double add_row_and_orthonormalize(double* A, int rowsA, const double* v_src, bool* ok);
-    double gs_fused(double* v, const double* A, int m, const std::vector<double>& d, double a,
+    double add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok);
+    double gs_fused(double* v, const double* src, const double* A, int m, const std::vector<double>& d, double a,
                     std::vector<double>& ov);
     int add_rows_and_orthonormalize(double* A, int rowsA, const double* B, int rowsB);
     void project_out(double* v, int nvec);

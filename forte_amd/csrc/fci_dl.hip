@@ -130,38 +130,39 @@ __global__ __launch_bounds__(DL_BS) void k_dots8(const double* __restrict__ x, c
     }
 }
 
-// Fused classical Gram-Schmidt step: v <- a v - sum_i d_i A_i (i < m), then the partial sums of
-// ||v||^2 (slot m) and v . A_i (slots i < m) of the updated v, in one pass over A.
+// Fused classical Gram-Schmidt step: v <- a src - sum_i d_i A_i (i < m; src may be v), then the
+// partial sums of ||v||^2 (slot m) and v . A_i (slots i < m) of the updated v, in one pass over A.
 constexpr int GS_MAXM = 32;
-__global__ __launch_bounds__(DL_BS) void k_gs_fused(double* __restrict__ v, const double* __restrict__ A, int m,
-                                                    const double* __restrict__ d, double a, int64_t n,
+template <int GS_M>  // compile-time bound on m: the accumulators live in registers
+__global__ __launch_bounds__(DL_BS) void k_gs_fused(double* v, const double* src, const double* __restrict__ A,
+                                                    int m, const double* __restrict__ d, double a, int64_t n,
                                                     double* __restrict__ part) {
-    __shared__ double dd[GS_MAXM];
-    __shared__ double red[GS_MAXM + 1][DL_BS / 64];
+    __shared__ double dd[GS_M];
+    __shared__ double red[GS_M + 1][DL_BS / 64];
     for (int i = threadIdx.x; i < m; i += DL_BS) dd[i] = d[i];
     __syncthreads();
-    double acc[GS_MAXM + 1];
+    double acc[GS_M + 1];
 #pragma unroll
-    for (int i = 0; i <= GS_MAXM; ++i) acc[i] = 0.0;
+    for (int i = 0; i <= GS_M; ++i) acc[i] = 0.0;
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
-        double av[GS_MAXM];
-        double x = a * v[e];
+        double av[GS_M];
+        double x = a * src[e];
 #pragma unroll
-        for (int i = 0; i < GS_MAXM; ++i)
+        for (int i = 0; i < GS_M; ++i)
             if (i < m) {
                 av[i] = A[size_t(i) * n + e];
                 x -= dd[i] * av[i];
             }
         v[e] = x;
-        acc[GS_MAXM] += x * x;
+        acc[GS_M] += x * x;
 #pragma unroll
-        for (int i = 0; i < GS_MAXM; ++i)
+        for (int i = 0; i < GS_M; ++i)
             if (i < m) acc[i] += x * av[i];
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i <= GS_MAXM; ++i) {
-        if (i < m || i == GS_MAXM) {
+    for (int i = 0; i <= GS_M; ++i) {
+        if (i < m || i == GS_M) {
             double t = acc[i];
             for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
             if (lane == 0) red[i][wave] = t;
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(DL_BS) void k_gs_fused(double* __restrict__ v, cons
     }
     __syncthreads();
     for (int i = threadIdx.x; i <= m; i += DL_BS) {
-        const int src = i < m ? i : GS_MAXM;
+        const int src = i < m ? i : GS_M;
         double t = 0.0;
         for (int w = 0; w < DL_BS / 64; ++w) t += red[src][w];
         part[size_t(blockIdx.x) * (m + 1) + i] = t;
@@ -412,12 +413,17 @@ void DavidsonLiu::dots(const double* x, const double* Y, int m, double* out) {
 }
 
 // v <- a v - sum_i d_i A_i; returns ||v||^2 and ov[i] = v . A_i of the updated v
-double DavidsonLiu::gs_fused(double* v, const double* A, int m, const std::vector<double>& d, double a,
-                             std::vector<double>& ov) {
+double DavidsonLiu::gs_fused(double* v, const double* src, const double* A, int m, const std::vector<double>& d,
+                             double a, std::vector<double>& ov) {
     const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
     small_.upload(d.data(), std::max(m, 1));
     double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
-    k_gs_fused<<<nb, DL_BS>>>(v, A, m, small_.p, a, size_, part_.p);
+    if (m <= 8)
+        k_gs_fused<8><<<nb, DL_BS>>>(v, src, A, m, small_.p, a, size_, part_.p);
+    else if (m <= 16)
+        k_gs_fused<16><<<nb, DL_BS>>>(v, src, A, m, small_.p, a, size_, part_.p);
+    else
+        k_gs_fused<GS_MAXM><<<nb, DL_BS>>>(v, src, A, m, small_.p, a, size_, part_.p);
     k_reduce_parts<<<m + 1, 256>>>(part_.p, nb, m + 1, res);
     HIPCHK(hipGetLastError());
     std::vector<double> out(m + 1);
@@ -446,53 +452,60 @@ void DavidsonLiu::rotate(const double* X, double* out, int kout) {
     lincomb(X, basis_size_, coef.data(), kout, out);
 }
 
+// Gram-Schmidt of vsrc against the rows of A into row rowsA, in fused passes: each cycle
+// subtracts the projections found by the previous pass and, in the same sweep, measures the norm
+// and the overlaps of the result; the normalization of a cycle is folded into the next update
+// (v <- v / |v| - sum_i (ov_i / |v|) A_i). The first pass reads vsrc directly (no copy).
+double DavidsonLiu::add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok) {
+    double* v = A + size_t(rowsA) * size_;
+    std::vector<double> d(std::max(rowsA, 1));
+    *ok = false;
+    if (rowsA > 0) dots(vsrc, A, rowsA, d.data());
+    double a = 1.0;
+    std::vector<double> ov;
+    const double* src = vsrc;
+    for (int cycle = 0; cycle < 10; ++cycle) {
+        const double s2 = gs_fused(v, src, A, rowsA, d, a, ov);
+        src = v;
+        const double normval = std::sqrt(s2);
+        if (normval < discard_) return 0.0;
+        // after scaling by 1/normval: overlaps ov_i / normval, norm 1 up to rounding
+        double mo = 0.0;
+        for (int i = 0; i < rowsA; ++i) mo = std::max(mo, std::fabs(ov[i] / normval));
+        if (mo < orth_) {
+            k_scale<<<nblk(size_, 256), 256>>>(v, size_, 1. / normval);
+            HIPCHK(hipGetLastError());
+            double nn = 0.0;
+            dots(v, v, 1, &nn);
+            if (std::fabs(nn - 1.0) >= orth_) {
+                a = 1.0;
+                std::fill(d.begin(), d.end(), 0.0);
+                continue;
+            }
+            *ok = true;
+            int m = subspace_size_;
+            if (A == b_.p) {
+                for (int i = 0; i < rowsA; ++i) S_[size_t(i) * m + rowsA] = S_[size_t(rowsA) * m + i] = ov[i] / normval;
+                S_[size_t(rowsA) * m + rowsA] = nn;
+            }
+            return 1.0;
+        }
+        a = 1.0 / normval;
+        for (int i = 0; i < rowsA; ++i) d[i] = ov[i] / normval;
+    }
+    return 0.0;
+}
+
 // add_row_and_orthonormalize (davidson_liu_solver.cc:648-687): returns norm check pass via ok.
 // Gram-Schmidt against the rows of A is applied to all rows at once per cycle (classical GS);
 // the cycle structure and thresholds are the reference's.
 double DavidsonLiu::add_row_and_orthonormalize(double* A, int rowsA, const double* vsrc, bool* ok) {
     double* v = A + size_t(rowsA) * size_;
+    if (rowsA <= GS_MAXM) return add_row_fused(A, rowsA, vsrc, ok);
     if (v != vsrc) HIPCHK(hipMemcpy(v, vsrc, size_ * sizeof(double), hipMemcpyDeviceToDevice));
     int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
     std::vector<double> d(std::max(rowsA, 1)), parts(nb);
     *ok = false;
-    if (rowsA <= GS_MAXM) {
-        // Fused passes: each cycle subtracts the projections found by the previous pass and, in the
-        // same sweep, measures the norm and the overlaps of the result; the normalization of a
-        // cycle is folded into the next update (v <- v / |v| - sum_i (ov_i / |v|) A_i).
-        if (rowsA > 0) dots(v, A, rowsA, d.data());
-        double a = 1.0;
-        std::vector<double> ov;
-        for (int cycle = 0; cycle < 10; ++cycle) {
-            const double s2 = gs_fused(v, A, rowsA, d, a, ov);
-            const double normval = std::sqrt(s2);
-            if (normval < discard_) return 0.0;
-            // after scaling by 1/normval: overlaps ov_i / normval, norm 1 up to rounding
-            double mo = 0.0;
-            for (int i = 0; i < rowsA; ++i) mo = std::max(mo, std::fabs(ov[i] / normval));
-            if (mo < orth_) {
-                k_scale<<<nblk(size_, 256), 256>>>(v, size_, 1. / normval);
-                HIPCHK(hipGetLastError());
-                double nn = 0.0;
-                dots(v, v, 1, &nn);
-                if (std::fabs(nn - 1.0) >= orth_) {
-                    a = 1.0;
-                    std::fill(d.begin(), d.end(), 0.0);
-                    continue;
-                }
-                *ok = true;
-                int m = subspace_size_;
-                if (A == b_.p) {
-                    for (int i = 0; i < rowsA; ++i)
-                        S_[size_t(i) * m + rowsA] = S_[size_t(rowsA) * m + i] = ov[i] / normval;
-                    S_[size_t(rowsA) * m + rowsA] = nn;
-                }
-                return 1.0;
-            }
-            a = 1.0 / normval;
-            for (int i = 0; i < rowsA; ++i) d[i] = ov[i] / normval;
-        }
-        return 0.0;
-    }
     for (int cycle = 0; cycle < 10; ++cycle) {
         if (rowsA > 0) dots(v, A, rowsA, d.data());
         small_.upload(d.data(), std::max(rowsA, 1));
