@@ -1563,6 +1563,14 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
             const char* e = std::getenv("FCI_AB_LAYOUT");
             return e ? std::atoi(e) : 8;
         }();
+        if (layout == 3 && (nmt + 2) / 3 * ks <= 6) {
+            const int tpw = (nmt + 2) / 3;
+#define MXCASE(K, T) \
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 3, 4>(C, S, a, b, ints, st);
+            MXCASE(1, 1) MXCASE(1, 2) MXCASE(1, 3) MXCASE(1, 4) MXCASE(2, 1) MXCASE(2, 2) MXCASE(2, 3)
+            MXCASE(3, 1) MXCASE(3, 2)
+#undef MXCASE
+        }
         if (layout == 12 && (nmt + 3) / 4 * ks <= 6) {
             const int tpw = (nmt + 3) / 4;
 #define MXCASE(K, T) \
