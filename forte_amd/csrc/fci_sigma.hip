@@ -420,6 +420,12 @@ __global__ __launch_bounds__(64 * CQ2_WAVES) void k_clique2_mixed(const float* _
 // W_K[m][m'] = g'[pair(m)][pair(m')] in registers and streams 16-column tiles of
 // X[I_m'][c] straight from global memory into the B operand layout, one tile ahead.
 // E[(K * M + m) * Ls + c - cbeg] = s_m sum_m' W_K[m][m'] s_m' X[I_m'][c].
+// Row of a gathered string for an unconditional load: an invalid entry reads row 0, whose
+// contribution the zero A-operand column of that entry cancels. Keeping the gather loads free of
+// per-lane branches lets all of a tile's loads be in flight at once (a load whose result is used
+// inside a divergent branch is waited for on the spot).
+__device__ __forceinline__ int64_t gather_row(uint32_t e) { return e != kInvalid ? int64_t(e & kAddrMask) : 0; }
+
 constexpr int CQ3_CW = 64;  // columns per work item
 
 template <int NMT, bool GL>  // row tiles of 16 members; k steps of 32 members: (NMT + 1) / 2; GL: g' in LDS
@@ -463,7 +469,8 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t e = mb[ks][x];
-                const float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                if (e >> 31) v = -v;  // sign of the gathered row folded into A
                 __bf16 h, l;
                 split_bf16(v, h, l);
                 ah[mt][ks][x] = h;
@@ -481,19 +488,11 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
     const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
     float xv[NKS][8], xn[NKS][8];
     auto load = [&](int64_t col, float (&xx)[NKS][8]) {
-        const bool ok = col < cend;
+        const int64_t cc = col < cend ? col : cbeg;  // columns past the end are computed, not stored
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
-            for (int x = 0; x < 8; ++x) {
-                const uint32_t e = mb[ks][x];
-                float v = 0.f;
-                if (ok && e != kInvalid) {
-                    v = X[int64_t(e & kAddrMask) * L + col];
-                    if (e >> 31) v = -v;
-                }
-                xx[ks][x] = v;
-            }
+            for (int x = 0; x < 8; ++x) xx[ks][x] = X[gather_row(mb[ks][x]) * L + cc];
     };
     load(c0 + cl, xv);
 #pragma unroll 1
@@ -573,7 +572,8 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t e = mb[ks][x];
-                const float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                if (e >> 31) v = -v;  // sign of the gathered row folded into A
                 __bf16 h, l;
                 split_bf16(v, h, l);
                 ah[mt][ks][x] = h;
@@ -591,19 +591,12 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
     const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
     float2 xv[NKS][8], xn[NKS][8];
     auto load = [&](int64_t col, float2 (&xx)[NKS][8]) {
-        const bool ok = col < cend;  // col even, so col + 1 < cend as well
+        const int64_t cc = col < cend ? col : cbeg;  // col, cbeg even; past the end: not stored
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
-            for (int x = 0; x < 8; ++x) {
-                const uint32_t e = mb[ks][x];
-                float2 v = make_float2(0.f, 0.f);
-                if (ok && e != kInvalid) {
-                    v = *reinterpret_cast<const float2*>(X + int64_t(e & kAddrMask) * L + col);
-                    if (e >> 31) v = make_float2(-v.x, -v.y);
-                }
-                xx[ks][x] = v;
-            }
+            for (int x = 0; x < 8; ++x)
+                xx[ks][x] = *reinterpret_cast<const float2*>(X + gather_row(mb[ks][x]) * L + cc);
     };
     auto compute = [&](const float2 (&x)[NKS][8], int64_t col) {
         bf8 bh0[NKS], bl0[NKS], bh1[NKS], bl1[NKS];
@@ -688,7 +681,8 @@ __global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ 
 #pragma unroll
         for (int ks = 0; ks < NK4; ++ks) {
             const uint32_t e = mb[ks];
-            va[mt][ks] = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.0;
+            const double v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.0;
+            va[mt][ks] = (e >> 31) ? -v : v;  // sign of the gathered row folded into A
         }
         uint32_t nb = 0;
 #pragma unroll
@@ -702,17 +696,9 @@ __global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ 
     const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
     double xv[NK4], xn[NK4];
     auto load = [&](int64_t col, double (&xx)[NK4]) {
-        const bool ok = col < cend;
+        const int64_t cc = col < cend ? col : cbeg;  // columns past the end are computed, not stored
 #pragma unroll
-        for (int ks = 0; ks < NK4; ++ks) {
-            const uint32_t e = mb[ks];
-            double v = 0.0;
-            if (ok && e != kInvalid) {
-                v = X[int64_t(e & kAddrMask) * L + col];
-                if (e >> 31) v = -v;
-            }
-            xx[ks] = v;
-        }
+        for (int ks = 0; ks < NK4; ++ks) xx[ks] = X[gather_row(mb[ks]) * L + cc];
     };
     auto compute = [&](const double (&xx)[NK4], int64_t col) {
 #pragma unroll
@@ -1047,16 +1033,26 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
     extern __shared__ __attribute__((aligned(16))) double srow_m[];  // [Nb]
     double* srow = srow_m;
     __shared__ uint32_t ent[KS * 32];
+    __shared__ float vdg[12 * 16];
     constexpr int NTH = 64 * WM * WC;
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave % WM, wc = wave / WM;
     const int g = lane >> 4, cl = lane & 15;
     for (int i = tid; i < KS * 32; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : kInvalid;
     for (int64_t i = tid; i < Nb; i += NTH) srow[i] = 0.0;
-    __syncthreads();
     const uint32_t occ = str_a[Ja];
+    for (int r = tid; r < nmt * 16; r += NTH) {  // diagonal excitations: sum_{q in Ja} (qq|r)
+        float d = 0.f;
+        if (r < np)
+            for (uint32_t o = occ; o; o &= o - 1u) {
+                const int q = __ffs(o) - 1;
+                d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+            }
+        vdg[r] = d;
+    }
+    __syncthreads();
     bf8 ah[TPW][KS], al[TPW][KS];
     float vd[TPW][4];
 #pragma unroll
@@ -1068,8 +1064,10 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const int e = ks * 32 + 8 * g + x;
-                float v = 0.f;
-                if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
+                const uint32_t en = ent[e];
+                const bool use = mtile < nmt && rs < np && e < ns && en != kInvalid;
+                const float t = vpk[use ? size_t((en >> 21) & 0x3FFu) * np + rs : 0];  // unconditional load
+                const float v = use ? ((en >> 31) ? -t : t) : 0.f;  // sign of the gathered row folded into A
                 __bf16 h, l;
                 split_bf16(v, h, l);
                 ah[j][ks][x] = h;
@@ -1077,47 +1075,28 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = mtile * 16 + 4 * g + v;
-            float d = 0.f;
-            if (mtile < nmt && r < np) {
-                uint32_t o = occ;
-                while (o) {
-                    const int q = __ffs(o) - 1;
-                    o &= o - 1u;
-                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
-                }
-            }
-            vd[j][v] = d;
-        }
+        for (int v = 0; v < 4; ++v) vd[j][v] = mtile < nmt ? vdg[mtile * 16 + 4 * g + v] : 0.f;
     }
     const float* Crow = C + Ja * Nb;
     const int ntile = npp >> 4;
     for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 32 * WC) {
         const int64_t Ib = c0 + 2 * cl;  // and Ib + 1 (Nb even)
         const bool ok = Ib < Nb;
+        const int64_t Ibc = ok ? Ib : 0;  // columns past the end: computed, never scattered
         uint4 tg[TPW];
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            const int mtile = wm + WM * j;
-            tg[j] = (ok && mtile < nmt)
-                        ? *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtile) * 64 + lane) * 8)
-                        : make_uint4(~0u, ~0u, ~0u, ~0u);
+            const int mtile = wm + WM * j;  // the table row c0 / 32 exists for all lanes; tiles past nmt
+            const int mtc = mtile < nmt ? mtile : 0;  // are skipped below (wave-uniform)
+            tg[j] = *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtc) * 64 + lane) * 8);
         }
         float2 xv[KS][8];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-            for (int xx = 0; xx < 8; ++xx) {
-                const uint32_t e = ent[ks * 32 + 8 * g + xx];
-                float2 v = make_float2(0.f, 0.f);
-                if (ok && e != kInvalid) {
-                    v = *reinterpret_cast<const float2*>(C + int64_t(e & kAddrMask) * Nb + Ib);
-                    if (e >> 31) v = make_float2(-v.x, -v.y);
-                }
-                xv[ks][xx] = v;
-            }
-        const float2 cd = ok ? *reinterpret_cast<const float2*>(Crow + Ib) : make_float2(0.f, 0.f);
+            for (int xx = 0; xx < 8; ++xx)
+                xv[ks][xx] = *reinterpret_cast<const float2*>(C + gather_row(ent[ks * 32 + 8 * g + xx]) * Nb + Ibc);
+        const float2 cd = *reinterpret_cast<const float2*>(Crow + Ibc);
         bf8 bh0[KS], bl0[KS], bh1[KS], bl1[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
@@ -1133,6 +1112,7 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
             }
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
+            if (wm + WM * j >= nmt) continue;  // wave-uniform
             f4 a0 = f4{0.f, 0.f, 0.f, 0.f}, a1 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
@@ -1149,11 +1129,154 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
-                    if ((tt & 0x7FFFu) == kInv16) continue;
+                    if (!ok || (tt & 0x7FFFu) == kInv16) continue;
                     const float a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
                     atomicAdd(&srow[tt & 0x7FFFu], double((tt >> 15) ? -a : a));
                 }
         }
+    }
+    __syncthreads();
+    float* Srow = S + Ja * Nb;
+    for (int64_t i = tid; i < Nb; i += NTH) Srow[i] = float(double(Srow[i]) + srow[i]);
+}
+
+// Variant with the A operand in LDS (built once per block, read per MFMA) so that the registers it
+// frees hold a second set of gathered rows and targets: the loads of the next 32-column tile are
+// in flight while the current one is multiplied and scattered (two buffers used alternately).
+template <int KS, int TPW, int WM, int WC>
+__global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
+                                                      int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
+                                                      const uint32_t* __restrict__ str_a,
+                                                      const uint16_t* __restrict__ outT, int npp, int np,
+                                                      const float* __restrict__ vpk, int nmt) {
+    extern __shared__ __attribute__((aligned(16))) double srow_m3[];  // [Nb] then A operands
+    double* srow = srow_m3;
+    bf8* Ash = reinterpret_cast<bf8*>(srow + ((Nb + 1) & ~int64_t(1)));  // [nmt][KS][2][64]
+    __shared__ uint32_t ent[KS * 32];
+    __shared__ float vdg[12 * 16];
+    constexpr int NTH = 64 * WM * WC, NW = WM * WC;
+    const int64_t Ja = xcd_remap(blockIdx.x, Na);
+    if (Ja >= Na) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % WM, wc = wave / WM;
+    const int g = lane >> 4, cl = lane & 15;
+    for (int i = tid; i < KS * 32; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : kInvalid;
+    for (int64_t i = tid; i < Nb; i += NTH) srow[i] = 0.0;
+    const uint32_t occ = str_a[Ja];
+    for (int r = tid; r < nmt * 16; r += NTH) {  // diagonal excitations: sum_{q in Ja} (qq|r)
+        float d = 0.f;
+        if (r < np)
+            for (uint32_t o = occ; o; o &= o - 1u) {
+                const int q = __ffs(o) - 1;
+                d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+            }
+        vdg[r] = d;
+    }
+    __syncthreads();
+    // A operand slots (wm, j) built by one wave each
+    for (int slot = wave; slot < nmt; slot += NW) {
+        const int mtile = slot;
+        const int rs = mtile * 16 + cl;
+        for (int ks = 0; ks < KS; ++ks) {
+            bf8 h8, l8;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const int e = ks * 32 + 8 * g + x;
+                const uint32_t en = ent[e];
+                const bool use = rs < np && e < ns && en != kInvalid;
+                const float t = vpk[use ? size_t((en >> 21) & 0x3FFu) * np + rs : 0];
+                const float v = use ? ((en >> 31) ? -t : t) : 0.f;
+                __bf16 h, l;
+                split_bf16(v, h, l);
+                h8[x] = h;
+                l8[x] = l;
+            }
+            Ash[((slot * KS + ks) * 2 + 0) * 64 + lane] = h8;
+            Ash[((slot * KS + ks) * 2 + 1) * 64 + lane] = l8;
+        }
+    }
+    float vd[TPW][4];
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const int mtile = wm + WM * j;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) vd[j][v] = mtile < nmt ? vdg[mtile * 16 + 4 * g + v] : 0.f;
+    }
+    __syncthreads();
+    const float* Crow = C + Ja * Nb;
+    const int ntile = npp >> 4;
+    auto load = [&](int64_t c0, float2 (&x)[KS][8], uint4 (&t)[TPW], float2& d) {
+        const int64_t Ib = c0 + 2 * cl;
+        const bool ok = Ib < Nb;
+        const int64_t Ibc = ok ? Ib : 0;
+        const int64_t crow = c0 < Nb ? (c0 >> 5) : 0;  // the prefetch past the last tile reads row 0
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            const int mtile = wm + WM * j;
+            const int mtc = mtile < nmt ? mtile : 0;  // tiles past nmt are skipped (wave-uniform)
+            t[j] = *reinterpret_cast<const uint4*>(outT + ((crow * ntile + mtc) * 64 + lane) * 8);
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx)
+                x[ks][xx] = *reinterpret_cast<const float2*>(C + gather_row(ent[ks * 32 + 8 * g + xx]) * Nb + Ibc);
+        d = *reinterpret_cast<const float2*>(Crow + Ibc);
+    };
+    auto compute = [&](const float2 (&x)[KS][8], const uint4 (&t)[TPW], float2 cd, int64_t c0) {
+        const bool ok = c0 + 2 * cl < Nb;
+        bf8 bh0[KS], bl0[KS], bh1[KS], bl1[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int xx = 0; xx < 8; ++xx) {
+                __bf16 h, l;
+                split_bf16(x[ks][xx].x, h, l);
+                bh0[ks][xx] = h;
+                bl0[ks][xx] = l;
+                split_bf16(x[ks][xx].y, h, l);
+                bh1[ks][xx] = h;
+                bl1[ks][xx] = l;
+            }
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
+            const int slot = wm + WM * j;
+            if (slot >= nmt) continue;  // wave-uniform
+            f4 a0 = f4{0.f, 0.f, 0.f, 0.f}, a1 = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const bf8 ah = Ash[((slot * KS + ks) * 2 + 0) * 64 + lane];
+                const bf8 al = Ash[((slot * KS + ks) * 2 + 1) * 64 + lane];
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl1[ks], a1, 0, 0, 0);
+                a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh0[ks], a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh1[ks], a1, 0, 0, 0);
+            }
+            const uint32_t tw[4] = {t[j].x, t[j].y, t[j].z, t[j].w};
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
+                    if (!ok || (tt & 0x7FFFu) == kInv16) continue;
+                    const float a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
+                    atomicAdd(&srow[tt & 0x7FFFu], double((tt >> 15) ? -a : a));
+                }
+        }
+    };
+    float2 xa[KS][8], xb[KS][8];
+    uint4 ta[TPW], tb[TPW];
+    float2 da, db;
+    const int64_t step = 32 * WC;
+    load(int64_t(wc) * 32, xa, ta, da);
+    for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 2 * step) {
+        load(c0 + step, xb, tb, db);
+        compute(xa, ta, da, c0);
+        if (c0 + step >= Nb) break;
+        load(c0 + 2 * step, xa, ta, da);
+        compute(xb, tb, db, c0 + step);
     }
     __syncthreads();
     float* Srow = S + Ja * Nb;
@@ -1171,16 +1294,26 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
                                                     const double* __restrict__ vpk, int nmt) {
     extern __shared__ __attribute__((aligned(16))) double srow64[];  // [Nb]
     __shared__ uint32_t ent[KS * 4];
+    __shared__ double vdg[12 * 16];
     constexpr int WM = 4, WC = 2, NTH = 64 * WM * WC;
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave % WM, wc = wave / WM;
     const int g = lane >> 4, cl = lane & 15;
     for (int i = tid; i < KS * 4; i += NTH) ent[i] = i < ns ? into_a[Ja * ns + i] : kInvalid;
     for (int64_t i = tid; i < Nb; i += NTH) srow64[i] = 0.0;
-    __syncthreads();
     const uint32_t occ = str_a[Ja];
+    for (int r = tid; r < nmt * 16; r += NTH) {  // diagonal excitations: sum_{q in Ja} (qq|r)
+        double d = 0.0;
+        if (r < np)
+            for (uint32_t o = occ; o; o &= o - 1u) {
+                const int q = __ffs(o) - 1;
+                d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
+            }
+        vdg[r] = d;
+    }
+    __syncthreads();
     double va[TPW][KS];
     double vd[TPW][4];
 #pragma unroll
@@ -1190,52 +1323,35 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int e = ks * 4 + g;
-            double v = 0.0;
-            if (mtile < nmt && rs < np && e < ns) v = vpk[size_t((ent[e] >> 21) & 0x3FFu) * np + rs];
-            va[j][ks] = v;
+            const uint32_t en = ent[e];
+            const bool use = mtile < nmt && rs < np && e < ns && en != kInvalid;
+            const double t = vpk[use ? size_t((en >> 21) & 0x3FFu) * np + rs : 0];  // unconditional load
+            va[j][ks] = use ? ((en >> 31) ? -t : t) : 0.0;  // sign of the gathered row folded into A
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = mtile * 16 + 4 * g + v;
-            double d = 0.0;
-            if (mtile < nmt && r < np) {
-                uint32_t o = occ;
-                while (o) {
-                    const int q = __ffs(o) - 1;
-                    o &= o - 1u;
-                    d += vpk[size_t(q * (q + 1) / 2 + q) * np + r];
-                }
-            }
-            vd[j][v] = d;
-        }
+        for (int v = 0; v < 4; ++v) vd[j][v] = mtile < nmt ? vdg[mtile * 16 + 4 * g + v] : 0.0;
     }
     const double* Crow = C + Ja * Nb;
     const int ntile = npp >> 4;
     for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 32 * WC) {
         const int64_t Ib = c0 + 2 * cl;
         const bool ok = Ib < Nb;
+        const int64_t Ibc = ok ? Ib : 0;  // columns past the end: computed, never scattered
         uint4 tg[TPW];
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            const int mtile = wm + WM * j;
-            tg[j] = (ok && mtile < nmt)
-                        ? *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtile) * 64 + lane) * 8)
-                        : make_uint4(~0u, ~0u, ~0u, ~0u);
+            const int mtile = wm + WM * j;  // the table row c0 / 32 exists for all lanes; tiles past nmt
+            const int mtc = mtile < nmt ? mtile : 0;  // are skipped below (wave-uniform)
+            tg[j] = *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtc) * 64 + lane) * 8);
         }
         double2 xv[KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const uint32_t e = ent[ks * 4 + g];
-            double2 v = make_double2(0.0, 0.0);
-            if (ok && e != kInvalid) {
-                v = *reinterpret_cast<const double2*>(C + int64_t(e & kAddrMask) * Nb + Ib);
-                if (e >> 31) v = make_double2(-v.x, -v.y);
-            }
-            xv[ks] = v;
-        }
-        const double2 cd = ok ? *reinterpret_cast<const double2*>(Crow + Ib) : make_double2(0.0, 0.0);
+        for (int ks = 0; ks < KS; ++ks)
+            xv[ks] = *reinterpret_cast<const double2*>(C + gather_row(ent[ks * 4 + g]) * Nb + Ibc);
+        const double2 cd = *reinterpret_cast<const double2*>(Crow + Ibc);
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
+            if (wm + WM * j >= nmt) continue;  // wave-uniform
             d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
@@ -1248,7 +1364,7 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
-                    if ((tt & 0x7FFFu) == kInv16) continue;
+                    if (!ok || (tt & 0x7FFFu) == kInv16) continue;
                     const double a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
                     atomicAdd(&srow64[tt & 0x7FFFu], (tt >> 15) ? -a : a);
                 }
@@ -1512,6 +1628,22 @@ static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, cons
 template <int KS, int TPW, int WM, int WC>
 static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                              const DevInts& ints, hipStream_t st) {
+    static const bool v3 = [] {
+        const char* e = std::getenv("FCI_AB_KERNEL");
+        return e && std::atoi(e) == 3;
+    }();
+    if (v3) {
+        const int nmt = (b.np + 15) / 16;
+        const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * sizeof(double) + size_t(nmt) * KS * 2 * 64 * 16;
+        if (nmt <= WM * TPW && sh3 + KS * 32 * 4 <= 160 * 1024) {
+            auto kern3 = k_ab3_mixed<KS, TPW, WM, WC>;
+            HIPCHK(hipFuncSetAttribute((const void*)kern3, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh3)));
+            kern3<<<xcd_grid(a.nstr), 64 * WM * WC, sh3, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p,
+                                                               b.outT.p, b.npp, b.np, ints.vpk32.p, nmt);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+    }
     const size_t sh = size_t(b.nstr) * sizeof(double);
     auto kern = k_ab2_mixed<KS, TPW, WM, WC>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
