@@ -456,20 +456,22 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
             const int m = ks * 32 + 8 * g + x;
-            mb[ks][x] = m < M ? ent[m] : kInvalid;
+            const uint32_t t = ent[m < M ? m : 0];  // unconditional loads (M >= 1): all in flight at once
+            mb[ks][x] = m < M ? t : kInvalid;
         }
     bf8 ah[NMT][NKS], al[NMT][NKS];
     uint32_t neg[NMT];  // sign bits of this lane's 4 output rows, bit v
 #pragma unroll
     for (int mt = 0; mt < NMT; ++mt) {
         const int r = mt * 16 + cl;
-        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+        const int pr = int((ent[r < M ? r : 0] >> 21) & 0x3FFu);  // rows r >= M get zero below
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t e = mb[ks][x];
-                float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                const float t = G[pr * gdim + (e != kInvalid ? int((e >> 21) & 0x3FFu) : 0)];
+                float v = (r < M && e != kInvalid) ? t : 0.f;
                 if (e >> 31) v = -v;  // sign of the gathered row folded into A
                 __bf16 h, l;
                 split_bf16(v, h, l);
@@ -480,7 +482,8 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int ro = mt * 16 + 4 * g + v;
-            if (ro < M) nb |= (ent[ro] >> 31) << v;
+            const uint32_t t = ent[ro < M ? ro : 0];
+            nb |= (ro < M ? t >> 31 : 0u) << v;
         }
         neg[mt] = nb;
     }
@@ -542,16 +545,18 @@ template <int NMT>
 __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict__ X, float* __restrict__ E, int64_t L,
                                                         const uint32_t* __restrict__ clq, int M, int64_t nK,
                                                         const float* __restrict__ G, int gdim, int nchunk,
-                                                        int64_t cbeg, int64_t Ls) {
+                                                        int64_t cbeg, int64_t Ls, int cpw) {
     constexpr int NKS = (NMT + 1) / 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
-    const int64_t nwork = nK * nchunk;
+    const int ngrp = (nchunk + cpw - 1) / cpw;  // work item: one K and up to cpw consecutive chunks
+    const int64_t nwork = nK * ngrp;
     const int64_t nblk = (nwork + 3) / 4;
     const int64_t w = xcd_remap(blockIdx.x, nblk) * 4 + wave;
     if (w >= nwork) return;
     const int64_t K = w % nK;
-    const int chunk = int(w / nK);
+    const int ch0 = int(w / nK) * cpw;
+    const int nh = 2 * min(cpw, nchunk - ch0);  // 32-column halves of this work item
     const uint32_t* ent = clq + K * M;
     uint32_t mb[NKS][8];
 #pragma unroll
@@ -559,20 +564,22 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
             const int m = ks * 32 + 8 * g + x;
-            mb[ks][x] = m < M ? ent[m] : kInvalid;
+            const uint32_t t = ent[m < M ? m : 0];  // unconditional loads (M >= 1): all in flight at once
+            mb[ks][x] = m < M ? t : kInvalid;
         }
     bf8 ah[NMT][NKS], al[NMT][NKS];
     uint32_t neg[NMT];
 #pragma unroll
     for (int mt = 0; mt < NMT; ++mt) {
         const int r = mt * 16 + cl;
-        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+        const int pr = int((ent[r < M ? r : 0] >> 21) & 0x3FFu);  // rows r >= M get zero below
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
                 const uint32_t e = mb[ks][x];
-                float v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.f;
+                const float t = G[pr * gdim + (e != kInvalid ? int((e >> 21) & 0x3FFu) : 0)];
+                float v = (r < M && e != kInvalid) ? t : 0.f;
                 if (e >> 31) v = -v;  // sign of the gathered row folded into A
                 __bf16 h, l;
                 split_bf16(v, h, l);
@@ -583,12 +590,12 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int ro = mt * 16 + 4 * g + v;
-            if (ro < M) nb |= (ent[ro] >> 31) << v;
+            const uint32_t t = ent[ro < M ? ro : 0];
+            nb |= (ro < M ? t >> 31 : 0u) << v;
         }
         neg[mt] = nb;
     }
     const int64_t cend = cbeg + Ls < L ? cbeg + Ls : L;  // even
-    const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
     float2 xv[NKS][8], xn[NKS][8];
     auto load = [&](int64_t col, float2 (&xx)[NKS][8]) {
         const int64_t cc = col < cend ? col : cbeg;  // col, cbeg even; past the end: not stored
@@ -598,7 +605,10 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
             for (int x = 0; x < 8; ++x)
                 xx[ks][x] = *reinterpret_cast<const float2*>(X + gather_row(mb[ks][x]) * L + cc);
     };
-    auto compute = [&](const float2 (&x)[NKS][8], int64_t col) {
+    auto colof = [&](int h) { return cbeg + int64_t(ch0 + (h >> 1)) * CQ3_CW + (h & 1) * 32 + 2 * cl; };
+    auto compute = [&](const float2 (&x)[NKS][8], int h) {
+        const int chunk = ch0 + (h >> 1);
+        const int64_t col = colof(h), c0 = cbeg + int64_t(chunk) * CQ3_CW;
         bf8 bh0[NKS], bl0[NKS], bh1[NKS], bl1[NKS];
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
@@ -637,11 +647,16 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
             }
         }
     };
-    // the two 32-column halves of the chunk, the second one's loads in flight during the first
-    load(c0 + 2 * cl, xv);
-    load(c0 + 32 + 2 * cl, xn);
-    compute(xv, c0 + 2 * cl);
-    compute(xn, c0 + 32 + 2 * cl);
+    // halves in two buffers used alternately: the loads of the next half are in flight while the
+    // current one is multiplied and stored
+    load(colof(0), xv);
+#pragma unroll 1
+    for (int h = 0; h < nh; h += 2) {
+        load(colof(h + 1), xn);
+        compute(xv, h);
+        if (h + 2 < nh) load(colof(h + 2), xv);
+        compute(xn, h + 1);
+    }
 }
 
 template <int NK4, bool GL>  // k steps of 4 members; row tiles of 16: (NK4 + 3) / 4; GL: g' in LDS
@@ -670,25 +685,28 @@ __global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ 
 #pragma unroll
     for (int ks = 0; ks < NK4; ++ks) {
         const int m = ks * 4 + g;
-        mb[ks] = m < M ? ent[m] : kInvalid;
+        const uint32_t t = ent[m < M ? m : 0];  // unconditional loads (M >= 1): all in flight at once
+        mb[ks] = m < M ? t : kInvalid;
     }
     double va[NMT][NK4];
     uint32_t neg[NMT];
 #pragma unroll
     for (int mt = 0; mt < NMT; ++mt) {
         const int r = mt * 16 + cl;
-        const int pr = r < M ? int((ent[r] >> 21) & 0x3FFu) : 0;
+        const int pr = int((ent[r < M ? r : 0] >> 21) & 0x3FFu);  // rows r >= M get zero below
 #pragma unroll
         for (int ks = 0; ks < NK4; ++ks) {
             const uint32_t e = mb[ks];
-            const double v = (r < M && e != kInvalid) ? G[pr * gdim + int((e >> 21) & 0x3FFu)] : 0.0;
+            const double t = G[pr * gdim + (e != kInvalid ? int((e >> 21) & 0x3FFu) : 0)];
+            const double v = (r < M && e != kInvalid) ? t : 0.0;
             va[mt][ks] = (e >> 31) ? -v : v;  // sign of the gathered row folded into A
         }
         uint32_t nb = 0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int ro = mt * 16 + 4 * v + g;  // f64 output row
-            if (ro < M) nb |= (ent[ro] >> 31) << v;
+            const uint32_t t = ent[ro < M ? ro : 0];
+            nb |= (ro < M ? t >> 31 : 0u) << v;
         }
         neg[mt] = nb;
     }
@@ -1526,13 +1544,17 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
                     default: throw Error(-1, "clique kernel: no instance for this size");
                 }
             } else if (L % 2 == 0) {
-                const int64_t nblk3 = (sp.nK * nchunk + 3) / 4;
+                static const int cpw = [] {
+                    const char* e = std::getenv("FCI_CQ_CPW");
+                    return e ? std::max(1, std::atoi(e)) : 4;
+                }();
+                const int64_t nblk3p = (sp.nK * ((nchunk + cpw - 1) / cpw) + 3) / 4;
                 const int nmt = (sp.M + 15) / 16;
                 switch (nmt) {
 #define C3(K)                                                                                                 \
     case K:                                                                                                   \
-        k_clique3p_mixed<K><<<xcd_grid(nblk3), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,                 \
-                                                             ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls);     \
+        k_clique3p_mixed<K><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,                \
+                                                              ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
         break;
                     C3(1) C3(2) C3(3) C3(4)
 #undef C3
