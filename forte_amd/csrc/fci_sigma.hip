@@ -1446,10 +1446,12 @@ template <class T> static DBuf<T>& scratch(SigmaWork& w, int which);
 template <> DBuf<double>& scratch<double>(SigmaWork& w, int which) { return which ? w.t64b : w.t64a; }
 template <> DBuf<float>& scratch<float>(SigmaWork& w, int which) { return which ? w.t32b : w.t32a; }
 
-// Bytes of scratch for the two-pass same-spin product (E holds nK*M rows of a column chunk).
+// Bytes of scratch for the two-pass same-spin product (E holds nK*M rows of a column slab). The
+// slab is sized so that E stays in the 256 MB memory-side cache between pass 1 and pass 2: 192 MB
+// measured fastest at CAS(16,16) in both precisions (128-column slabs in FP32, 64 in FP64).
 static size_t clique_scratch_bytes() {
     static size_t v = [] {
-        size_t mb = 4096;
+        size_t mb = 192;
         if (const char* e = std::getenv("FCI_CLIQUE_SCRATCH_MB")) {
             long x = std::atol(e);
             if (x >= 16) mb = size_t(x);
