@@ -277,6 +277,15 @@ __global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ S
     }
 }
 
+// the diagonal preconditioner alone, r /= (lambda - h) with |denom| <= 1e-6 -> 0
+// (davidson_liu_solver.cc:432-448), bit-identical to k_residual's precond branch
+__global__ void k_diag_precond(double* __restrict__ r, const double* __restrict__ h, double lambda, int64_t n) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double den = lambda - h[e];
+    r[e] = (fabs(den) > 1.0e-6) ? r[e] / den : 0.0;
+}
+
 // project_out_roots (davidson_liu_solver.cc:590-603): sequential over bad roots, one block
 __global__ void k_project_out(double* __restrict__ v, const int64_t* __restrict__ off, int nbad,
                               const int64_t* __restrict__ idx, const double* __restrict__ val) {
@@ -711,14 +720,9 @@ bool DavidsonLiu::solve() {
         }
         if (precond_) {
             if (!precond_(r_.p, lambda_.data(), nroot_, iter)) {
-                // the custom preconditioner declined: apply the diagonal one to the residuals
-                for (int k = 0; k < nroot_; ++k) {
-                    std::vector<double> a(bs);
-                    for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];
-                    ak.upload(a.data(), bs);
-                    k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak.p, lambda_[k], hdiag_.p, 1, size_,
-                                              r_.p + size_t(k) * size_, part_.p);
-                }
+                // the custom preconditioner declined: the diagonal one, in place on the residuals
+                for (int k = 0; k < nroot_; ++k)
+                    k_diag_precond<<<nblk(size_, 256), 256>>>(r_.p + size_t(k) * size_, hdiag_.p, lambda_[k], size_);
                 HIPCHK(hipGetLastError());
             }
         }
