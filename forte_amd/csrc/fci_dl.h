@@ -83,7 +83,13 @@ class DavidsonLiu {
     void lincomb(const double* X, int m, const double* coef /*m x k, col-major per output*/, int k,
                  double* out);
     double add_row_and_orthonormalize(double* A, int rowsA, const double* v_src, bool* ok);
-    double add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok);
+    // a0, d0 (optional): scale of vsrc and its overlaps d0_i = a0 vsrc . A_i, when already known
+    double add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok, double a0 = 0.0,
+                         const std::vector<double>* d0 = nullptr);
+    double scale_norm(double* v, double a);  // v <- a v; returns ||v||^2
+    // residual of root k into r (diagonal preconditioner when precond); returns ||r||^2 before
+    // preconditioning; with ov: ov[0] = ||r||^2 after it and ov[1 + i] = r . b_i (i < bs)
+    double residual(int k, bool precond, std::vector<double>* ov);
     double gs_fused(double* v, const double* src, const double* A, int m, const std::vector<double>& d, double a,
                     std::vector<double>& ov);
     int add_rows_and_orthonormalize(double* A, int rowsA, const double* B, int rowsB);

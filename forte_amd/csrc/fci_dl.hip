@@ -277,6 +277,89 @@ __global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ S
     }
 }
 
+// k_residual with the basis loop unrolled to MB (>= m) so that all 2m loads of an element are in
+// flight together; the per-element arithmetic and its order are k_residual's. OV: the partials of
+// the corrected vector's norm and of its overlaps with the basis vectors are accumulated in the
+// same sweep (part stride m + 2: [0] ||r||^2 before preconditioning, [1] after, [2 + i] r . B_i).
+template <int MB, bool OV>
+__global__ __launch_bounds__(DL_BS) void k_residual_t(const double* __restrict__ Sg, const double* __restrict__ B,
+                                                      int m, const double* __restrict__ a, double lambda,
+                                                      const double* __restrict__ h, int precond, int64_t n,
+                                                      double* __restrict__ r, double* __restrict__ part) {
+    constexpr int NA = OV ? MB + 2 : 1;
+    __shared__ double ca[MB];
+    __shared__ double red[NA][DL_BS / 64];
+    for (int i = threadIdx.x; i < m; i += DL_BS) ca[i] = a[i];
+    __syncthreads();
+    double acc[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) acc[i] = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        double sv[MB], bv[MB];
+#pragma unroll
+        for (int j = 0; j < MB; ++j)
+            if (j < m) {
+                sv[j] = Sg[size_t(j) * n + e];
+                bv[j] = B[size_t(j) * n + e];
+            }
+        double rs = 0.0, tb = 0.0;
+#pragma unroll
+        for (int j = 0; j < MB; ++j)
+            if (j < m) {
+                rs += ca[j] * sv[j];
+                tb += ca[j] * bv[j];
+            }
+        double x = rs - lambda * tb;
+        acc[0] += x * x;
+        if (precond) {
+            const double den = lambda - h[e];
+            x = (fabs(den) > 1.0e-6) ? x / den : 0.0;
+        }
+        r[e] = x;
+        if constexpr (OV) {
+            acc[1] += x * x;
+#pragma unroll
+            for (int j = 0; j < MB; ++j)
+                if (j < m) acc[2 + j] += x * bv[j];
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int na = OV ? m + 2 : 1;
+    for (int i = 0; i < NA; ++i) {
+        if (i >= na) break;
+        double t = acc[i];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if (lane == 0) red[i][wave] = t;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < na; i += DL_BS) {
+        double t = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) t += red[i][w];
+        part[size_t(blockIdx.x) * na + i] = t;
+    }
+}
+
+// v <- a v and the partials of ||v||^2 of the result
+__global__ __launch_bounds__(DL_BS) void k_scale_norm(double* __restrict__ v, int64_t n, double a,
+                                                      double* __restrict__ part) {
+    __shared__ double red[DL_BS / 64];
+    double s2 = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        const double x = a * v[e];
+        v[e] = x;
+        s2 += x * x;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) s2 += __shfl_down(s2, off, 64);
+    if (lane == 0) red[wave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < DL_BS / 64; ++w) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
 // the diagonal preconditioner alone, r /= (lambda - h) with |denom| <= 1e-6 -> 0
 // (davidson_liu_solver.cc:432-448), bit-identical to k_residual's precond branch
 __global__ void k_diag_precond(double* __restrict__ r, const double* __restrict__ h, double lambda, int64_t n) {
@@ -441,6 +524,46 @@ double DavidsonLiu::gs_fused(double* v, const double* src, const double* A, int 
     return out[m];
 }
 
+double DavidsonLiu::scale_norm(double* v, double a) {
+    const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
+    k_scale_norm<<<nb, DL_BS>>>(v, size_, a, part_.p);
+    k_reduce_parts<<<1, 256>>>(part_.p, nb, 1, res);
+    HIPCHK(hipGetLastError());
+    double nn = 0.0;
+    HIPCHK(hipMemcpy(&nn, res, sizeof(double), hipMemcpyDeviceToHost));
+    return nn;
+}
+
+double DavidsonLiu::residual(int k, bool precond, std::vector<double>* ov) {
+    const int m = subspace_size_, bs = basis_size_;
+    std::vector<double> a(bs);
+    for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];
+    ak_.upload(a.data(), bs);
+    const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
+    double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
+    double* rk = r_.p + size_t(k) * size_;
+    const int pc = precond ? 1 : 0;
+    if (ov && bs > 16) throw Error(-1, "DavidsonLiu::residual: overlaps need a basis of at most 16 vectors");
+    if (bs <= 8 && ov)
+        k_residual_t<8, true><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    else if (ov)
+        k_residual_t<16, true><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    else if (bs <= 8)
+        k_residual_t<8, false><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    else if (bs <= 16)
+        k_residual_t<16, false><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    else  // larger bases: the loop kernel (same arithmetic, fewer registers)
+        k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    const int na = ov ? bs + 2 : 1;
+    k_reduce_parts<<<na, 256>>>(part_.p, nb, na, res);
+    HIPCHK(hipGetLastError());
+    std::vector<double> out(na);
+    HIPCHK(hipMemcpy(out.data(), res, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (ov) ov->assign(out.begin() + 1, out.end());
+    return out[0];
+}
+
 void DavidsonLiu::lincomb(const double* X, int m, const double* coef, int k, double* out) {
     // coef: k outputs x m inputs (coef[kk*m + j]); split outputs in groups of 8
     int nb = int(std::min<int64_t>(4 * DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
@@ -465,12 +588,18 @@ void DavidsonLiu::rotate(const double* X, double* out, int kout) {
 // subtracts the projections found by the previous pass and, in the same sweep, measures the norm
 // and the overlaps of the result; the normalization of a cycle is folded into the next update
 // (v <- v / |v| - sum_i (ov_i / |v|) A_i). The first pass reads vsrc directly (no copy).
-double DavidsonLiu::add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok) {
+double DavidsonLiu::add_row_fused(double* A, int rowsA, const double* vsrc, bool* ok, double a0,
+                                  const std::vector<double>* d0) {
     double* v = A + size_t(rowsA) * size_;
     std::vector<double> d(std::max(rowsA, 1));
     *ok = false;
-    if (rowsA > 0) dots(vsrc, A, rowsA, d.data());
     double a = 1.0;
+    if (d0) {
+        a = a0;
+        for (int i = 0; i < rowsA; ++i) d[i] = (*d0)[i];
+    } else if (rowsA > 0) {
+        dots(vsrc, A, rowsA, d.data());
+    }
     std::vector<double> ov;
     const double* src = vsrc;
     for (int cycle = 0; cycle < 10; ++cycle) {
@@ -482,10 +611,7 @@ double DavidsonLiu::add_row_fused(double* A, int rowsA, const double* vsrc, bool
         double mo = 0.0;
         for (int i = 0; i < rowsA; ++i) mo = std::max(mo, std::fabs(ov[i] / normval));
         if (mo < orth_) {
-            k_scale<<<nblk(size_, 256), 256>>>(v, size_, 1. / normval);
-            HIPCHK(hipGetLastError());
-            double nn = 0.0;
-            dots(v, v, 1, &nn);
+            const double nn = scale_norm(v, 1. / normval);
             if (std::fabs(nn - 1.0) >= orth_) {
                 a = 1.0;
                 std::fill(d.begin(), d.end(), 0.0);
@@ -705,19 +831,14 @@ bool DavidsonLiu::solve() {
         }
         // form_residual_vectors + compute_residual_norm + form_correction_vectors, :412-455
         if (ak_.n < size_t(m)) ak_.alloc(m);
-        DBuf<double>& ak = ak_;
-        for (int k = 0; k < nroot_; ++k) {
-            std::vector<double> a(bs);
-            for (int j = 0; j < bs; ++j) a[j] = alpha_[size_t(j) * m + k];
-            ak.upload(a.data(), bs);
-            k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak.p, lambda_[k], hdiag_.p, precond_ ? 0 : 1, size_,
-                                      r_.p + size_t(k) * size_, part_.p);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpy(parts.data(), part_.p, nb * sizeof(double), hipMemcpyDeviceToHost));
-            double s2 = 0.0;
-            for (double x : parts) s2 += x;
-            resid_[k] = std::sqrt(s2);
-        }
+        // One root, the diagonal preconditioner, nothing to project out and no collapse ahead:
+        // the residual sweep also measures the correction's norm and its overlaps with the basis,
+        // which replaces the normalization pass and the first overlap pass of Gram-Schmidt.
+        const bool fuse = nroot_ == 1 && !precond_ && poff_.size() <= 1 && basis_size_ + nroot_ <= m &&
+                          basis_size_ <= 16;  // the overlaps stay in registers up to 16 vectors
+        std::vector<double> fov;
+        for (int k = 0; k < nroot_; ++k)
+            resid_[k] = std::sqrt(residual(k, !precond_, fuse ? &fov : nullptr));
         if (precond_) {
             if (!precond_(r_.p, lambda_.data(), nroot_, iter)) {
                 // the custom preconditioner declined: the diagonal one, in place on the residuals
@@ -728,12 +849,14 @@ bool DavidsonLiu::solve() {
         }
         // project_out_roots(r_), normalize_vectors(r_, nroot_), :457-465
         project_out(r_.p, nroot_);
-        for (int k = 0; k < nroot_; ++k) {
-            double nr2 = 0.0;
-            dots(r_.p + size_t(k) * size_, r_.p + size_t(k) * size_, 1, &nr2);
-            k_scale<<<nblk(size_, 256), 256>>>(r_.p + size_t(k) * size_, size_, 1.0 / std::sqrt(nr2));
+        if (!fuse) {
+            for (int k = 0; k < nroot_; ++k) {
+                double nr2 = 0.0;
+                dots(r_.p + size_t(k) * size_, r_.p + size_t(k) * size_, 1, &nr2);
+                k_scale<<<nblk(size_, 256), 256>>>(r_.p + size_t(k) * size_, size_, 1.0 / std::sqrt(nr2));
+            }
+            HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipGetLastError());
         if (print_ >= 2) {
             double avg = 0.0, mx_de = 0.0, mx_r = 0.0;
             for (int k = 0; k < nroot_; ++k) {
@@ -777,7 +900,23 @@ bool DavidsonLiu::solve() {
         }
         // add the correction vectors, :221-251
         int num_to_add = std::min(nroot_, m - basis_size_);
-        int added = add_rows_and_orthonormalize(b_.p, basis_size_, r_.p, num_to_add);
+        int added = 0;
+        if (fuse && basis_size_ + 1 <= GS_MAXM) {
+            // normalized correction r / |r| with overlaps (r . b_i) / |r|
+            const double a0 = 1.0 / std::sqrt(fov[0]);
+            std::vector<double> d0(basis_size_);
+            for (int i = 0; i < basis_size_; ++i) d0[i] = fov[1 + i] * a0;
+            bool ok = false;
+            add_row_fused(b_.p, basis_size_, r_.p, &ok, a0, &d0);
+            added = ok ? 1 : 0;
+        } else {
+            if (fuse) {  // the normalization skipped above
+                double nr2 = fov[0];
+                k_scale<<<nblk(size_, 256), 256>>>(r_.p, size_, 1.0 / std::sqrt(nr2));
+                HIPCHK(hipGetLastError());
+            }
+            added = add_rows_and_orthonormalize(b_.p, basis_size_, r_.p, num_to_add);
+        }
         basis_size_ += added;
         int missing = num_to_add - added;
         if (missing > 0) {

@@ -553,8 +553,12 @@ double Solver::compute_energy() {
     } else {
         dl_->set_preconditioner(nullptr);
     }
-    auto lp_observer = [lpr, learned](int, const std::vector<double>& r, const std::vector<double>&) {
-        if (learned) lpr->observe(r);
+    auto lp_observer = [this, lpr, learned](int, const std::vector<double>& r, const std::vector<double>&) {
+        if (learned) {
+            lpr->observe(r);
+            // from now on the plain diagonal path (its residual sweep is fused with Gram-Schmidt)
+            if (lpr->fell_back) dl_->set_preconditioner(nullptr);
+        }
         return false;
     };
     if (opt_.precision == FCI_PREC_MIXED) {
