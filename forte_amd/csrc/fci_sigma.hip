@@ -1158,9 +1158,12 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab2_mixed(const float* __restr
     for (int64_t i = tid; i < Nb; i += NTH) Srow[i] = float(double(Srow[i]) + srow[i]);
 }
 
-// Variant with the A operand in LDS (built once per block, read per MFMA) so that the registers it
-// frees hold a second set of gathered rows and targets: the loads of the next 32-column tile are
-// in flight while the current one is multiplied and scattered (two buffers used alternately).
+// Variant with the A operand in LDS (built once per block, read per MFMA; ≈37 KB next to the
+// FP64 row) so that the registers it frees hold a second set of gathered rows and targets: the
+// loads of the next 32-column tile are in flight while the current one is multiplied and
+// scattered (two buffers used alternately). With WM = 1 every wave covers all row tiles of its
+// own columns, so each gathered X tile is loaded once per block instead of once per row-tile
+// group. The diagonal rank-1 term reads its row factors from LDS (vdg).
 template <int KS, int TPW, int WM, int WC>
 __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restrict__ C, float* __restrict__ S, int64_t Na,
                                                       int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
@@ -1171,7 +1174,7 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restr
     double* srow = srow_m3;
     bf8* Ash = reinterpret_cast<bf8*>(srow + ((Nb + 1) & ~int64_t(1)));  // [nmt][KS][2][64]
     __shared__ uint32_t ent[KS * 32];
-    __shared__ float vdg[12 * 16];
+    __shared__ __attribute__((aligned(16))) float vdg[12 * 16];
     constexpr int NTH = 64 * WM * WC, NW = WM * WC;
     const int64_t Ja = xcd_remap(blockIdx.x, Na);
     if (Ja >= Na) return;
@@ -1212,13 +1215,6 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restr
             Ash[((slot * KS + ks) * 2 + 0) * 64 + lane] = h8;
             Ash[((slot * KS + ks) * 2 + 1) * 64 + lane] = l8;
         }
-    }
-    float vd[TPW][4];
-#pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-        const int mtile = wm + WM * j;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) vd[j][v] = mtile < nmt ? vdg[mtile * 16 + 4 * g + v] : 0.f;
     }
     __syncthreads();
     const float* Crow = C + Ja * Nb;
@@ -1272,6 +1268,8 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restr
                 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh0[ks], a0, 0, 0, 0);
                 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh1[ks], a1, 0, 0, 0);
             }
+            const float4 vq = *reinterpret_cast<const float4*>(vdg + slot * 16 + 4 * g);  // diagonal term rows
+            const float vd4[4] = {vq.x, vq.y, vq.z, vq.w};
             const uint32_t tw[4] = {t[j].x, t[j].y, t[j].z, t[j].w};
 #pragma unroll
             for (int p = 0; p < 2; ++p)
@@ -1279,7 +1277,7 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restr
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
                     if (!ok || (tt & 0x7FFFu) == kInv16) continue;
-                    const float a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
+                    const float a = (p ? a1[v] : a0[v]) + vd4[v] * (p ? cd.y : cd.x);
                     atomicAdd(&srow[tt & 0x7FFFu], double((tt >> 15) ? -a : a));
                 }
         }
@@ -1650,24 +1648,22 @@ static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, cons
     HIPCHK(hipGetLastError());
 }
 template <int KS, int TPW, int WM, int WC>
+static void ab3_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
+                             const DevInts& ints, hipStream_t st) {
+    const int nmt = (b.np + 15) / 16;
+    const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * sizeof(double) + size_t(nmt) * KS * 2 * 64 * 16;
+    if (nmt > WM * TPW || sh3 + KS * 32 * 4 + 12 * 16 * 4 > 160 * 1024)
+        throw Error(-1, "alpha-beta kernel: LDS A-operand layout does not fit this shape");
+    auto kern3 = k_ab3_mixed<KS, TPW, WM, WC>;
+    HIPCHK(hipFuncSetAttribute((const void*)kern3, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh3)));
+    kern3<<<xcd_grid(a.nstr), 64 * WM * WC, sh3, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p,
+                                                       b.npp, b.np, ints.vpk32.p, nmt);
+    HIPCHK(hipGetLastError());
+}
+
+template <int KS, int TPW, int WM, int WC>
 static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
                              const DevInts& ints, hipStream_t st) {
-    static const bool v3 = [] {
-        const char* e = std::getenv("FCI_AB_KERNEL");
-        return e && std::atoi(e) == 3;
-    }();
-    if (v3) {
-        const int nmt = (b.np + 15) / 16;
-        const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * sizeof(double) + size_t(nmt) * KS * 2 * 64 * 16;
-        if (nmt <= WM * TPW && sh3 + KS * 32 * 4 <= 160 * 1024) {
-            auto kern3 = k_ab3_mixed<KS, TPW, WM, WC>;
-            HIPCHK(hipFuncSetAttribute((const void*)kern3, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh3)));
-            kern3<<<xcd_grid(a.nstr), 64 * WM * WC, sh3, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p,
-                                                               b.outT.p, b.npp, b.np, ints.vpk32.p, nmt);
-            HIPCHK(hipGetLastError());
-            return;
-        }
-    }
     const size_t sh = size_t(b.nstr) * sizeof(double);
     auto kern = k_ab2_mixed<KS, TPW, WM, WC>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
@@ -1713,11 +1709,14 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 31) / 32);
         const int nmt = (b.np + 15) / 16;
-        // 8 waves (2 row-tile groups x 4 column groups); FCI_AB_LAYOUT=12 selects 4 x 3 (more
-        // waves, fewer registers, but each X tile is loaded by four waves: slower at CAS(16,16))
+        // Default for 7..10 row tiles: 8 waves each on its own 32 columns and all row tiles, the A
+        // operand in LDS, the next tile's loads in flight (k_ab3_mixed<K, T, 1, 8>: 16.5 ms at
+        // CAS(16,16)); otherwise 8 waves as 2 row-tile groups x 4 column groups with A in
+        // registers (k_ab2_mixed<K, T, 2, 4>: 19.7 ms there, each X tile loaded by two waves).
+        // FCI_AB_LAYOUT=8 forces the latter, 12 and 3 select 4 x 3 and 3 x 4 groups.
         static const int layout = [] {
             const char* e = std::getenv("FCI_AB_LAYOUT");
-            return e ? std::atoi(e) : 8;
+            return e ? std::atoi(e) : 1;
         }();
         if (layout == 3 && (nmt + 2) / 3 * ks <= 6) {
             const int tpw = (nmt + 2) / 3;
@@ -1733,6 +1732,13 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
     if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 4, 3>(C, S, a, b, ints, st);
             MXCASE(1, 1) MXCASE(1, 2) MXCASE(1, 3) MXCASE(2, 1) MXCASE(2, 2) MXCASE(2, 3)
             MXCASE(3, 1) MXCASE(3, 2)
+#undef MXCASE
+        }
+        const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * 8 + size_t(nmt) * ks * 2 * 64 * 16 + 1024 + 768;
+        if (layout == 1 && sh3 <= 160 * 1024) {
+#define MXCASE(K, T) \
+    if (ks == K && nmt == T) return ab3_mixed_launch<K, T, 1, 8>(C, S, a, b, ints, st);
+            MXCASE(1, 7) MXCASE(1, 8) MXCASE(1, 9) MXCASE(1, 10) MXCASE(2, 7) MXCASE(2, 8) MXCASE(2, 9) MXCASE(2, 10)
 #undef MXCASE
         }
         const int tpw = (nmt + 1) / 2;
