@@ -8,7 +8,7 @@ seeded integrals, seed 3) with the mixed-precision kernels (FP32 C and sigma, BF
 contractions accumulated in FP32). `value` is the sigma throughput in GB/s, defined as
 9 * N_det * 4 bytes / t_sigma (C read by the two same-spin and the alpha-beta passes, sigma
 written and re-read), inputs resident in HBM. The line also carries:
-  roofline      MFMA roofline of the dominant kernel (k_ab2_mixed, the alpha-beta contraction),
+  roofline      MFMA roofline of the dominant kernel (k_ab3_mixed, the alpha-beta contraction),
                 timed with HIP events on the stream it runs on; algorithmic flops per launch =
                 3 * 2 * N_det * P^2 with P = n(n+1)/2 packed pairs (the packed-pair GEMM
                 formulation the benchmark fixes; BF16x3 = 3 BF16 products), against the
@@ -37,7 +37,7 @@ BF16_PEAK_TFLOPS = 2500.0
 
 def traffic_from_profiles(n):
     """HBM bytes per launch of the alpha-beta kernel from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", f"k_ab2_mixed_cas{n}_{n}_pmc.json")
+    path = os.path.join(ROOT, "profiles", f"k_ab3_mixed_cas{n}_{n}_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -217,7 +217,7 @@ def main():
         "data": "synthetic (seeded integrals, seed 3; random normalized C)",
         "config": {"workload": f"FCI sigma CAS({n},{n}) Ms=0 C1", "ndet": nd, "n_orbitals": n,
                    "n_alpha": na, "n_beta": nb, "precision": "mixed"},
-        "roofline": {"bound": "mfma", "kernel": "k_ab2_mixed (alpha-beta BF16x3 contraction)",
+        "roofline": {"bound": "mfma", "kernel": "k_ab3_mixed (alpha-beta BF16x3 contraction)",
                      "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(n),
                      "kernel_ms": round(t3[2], 3),
