@@ -772,16 +772,14 @@ __global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, 
     const T* Eb = CHUNKED ? E + (cl / 64) * nrowE * 64 + (cl % 64) : E + cl;
     const int64_t rs = CHUNKED ? 64 : Ls;
     T acc = T(0);
-    int j = 0;
-    for (; j + 4 <= nown; j += 4) {  // four independent loads in flight
-        const T a0 = Eb[int64_t(o[j]) * rs], a1 = Eb[int64_t(o[j + 1]) * rs];
-        const T a2 = Eb[int64_t(o[j + 2]) * rs], a3 = Eb[int64_t(o[j + 3]) * rs];
-        acc += a0;
-        acc += a1;
-        acc += a2;
-        acc += a3;
+    constexpr int GU = 32;  // up to 32 independent loads in flight; the sum runs in j order
+    for (int j = 0; j < nown; j += GU) {
+        T a[GU];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) a[u] = j + u < nown ? Eb[int64_t(o[j + u]) * rs] : T(0);
+#pragma unroll
+        for (int u = 0; u < GU; ++u) acc += a[u];  // + 0 for absent terms is exact
     }
-    for (; j < nown; ++j) acc += Eb[int64_t(o[j]) * rs];
     if (gdiag) acc += T(gdiag[I]) * X[I * L + cbeg + cl];
     T* y = Y + I * L + cbeg + cl;
     if constexpr (ACC)
