@@ -59,9 +59,9 @@ def test_hdiag_bit_exact(oracle, n, na, nb):
 SIGMA_CASES = [(2, 1, 1), (3, 2, 1), (4, 2, 2), (4, 3, 2), (5, 1, 1), (6, 3, 3), (6, 2, 4), (6, 5, 1),
                (4, 4, 0), (4, 0, 4), (6, 6, 6), (7, 1, 6), (8, 4, 4), (8, 3, 5), (9, 2, 2), (10, 5, 5),
                (10, 2, 8), (12, 6, 6), (12, 4, 7),
-               # 14 and 16 orbitals: 7 and 9 row tiles of pairs, the alpha-beta layout of CAS(14,14) and
-               # CAS(16,16) (few electrons keep the oracle fast)
-               (14, 3, 3), (16, 2, 2), (16, 3, 3)]
+               # 14 to 18 orbitals: 7, 9, 10 and 11 row tiles of pairs, the alpha-beta layouts of CAS(14,14)
+               # to CAS(18,18) (few electrons keep the oracle fast)
+               (14, 3, 3), (16, 2, 2), (16, 3, 3), (17, 2, 2), (18, 3, 3)]
 
 
 @pytest.mark.parametrize("n,na,nb", SIGMA_CASES)
@@ -77,7 +77,7 @@ def test_sigma_fp64(oracle, n, na, nb):
 
 
 @pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 4, 4), (10, 5, 5), (12, 6, 6), (9, 3, 5), (14, 3, 3), (16, 2, 2),
-                                     (16, 3, 3)])
+                                     (16, 3, 3), (17, 2, 2), (18, 3, 3)])
 def test_sigma_mixed(oracle, n, na, nb):
     import forte_amd
     say("sigma mixed", n, na, nb)
