@@ -541,12 +541,18 @@ __global__ __launch_bounds__(256) void k_clique3_mixed(const float* __restrict__
 // Paired-column variant for even row lengths: each lane loads two adjacent columns with one 8-byte
 // load, which feeds two MFMA column tiles (columns c0 + 2j and c0 + 2j + 1, j = lane % 16), and
 // stores both results with one 8-byte store: half the memory instructions of k_clique3_mixed.
-template <int NMT>
+// MS > 0: the clique size M = MS is known at compile time. Member m sits in k slot 8g + x of
+// k-step ks with m = 32 ks + 4 x + g, so the padding m >= M fills whole x rows of the last k-step,
+// whose gathers (and their member-table loads) are then left out.
+template <int NMT, int MS>
 __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict__ X, float* __restrict__ E, int64_t L,
-                                                        const uint32_t* __restrict__ clq, int M, int64_t nK,
+                                                        const uint32_t* __restrict__ clq, int Mrt, int64_t nK,
                                                         const float* __restrict__ G, int gdim, int nchunk,
                                                         int64_t cbeg, int64_t Ls, int cpw) {
     constexpr int NKS = (NMT + 1) / 2;
+    const int M = MS > 0 ? MS : Mrt;
+    // k slots (ks, x) with no member for any g (only when M is static)
+    auto absent = [](int ks, int x) { return MS > 0 && ks * 32 + 4 * x >= MS; };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int ngrp = (nchunk + cpw - 1) / cpw;  // work item: one K and up to cpw consecutive chunks
@@ -563,7 +569,11 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
     for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            const int m = ks * 32 + 8 * g + x;
+            const int m = ks * 32 + 4 * x + g;
+            if (absent(ks, x)) {
+                mb[ks][x] = kInvalid;
+                continue;
+            }
             const uint32_t t = ent[m < M ? m : 0];  // unconditional loads (M >= 1): all in flight at once
             mb[ks][x] = m < M ? t : kInvalid;
         }
@@ -603,7 +613,8 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
         for (int ks = 0; ks < NKS; ++ks)
 #pragma unroll
             for (int x = 0; x < 8; ++x)
-                xx[ks][x] = *reinterpret_cast<const float2*>(X + gather_row(mb[ks][x]) * L + cc);
+                xx[ks][x] = absent(ks, x) ? make_float2(0.f, 0.f)
+                                          : *reinterpret_cast<const float2*>(X + gather_row(mb[ks][x]) * L + cc);
     };
     auto colof = [&](int h) { return cbeg + int64_t(ch0 + (h >> 1)) * CQ3_CW + (h & 1) * 32 + 2 * cl; };
     auto compute = [&](const float2 (&x)[NKS][8], int h) {
@@ -1551,8 +1562,21 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
                 switch (nmt) {
 #define C3(K)                                                                                                 \
     case K:                                                                                                   \
-        k_clique3p_mixed<K><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,                \
-                                                              ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
+        if (sp.M == 28 && K == 2)                                                                             \
+            k_clique3p_mixed<K, 28><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,         \
+                                                                      ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
+        else if (sp.M == 36 && K == 3)                                                                        \
+            k_clique3p_mixed<K, 36><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,         \
+                                                                      ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
+        else if (sp.M == 45 && K == 3)                                                                        \
+            k_clique3p_mixed<K, 45><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,         \
+                                                                      ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
+        else if (sp.M == 55 && K == 4)                                                                        \
+            k_clique3p_mixed<K, 55><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,         \
+                                                                      ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
+        else                                                                                                  \
+            k_clique3p_mixed<K, 0><<<xcd_grid(nblk3p), 256, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK,          \
+                                                                     ints.g2f_off[spin].p, gd, nchunk, cbeg, Ls, cpw); \
         break;
                     C3(1) C3(2) C3(3) C3(4)
 #undef C3

@@ -77,7 +77,10 @@ def test_sigma_fp64(oracle, n, na, nb):
 
 
 @pytest.mark.parametrize("n,na,nb", [(6, 3, 3), (8, 4, 4), (10, 5, 5), (12, 6, 6), (9, 3, 5), (14, 3, 3), (16, 2, 2),
-                                     (16, 3, 3), (17, 2, 2), (18, 3, 3)])
+                                     (16, 3, 3), (17, 2, 2), (18, 3, 3),
+                                     # same-spin cliques of 36, 45 and 55 members (the alpha side,
+                                     # even row length): the static-size pass-1 kernels
+                                     (9, 2, 2), (10, 2, 3), (12, 4, 5), (12, 3, 2)])
 def test_sigma_mixed(oracle, n, na, nb):
     import forte_amd
     say("sigma mixed", n, na, nb)
