@@ -26,7 +26,14 @@ class DavidsonLiu {
     void add_h_diag_device(const double* d_h, int64_t n);
     void add_guesses(const std::vector<SparseVec>& g) { guesses_ = g; }
     void add_project_out_vectors(const std::vector<SparseVec>& v);
-    void add_sigma_builder(SigmaFn f) { sigma_ = std::move(f); }
+    void add_sigma_builder(SigmaFn f) {
+        sigma_ = std::move(f);
+        s32_ = false;
+    }
+    // sigma rows kept in FP32 (a reduced-precision phase): dots, residuals and rotations read
+    // them and accumulate in FP64
+    using Sigma32Fn = std::function<void(const double* b, float* sigma)>;
+    void add_sigma_builder32(Sigma32Fn f);
     void set_preconditioner(PrecondFn f) { precond_ = std::move(f); }
     // called after the residual norms of an iteration are known; returning true ends the solve
     // as converged (used to end a reduced-precision phase once it stalls)
@@ -60,6 +67,9 @@ class DavidsonLiu {
     double e_conv_ = 1.0e-12, r_conv_ = 1.0e-6;
     double discard_ = 1.0e-7, orth_ = 1.0e-12;
     SigmaFn sigma_;
+    Sigma32Fn sigma32_;
+    bool s32_ = false;
+    DBuf<float> sigma32_v_;
     PrecondFn precond_;
     IterFn observe_;
     bool have_hdiag_ = false;
@@ -80,6 +90,9 @@ class DavidsonLiu {
 
     // device building blocks
     void dots(const double* x, const double* Y, int m, double* out);  // out[i] = x . Y_i
+    template <class TX, class TY> void dots_t(const TX* x, const TY* Y, int m, double* out);
+    template <class T> void lincomb_t(const T* X, int m, const double* coef, int k, T* out);
+    template <class T> void rotate_t(const T* X, T* out, int kout);
     void lincomb(const double* X, int m, const double* coef /*m x k, col-major per output*/, int k,
                  double* out);
     double add_row_and_orthonormalize(double* A, int rowsA, const double* v_src, bool* ok);

@@ -3,6 +3,8 @@
 // operation on vectors of the full space runs in FP64 on the GPU.
 #include "fci_dl.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -65,17 +67,18 @@ constexpr int DL_MAXM = 64;
 constexpr int DL_NBLK = 1024;  // blocks of the reduction kernels
 
 // out_part[blk][i] = sum over this block's elements of x[e] * Y_i[e]  (i < m)
-__global__ __launch_bounds__(DL_BS) void k_dots(const double* __restrict__ x, const double* __restrict__ Y,
+template <class TX, class TY>  // FP64 or FP32 vectors (FP32 sigma rows of the mixed phase), FP64 sums
+__global__ __launch_bounds__(DL_BS) void k_dots(const TX* __restrict__ x, const TY* __restrict__ Y,
                                                 int m, int64_t n, double* __restrict__ part) {
     __shared__ double red[DL_MAXM][DL_BS / 64];
     double acc[DL_MAXM];
 #pragma unroll
     for (int i = 0; i < DL_MAXM; ++i) acc[i] = 0.0;
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
-        double xv = x[e];
+        double xv = double(x[e]);
 #pragma unroll
         for (int i = 0; i < DL_MAXM; ++i)
-            if (i < m) acc[i] += xv * Y[size_t(i) * n + e];
+            if (i < m) acc[i] += xv * double(Y[size_t(i) * n + e]);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -96,8 +99,14 @@ __global__ __launch_bounds__(DL_BS) void k_dots(const double* __restrict__ x, co
 // part[blk][i] for the vectors i of chunk blockIdx.y (8 per chunk), double2 loads (x and every
 // Y_i 16-byte aligned: n even)
 constexpr int DOT_MC = 8;
-__global__ __launch_bounds__(DL_BS) void k_dots8(const double* __restrict__ x, const double* __restrict__ Y, int m,
+template <class T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+template <class TX, class TY>
+__global__ __launch_bounds__(DL_BS) void k_dots8(const TX* __restrict__ x, const TY* __restrict__ Y, int m,
                                                  int64_t n, double* __restrict__ part) {
+    using X2 = typename Vec2<TX>::type;
+    using Y2 = typename Vec2<TY>::type;
     __shared__ double red[DOT_MC][DL_BS / 64];
     const int i0 = blockIdx.y * DOT_MC;
     const int mc = min(DOT_MC, m - i0);
@@ -105,14 +114,14 @@ __global__ __launch_bounds__(DL_BS) void k_dots8(const double* __restrict__ x, c
 #pragma unroll
     for (int i = 0; i < DOT_MC; ++i) acc[i] = 0.0;
     const int64_t n2 = n / 2;
-    const double2* x2 = reinterpret_cast<const double2*>(x);
+    const X2* x2 = reinterpret_cast<const X2*>(x);
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n2; e += int64_t(gridDim.x) * DL_BS) {
-        const double2 xv = x2[e];
+        const X2 xv = x2[e];
 #pragma unroll
         for (int i = 0; i < DOT_MC; ++i)
             if (i < mc) {
-                const double2 y = reinterpret_cast<const double2*>(Y + size_t(i0 + i) * n)[e];
-                acc[i] += xv.x * y.x + xv.y * y.y;
+                const Y2 y = reinterpret_cast<const Y2*>(Y + size_t(i0 + i) * n)[e];
+                acc[i] += double(xv.x) * double(y.x) + double(xv.y) * double(y.y);
             }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -192,22 +201,23 @@ __global__ void k_reduce_parts(const double* __restrict__ part, int nparts, int 
 }
 
 // out_k[e] = sum_j coef[k*m + j] X_j[e]   (k < kout <= 8)
-__global__ __launch_bounds__(DL_BS) void k_lincomb(const double* __restrict__ X, int m, const double* __restrict__ coef,
-                                                   int kout, int64_t n, double* __restrict__ out) {
+template <class T>  // FP64 basis / sigma rows, or FP32 sigma rows (sums in FP64, rounded once)
+__global__ __launch_bounds__(DL_BS) void k_lincomb(const T* __restrict__ X, int m, const double* __restrict__ coef,
+                                                   int kout, int64_t n, T* __restrict__ out) {
     __shared__ double cf[DL_MAXM * 8];
     for (int i = threadIdx.x; i < m * kout; i += DL_BS) cf[i] = coef[i];
     __syncthreads();
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int j = 0; j < m; ++j) {
-            double xv = X[size_t(j) * n + e];
+            double xv = double(X[size_t(j) * n + e]);
 #pragma unroll
             for (int k = 0; k < 8; ++k)
                 if (k < kout) acc[k] += cf[k * m + j] * xv;
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (k < kout) out[size_t(k) * n + e] = acc[k];
+            if (k < kout) out[size_t(k) * n + e] = T(acc[k]);
     }
 }
 
@@ -243,7 +253,8 @@ __global__ void k_scale(double* __restrict__ v, int64_t n, double a) {
 
 // residual r = sum_j a_j sigma_j - lambda sum_j a_j b_j; part = ||r||^2 partial; then the
 // diagonal preconditioner r /= (lambda - h) with |denom| <= 1e-6 -> 0 (davidson_liu_solver.cc:412-448)
-__global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ Sg, const double* __restrict__ B, int m,
+template <class TS>
+__global__ __launch_bounds__(DL_BS) void k_residual(const TS* __restrict__ Sg, const double* __restrict__ B, int m,
                                                     const double* __restrict__ a, double lambda,
                                                     const double* __restrict__ h, int precond, int64_t n,
                                                     double* __restrict__ r, double* __restrict__ part) {
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ S
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
         double rs = 0.0, tb = 0.0;
         for (int j = 0; j < m; ++j) {
-            rs += ca[j] * Sg[size_t(j) * n + e];
+            rs += ca[j] * double(Sg[size_t(j) * n + e]);
             tb += ca[j] * B[size_t(j) * n + e];
         }
         double x = rs - lambda * tb;
@@ -281,8 +292,8 @@ __global__ __launch_bounds__(DL_BS) void k_residual(const double* __restrict__ S
 // flight together; the per-element arithmetic and its order are k_residual's. OV: the partials of
 // the corrected vector's norm and of its overlaps with the basis vectors are accumulated in the
 // same sweep (part stride m + 2: [0] ||r||^2 before preconditioning, [1] after, [2 + i] r . B_i).
-template <int MB, bool OV>
-__global__ __launch_bounds__(DL_BS) void k_residual_t(const double* __restrict__ Sg, const double* __restrict__ B,
+template <int MB, bool OV, class TS>
+__global__ __launch_bounds__(DL_BS) void k_residual_t(const TS* __restrict__ Sg, const double* __restrict__ B,
                                                       int m, const double* __restrict__ a, double lambda,
                                                       const double* __restrict__ h, int precond, int64_t n,
                                                       double* __restrict__ r, double* __restrict__ part) {
@@ -299,7 +310,7 @@ __global__ __launch_bounds__(DL_BS) void k_residual_t(const double* __restrict__
 #pragma unroll
         for (int j = 0; j < MB; ++j)
             if (j < m) {
-                sv[j] = Sg[size_t(j) * n + e];
+                sv[j] = double(Sg[size_t(j) * n + e]);
                 bv[j] = B[size_t(j) * n + e];
             }
         double rs = 0.0, tb = 0.0;
@@ -451,6 +462,16 @@ void DavidsonLiu::add_h_diag_device(const double* d, int64_t n) {
     have_hdiag_ = true;
 }
 
+void DavidsonLiu::add_sigma_builder32(Sigma32Fn f) {
+    sigma32_ = std::move(f);
+    s32_ = true;
+    const size_t m = size_t(subspace_size_);
+    if (sigma32_v_.n < m * size_t(size_)) {
+        sigma32_v_.alloc(m * size_);
+        HIPCHK(hipMemset(sigma32_v_.p, 0, m * size_ * sizeof(float)));
+    }
+}
+
 void DavidsonLiu::add_project_out_vectors(const std::vector<SparseVec>& v) {
     poff_dev_.release();  // re-uploaded on the next projection
     poff_.assign(1, 0);
@@ -484,25 +505,27 @@ void DavidsonLiu::eigenvector_host(int k, double* out) const {
     HIPCHK(hipMemcpy(out, b_.p + size_t(k) * size_, size_ * sizeof(double), hipMemcpyDeviceToHost));
 }
 
-void DavidsonLiu::dots(const double* x, const double* Y, int m, double* out) {
+template <class TX, class TY>
+void DavidsonLiu::dots_t(const TX* x, const TY* Y, int m, double* out) {
     if (m <= 0) return;
     double* part = part_.p;
     double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
     int nb;
-    // double2 loads need every vector 16-byte aligned: even lengths only
-    if (m <= DL_MAXM && size_ % 2 == 0 && reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
-        reinterpret_cast<uintptr_t>(Y) % 16 == 0) {
+    // two-element loads need every vector aligned to two elements: even lengths only
+    if (m <= DL_MAXM && size_ % 2 == 0 && reinterpret_cast<uintptr_t>(x) % (2 * sizeof(TX)) == 0 &&
+        reinterpret_cast<uintptr_t>(Y) % (2 * sizeof(TY)) == 0) {
         nb = int(std::min<int64_t>(DL_NBLK, (size_ / 2 + DL_BS - 1) / DL_BS));
         nb = std::max(nb, 1);
-        k_dots8<<<dim3(nb, (m + DOT_MC - 1) / DOT_MC), DL_BS>>>(x, Y, m, size_, part);
+        k_dots8<TX, TY><<<dim3(nb, (m + DOT_MC - 1) / DOT_MC), DL_BS>>>(x, Y, m, size_, part);
     } else {
         nb = int(std::min<int64_t>(DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
-        k_dots<<<nb, DL_BS>>>(x, Y, m, size_, part);
+        k_dots<TX, TY><<<nb, DL_BS>>>(x, Y, m, size_, part);
     }
     k_reduce_parts<<<m, 256>>>(part, nb, m, res);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(out, res, m * sizeof(double), hipMemcpyDeviceToHost));
 }
+void DavidsonLiu::dots(const double* x, const double* Y, int m, double* out) { dots_t(x, Y, m, out); }
 
 // v <- a v - sum_i d_i A_i; returns ||v||^2 and ov[i] = v . A_i of the updated v
 double DavidsonLiu::gs_fused(double* v, const double* src, const double* A, int m, const std::vector<double>& d,
@@ -545,16 +568,23 @@ double DavidsonLiu::residual(int k, bool precond, std::vector<double>* ov) {
     double* rk = r_.p + size_t(k) * size_;
     const int pc = precond ? 1 : 0;
     if (ov && bs > 16) throw Error(-1, "DavidsonLiu::residual: overlaps need a basis of at most 16 vectors");
-    if (bs <= 8 && ov)
-        k_residual_t<8, true><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
-    else if (ov)
-        k_residual_t<16, true><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
-    else if (bs <= 8)
-        k_residual_t<8, false><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
-    else if (bs <= 16)
-        k_residual_t<16, false><<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
-    else  // larger bases: the loop kernel (same arithmetic, fewer registers)
-        k_residual<<<nb, DL_BS>>>(sigma_v_.p, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    auto launch = [&](auto* Sg) {
+        using TS = std::remove_const_t<std::remove_pointer_t<decltype(Sg)>>;
+        if (bs <= 8 && ov)
+            k_residual_t<8, true, TS><<<nb, DL_BS>>>(Sg, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+        else if (ov)
+            k_residual_t<16, true, TS><<<nb, DL_BS>>>(Sg, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+        else if (bs <= 8)
+            k_residual_t<8, false, TS><<<nb, DL_BS>>>(Sg, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+        else if (bs <= 16)
+            k_residual_t<16, false, TS><<<nb, DL_BS>>>(Sg, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+        else  // larger bases: the loop kernel (same arithmetic, fewer registers)
+            k_residual<TS><<<nb, DL_BS>>>(Sg, b_.p, bs, ak_.p, lambda_[k], hdiag_.p, pc, size_, rk, part_.p);
+    };
+    if (s32_)
+        launch(static_cast<const float*>(sigma32_v_.p));
+    else
+        launch(static_cast<const double*>(sigma_v_.p));
     const int na = ov ? bs + 2 : 1;
     k_reduce_parts<<<na, 256>>>(part_.p, nb, na, res);
     HIPCHK(hipGetLastError());
@@ -564,25 +594,31 @@ double DavidsonLiu::residual(int k, bool precond, std::vector<double>* ov) {
     return out[0];
 }
 
-void DavidsonLiu::lincomb(const double* X, int m, const double* coef, int k, double* out) {
+template <class T>
+void DavidsonLiu::lincomb_t(const T* X, int m, const double* coef, int k, T* out) {
     // coef: k outputs x m inputs (coef[kk*m + j]); split outputs in groups of 8
     int nb = int(std::min<int64_t>(4 * DL_NBLK, (size_ + DL_BS - 1) / DL_BS));
     for (int k0 = 0; k0 < k; k0 += 8) {
         int kk = std::min(8, k - k0);
         small_.upload(coef + size_t(k0) * m, size_t(kk) * m);  // synchronous copy
-        k_lincomb<<<nb, DL_BS>>>(X, m, small_.p, kk, size_, out + size_t(k0) * size_);
+        k_lincomb<T><<<nb, DL_BS>>>(X, m, small_.p, kk, size_, out + size_t(k0) * size_);
         HIPCHK(hipGetLastError());
     }
 }
+void DavidsonLiu::lincomb(const double* X, int m, const double* coef, int k, double* out) {
+    lincomb_t(X, m, coef, k, out);
+}
 
 // out_k = sum_j alpha[j][k] X_j for k < kout over the current basis (temp_->gemm(true,false,...))
-void DavidsonLiu::rotate(const double* X, double* out, int kout) {
+template <class T>
+void DavidsonLiu::rotate_t(const T* X, T* out, int kout) {
     int m = subspace_size_;
     std::vector<double> coef(size_t(kout) * basis_size_);
     for (int k = 0; k < kout; ++k)
         for (int j = 0; j < basis_size_; ++j) coef[size_t(k) * basis_size_ + j] = alpha_[size_t(j) * m + k];
-    lincomb(X, basis_size_, coef.data(), kout, out);
+    lincomb_t(X, basis_size_, coef.data(), kout, out);
 }
+void DavidsonLiu::rotate(const double* X, double* out, int kout) { rotate_t(X, out, kout); }
 
 // Gram-Schmidt of vsrc against the rows of A into row rowsA, in fused passes: each cycle
 // subtracts the projections found by the previous pass and, in the same sweep, measures the norm
@@ -714,12 +750,18 @@ void DavidsonLiu::form_G() {
     std::vector<double> tmp(m);
     int gv = std::min(gvalid_, basis_size_);
     for (int j = gv; j < basis_size_; ++j) {
-        dots(sigma_v_.p + size_t(j) * size_, b_.p, basis_size_, tmp.data());
+        if (s32_)
+            dots_t(sigma32_v_.p + size_t(j) * size_, b_.p, basis_size_, tmp.data());
+        else
+            dots(sigma_v_.p + size_t(j) * size_, b_.p, basis_size_, tmp.data());
         for (int i = 0; i < basis_size_; ++i) G_[size_t(i) * m + j] = tmp[i];
     }
     for (int i = gv; i < basis_size_; ++i) {
         if (gv == 0) break;
-        dots(b_.p + size_t(i) * size_, sigma_v_.p, gv, tmp.data());
+        if (s32_)
+            dots_t(b_.p + size_t(i) * size_, static_cast<const float*>(sigma32_v_.p), gv, tmp.data());
+        else
+            dots(b_.p + size_t(i) * size_, sigma_v_.p, gv, tmp.data());
         for (int j = 0; j < gv; ++j) G_[size_t(i) * m + j] = tmp[j];
     }
     gvalid_ = basis_size_;
@@ -800,7 +842,7 @@ bool DavidsonLiu::solve() {
         throw Error(-1, "DavidsonLiuSolver: number of guess vectors must be between 0 and the subspace size");
     }
     // preiteration_sanity_checks, :332-345
-    if (!sigma_) throw Error(-1, "DavidsonLiuSolver: sigma builder has not been set");
+    if (s32_ ? !sigma32_ : !sigma_) throw Error(-1, "DavidsonLiuSolver: sigma builder has not been set");
     if (!have_hdiag_) throw Error(-1, "DavidsonLiuSolver: h_diag has not been set");
 
     std::fill(lambda_old_.begin(), lambda_old_.end(), 0.0);
@@ -812,7 +854,10 @@ bool DavidsonLiu::solve() {
         check_orthonormality();
         // compute_sigma, :374-383
         for (int j = sigma_size_; j < basis_size_; ++j) {
-            sigma_(b_.p + size_t(j) * size_, sigma_v_.p + size_t(j) * size_);
+            if (s32_)
+                sigma32_(b_.p + size_t(j) * size_, sigma32_v_.p + size_t(j) * size_);
+            else
+                sigma_(b_.p + size_t(j) * size_, sigma_v_.p + size_t(j) * size_);
             ++nsigma_;
         }
         sigma_size_ = basis_size_;
@@ -889,8 +934,14 @@ bool DavidsonLiu::solve() {
             basis_size_ = 0;
             int added = add_rows_and_orthonormalize(b_.p, 0, temp_.p, cs);
             basis_size_ = keep_bs;
-            rotate(sigma_v_.p, temp_.p, cs);
-            HIPCHK(hipMemcpy(sigma_v_.p, temp_.p, size_t(cs) * size_ * sizeof(double), hipMemcpyDeviceToDevice));
+            if (s32_) {  // temp_ holds cs FP32 rows in its first half
+                float* t32 = reinterpret_cast<float*>(temp_.p);
+                rotate_t(static_cast<const float*>(sigma32_v_.p), t32, cs);
+                HIPCHK(hipMemcpy(sigma32_v_.p, t32, size_t(cs) * size_ * sizeof(float), hipMemcpyDeviceToDevice));
+            } else {
+                rotate(sigma_v_.p, temp_.p, cs);
+                HIPCHK(hipMemcpy(sigma_v_.p, temp_.p, size_t(cs) * size_ * sizeof(double), hipMemcpyDeviceToDevice));
+            }
             if (added != cs)
                 throw Error(-5, "DavidsonLiuSolver: collapse_vectors generated less vectors (" + std::to_string(added) +
                                     ") than expected (" + std::to_string(cs) + ")");

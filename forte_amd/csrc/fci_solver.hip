@@ -384,6 +384,7 @@ class Solver {
 
     void sigma64(const double* C, double* S) { eng_->apply(C, S, 0); }
     void sigma_mixed(const double* C, double* S);
+    void sigma_mixed32(const double* C, float* S);
     void compute_hdiag() {
         if (!hdiag_.p) hdiag_.alloc(ndet_);
         hdiag_device(sa_, sb_, di_.oei_a.p, di_.oei_b.p, di_.tei_aa.p, di_.tei_ab.p, di_.tei_bb.p, di_.e_core,
@@ -411,6 +412,15 @@ void Solver::sigma_mixed(const double* C, double* S) {
     k_f64_to_f32<<<g, 256>>>(C, c32_.p, ndet_);
     eng_->apply(c32_.p, s32_.p, 0);
     k_f32_to_f64<<<g, 256>>>(s32_.p, S, ndet_);
+    HIPCHK(hipGetLastError());
+}
+
+// FP32 sigma straight into the Davidson-Liu FP32 sigma row (the mixed phase)
+void Solver::sigma_mixed32(const double* C, float* S) {
+    if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
+    unsigned g = unsigned((ndet_ + 255) / 256);
+    k_f64_to_f32<<<g, 256>>>(C, c32_.p, ndet_);
+    eng_->apply(c32_.p, S, 0);
     HIPCHK(hipGetLastError());
 }
 
@@ -563,7 +573,7 @@ double Solver::compute_energy() {
     };
     if (opt_.precision == FCI_PREC_MIXED) {
         // mixed phase: FP32 sigma until the residual reaches mixed_switch or stalls
-        dl_->add_sigma_builder([this](const double* b, double* s) { sigma_mixed(b, s); });
+        dl_->add_sigma_builder32([this](const double* b, float* s) { sigma_mixed32(b, s); });
         dl_->set_e_convergence(std::max(opt_.e_convergence, 1e-7));
         dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
         dl_->set_maxiter(opt_.maxiter);
