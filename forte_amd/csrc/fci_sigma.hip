@@ -71,17 +71,61 @@ __global__ void k_transpose(const T* __restrict__ in, T* __restrict__ out, int64
     }
 }
 
-template <class T> void transpose_device(const T* in, T* out, int64_t rows, int64_t cols, hipStream_t st) {
-    dim3 g(nblk(cols, 32), nblk(rows, 32)), b(32, 8);
-    k_transpose<T><<<g, b, 0, st>>>(in, out, rows, cols, nullptr, T(0));
+// 64x64 tiles with two-element loads and stores (even row and column counts): every load and
+// store instruction of a wave covers two 64-element row segments.
+template <class T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+template <class T>
+__global__ __launch_bounds__(256) void k_transpose64(const T* __restrict__ in, T* __restrict__ out, int64_t R,
+                                                     int64_t Cc, const T* __restrict__ base, T bs) {
+    using V2 = typename Pair<T>::type;
+    __shared__ T tile[64][65];
+    const int64_t c0 = int64_t(blockIdx.x) * 64, r0 = int64_t(blockIdx.y) * 64;
+    const int cp = threadIdx.x & 31, rr = threadIdx.x >> 5;  // column pair, row in a group of 8
+    for (int j = rr; j < 64; j += 8) {
+        const int64_t r = r0 + j, c = c0 + 2 * cp;  // c even and Cc even: c + 1 < Cc as well
+        if (r < R && c < Cc) {
+            const V2 v = *reinterpret_cast<const V2*>(in + r * Cc + c);
+            tile[j][2 * cp] = v.x;
+            tile[j][2 * cp + 1] = v.y;
+        }
+    }
+    __syncthreads();
+    for (int j = rr; j < 64; j += 8) {
+        const int64_t c = c0 + j, r = r0 + 2 * cp;  // output row c, columns r and r + 1
+        if (c < Cc && r < R) {
+            V2 v;
+            v.x = tile[2 * cp][j];
+            v.y = tile[2 * cp + 1][j];
+            if (base) {
+                const V2 b = *reinterpret_cast<const V2*>(base + c * R + r);
+                v.x += bs * b.x;
+                v.y += bs * b.y;
+            }
+            *reinterpret_cast<V2*>(out + c * R + r) = v;
+        }
+    }
+}
+
+template <class T>
+static void transpose_launch(const T* in, T* out, int64_t rows, int64_t cols, const T* base, T bs, hipStream_t st) {
+    if (rows % 2 == 0 && cols % 2 == 0) {
+        dim3 g(nblk(cols, 64), nblk(rows, 64));
+        k_transpose64<T><<<g, 256, 0, st>>>(in, out, rows, cols, base, bs);
+    } else {
+        dim3 g(nblk(cols, 32), nblk(rows, 32)), b(32, 8);
+        k_transpose<T><<<g, b, 0, st>>>(in, out, rows, cols, base, bs);
+    }
     HIPCHK(hipGetLastError());
+}
+template <class T> void transpose_device(const T* in, T* out, int64_t rows, int64_t cols, hipStream_t st) {
+    transpose_launch<T>(in, out, rows, cols, nullptr, T(0), st);
 }
 template <class T>
 void transpose_add_device(const T* in, T* out, int64_t rows, int64_t cols, const T* base, T bs,
                           hipStream_t st) {
-    dim3 g(nblk(cols, 32), nblk(rows, 32)), b(32, 8);
-    k_transpose<T><<<g, b, 0, st>>>(in, out, rows, cols, base, bs);
-    HIPCHK(hipGetLastError());
+    transpose_launch<T>(in, out, rows, cols, base, bs, st);
 }
 template void transpose_device<double>(const double*, double*, int64_t, int64_t, hipStream_t);
 template void transpose_device<float>(const float*, float*, int64_t, int64_t, hipStream_t);
