@@ -74,63 +74,92 @@ __global__ __launch_bounds__(256) void k_lp_apply(double* __restrict__ r, const 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int64_t ntile = (N + 15) / 16;
-    for (int64_t tile = int64_t(blockIdx.x) * 4 + wave; tile < ntile; tile += int64_t(gridDim.x) * 4) {
-        const int64_t I = tile * 16 + cl;
-        const bool ok = I < N;
-        const double rv = ok ? r[I] : 0.0, hv = ok ? h[I] : 0.0;
-        // B operand of layer 1: bits 8(g&1)..8(g&1)+7 of the alpha (g < 2) or beta string
-        const int64_t Ia = ok ? I / Nb : 0, Ib = ok ? I % Nb : 0;
-        const uint32_t s = g < 2 ? str_a[Ia] : str_b[Ib];
-        const int k = g < 2 ? na : nb;
-        bf8 bits, cont;
+    // two tiles of 16 samples per pass: each weight vector read from LDS feeds both, and the two
+    // independent MFMA chains overlap
+    const int64_t stride = int64_t(gridDim.x) * 4;
+    for (int64_t tile0 = int64_t(blockIdx.x) * 4 + wave; tile0 < ntile; tile0 += 2 * stride) {
+        // the weights are re-read from LDS every pass rather than held in ~200 registers: that
+        // keeps several waves per SIMD resident to hide the latency of the per-sample loads
+        asm volatile("" ::: "memory");
+        double rv[2], hv[2];
+        bool ok[2];
+        int64_t Iv[2];
+        bf8 bits[2], cont[2];
 #pragma unroll
-        for (int x = 0; x < 8; ++x) {
-            bits[x] = __bf16(ok ? lp_bit(s, n, k, 8 * (g & 1) + x) : 0.f);
-            cont[x] = __bf16(0.f);
-        }
-        if (g == 0 && ok) {
-            const float f0 = float((hv - lam) * (1.0 / 16.0));
-            const float f1 = float(rv * inv_rms);
-            cont[0] = __bf16(f0);
-            cont[1] = __bf16(f0 - float(cont[0]));
-            cont[2] = __bf16(f1);
-            cont[3] = __bf16(f1 - float(cont[2]));
-            cont[4] = __bf16(1.f);
-            cont[5] = __bf16(1.f);
-        }
-        f4 z[8];
+        for (int q = 0; q < 2; ++q) {
+            const int64_t I = (tile0 + q * stride) * 16 + cl;
+            Iv[q] = I;
+            ok[q] = I < N;
+            rv[q] = ok[q] ? r[I] : 0.0;
+            hv[q] = ok[q] ? h[I] : 0.0;
+            // B operand of layer 1: bits 8(g&1)..8(g&1)+7 of the alpha (g < 2) or beta string
+            const int64_t Ia = ok[q] ? I / Nb : 0, Ib = ok[q] ? I % Nb : 0;
+            const uint32_t st = g < 2 ? str_a[Ia] : str_b[Ib];
+            const int k = g < 2 ? na : nb;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            z[t] = f4{0.f, 0.f, 0.f, 0.f};
-            z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ws[(0 * 8 + t) * 64 + lane], bits, z[t], 0, 0, 0);
-            z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ws[(1 * 8 + t) * 64 + lane], cont, z[t], 0, 0, 0);
-        }
-        // z[t][v] = Z1[hidden 16t + 4g + v][sample cl]
-        bf8 b2[4];
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-            for (int x = 0; x < 8; ++x) b2[ks][x] = __bf16(gelu_t(z[2 * ks + x / 4][x % 4], nullptr));
-        float part = 0.f;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            f4 u = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int ks = 0; ks < 4; ++ks)
-                u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ws[((2 + ks) * 8 + t) * 64 + lane], b2[ks], u, 0, 0, 0);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int h2 = 16 * t + 4 * g + v;
-                part += Wf[LP_NH + h2] * gelu_t(u[v] + Wf[h2], nullptr);
+            for (int x = 0; x < 8; ++x) {
+                bits[q][x] = __bf16(ok[q] ? lp_bit(st, n, k, 8 * (g & 1) + x) : 0.f);
+                cont[q][x] = __bf16(0.f);
+            }
+            if (g == 0 && ok[q]) {
+                const float f0 = float((hv[q] - lam) * (1.0 / 16.0));
+                const float f1 = float(rv[q] * inv_rms);
+                cont[q][0] = __bf16(f0);
+                cont[q][1] = __bf16(f0 - float(cont[q][0]));
+                cont[q][2] = __bf16(f1);
+                cont[q][3] = __bf16(f1 - float(cont[q][2]));
+                cont[q][4] = __bf16(1.f);
+                cont[q][5] = __bf16(1.f);
             }
         }
-        part += __shfl_xor(part, 16, 64);
-        part += __shfl_xor(part, 32, 64);
-        float y = part + Wf[2 * LP_NH];
-        y = fminf(fmaxf(y, -30.f), 30.f);
-        if (g == 0 && ok) {
-            const double den = lam - hv;
-            r[I] = (fabs(den) > 1.0e-6) ? rv / den * double(__expf(y)) : 0.0;
+        f4 z[2][8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const bf8 w0 = Ws[(0 * 8 + t) * 64 + lane], w1 = Ws[(1 * 8 + t) * 64 + lane];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                z[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+                z[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, bits[q], z[q][t], 0, 0, 0);
+                z[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, cont[q], z[q][t], 0, 0, 0);
+            }
+        }
+        // z[q][t][v] = Z1[hidden 16t + 4g + v][sample cl of tile q]
+        bf8 b2[2][4];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+                for (int x = 0; x < 8; ++x) b2[q][ks][x] = __bf16(gelu_t(z[q][2 * ks + x / 4][x % 4], nullptr));
+        float part[2] = {0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            f4 u[2] = {f4{0.f, 0.f, 0.f, 0.f}, f4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const bf8 w = Ws[((2 + ks) * 8 + t) * 64 + lane];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) u[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, b2[q][ks], u[q], 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int h2 = 16 * t + 4 * g + v;
+                    part[q] += Wf[LP_NH + h2] * gelu_t(u[q][v] + Wf[h2], nullptr);
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            float pq = part[q];
+            pq += __shfl_xor(pq, 16, 64);
+            pq += __shfl_xor(pq, 32, 64);
+            float y = pq + Wf[2 * LP_NH];
+            y = fminf(fmaxf(y, -30.f), 30.f);
+            if (g == 0 && ok[q]) {
+                const double den = lam - hv[q];
+                r[Iv[q]] = (fabs(den) > 1.0e-6) ? rv[q] / den * double(__expf(y)) : 0.0;
+            }
         }
     }
 }
