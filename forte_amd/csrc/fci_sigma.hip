@@ -599,6 +599,11 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
     auto absent = [](int ks, int x) { return MS > 0 && ks * 32 + 4 * x >= MS; };
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
+    // MS > 0: a chunk's E rows are staged in this wave's LDS rows (stride CQ_ES floats) and
+    // written to E as whole 256-byte rows with 16-byte stores, half the store instructions
+    constexpr int CQ_ES = 68;
+    __shared__ __attribute__((aligned(16))) float et_all[MS > 0 ? 4 : 1][MS > 0 ? NMT * 16 * CQ_ES : 4];
+    float* et = et_all[MS > 0 ? wave : 0];
     const int ngrp = (nchunk + cpw - 1) / cpw;  // work item: one K and up to cpw consecutive chunks
     const int64_t nwork = nK * ngrp;
     const int64_t nblk = (nwork + 3) / 4;
@@ -689,7 +694,15 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
                 a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bh0[ks], a0, 0, 0, 0);
                 a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[mt][ks], bh1[ks], a1, 0, 0, 0);
             }
-            if (col < cend) {
+            if constexpr (MS > 0) {  // rows r >= M land in the unused staging rows
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * g + v;
+                    const bool ng = (neg[mt] >> v) & 1u;
+                    *reinterpret_cast<float2*>(et + r * CQ_ES + (col - c0)) =
+                        ng ? make_float2(-a0[v], -a1[v]) : make_float2(a0[v], a1[v]);
+                }
+            } else if (col < cend) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = mt * 16 + 4 * g + v;
@@ -702,6 +715,27 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
             }
         }
     };
+    // one chunk from the staging rows to E. The chunk's block of E is M contiguous rows of 64
+    // floats; columns past the slab end hold values of clamped loads that pass 2 never reads.
+    // The LDS accesses of one wave execute in order; the compiler barriers keep the writes of the
+    // two halves before these reads and these reads before the next chunk's writes.
+    auto flush = [&](int chunk) {
+        if constexpr (MS > 0) {
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_wave_barrier();
+            float* eb = E + (int64_t(chunk) * nK + K) * MS * CQ3_CW;
+#pragma unroll
+            for (int i0 = 0; i0 < MS * 16; i0 += 64) {
+                const int i = i0 + lane;
+                if (i < MS * 16) {
+                    const int r = i >> 4, c4 = (i & 15) * 4;
+                    *reinterpret_cast<float4*>(eb + r * CQ3_CW + c4) = *reinterpret_cast<const float4*>(et + r * CQ_ES + c4);
+                }
+            }
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
     // halves in two buffers used alternately: the loads of the next half are in flight while the
     // current one is multiplied and stored
     load(colof(0), xv);
@@ -711,6 +745,7 @@ __global__ __launch_bounds__(256) void k_clique3p_mixed(const float* __restrict_
         compute(xv, h);
         if (h + 2 < nh) load(colof(h + 2), xv);
         compute(xn, h + 1);
+        flush(ch0 + (h >> 1));
     }
 }
 
