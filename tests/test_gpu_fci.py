@@ -213,6 +213,28 @@ def test_mixed_precision_solve(oracle):
     assert st["sigma_builds_fp64"] < st["sigma_builds"]
 
 
+def test_mixed_precision_solve_cas14():
+    """CAS(14,14), 11.8 million determinants: the production kernels at scale (alpha-beta with the
+    A operand in LDS, same-spin pass 1 with a compile-time clique size of 36 and staged stores).
+    The mixed solve and its FP64 refinement must land on the FP64 energy."""
+    import forte_amd
+    say("mixed solve CAS(14,14)")
+    h, eri = forte_amd.seeded_integrals(14, 5)
+    ints = forte_amd.make_active_space_integrals(h, eri)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = forte_amd.FCISolver(7, 7, 1, 1, ints)
+        s.set_e_convergence(1e-10)
+        s.set_r_convergence(1e-5)
+        s.set_maxiter(400)
+        s.set_precision(prec)
+        out[prec] = (s.compute_energy(), s.stats())
+        say(" ", prec, out[prec])
+    assert out["mixed"][1]["converged"] and out["fp64"][1]["converged"]
+    assert abs(out["mixed"][0] - out["fp64"][0]) < 1e-8
+    assert out["mixed"][1]["sigma_builds_fp64"] < out["mixed"][1]["sigma_builds"]
+
+
 def test_errors_are_runtime_errors():
     import forte_amd
     p = seeded_problem(4, 2, 2, 1)
