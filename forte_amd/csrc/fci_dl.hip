@@ -306,6 +306,9 @@ __global__ __launch_bounds__(DL_BS) void k_residual_t(const TS* __restrict__ Sg,
 #pragma unroll
     for (int i = 0; i < NA; ++i) acc[i] = 0.0;
     for (int64_t e = int64_t(blockIdx.x) * DL_BS + threadIdx.x; e < n; e += int64_t(gridDim.x) * DL_BS) {
+        // coefficients re-read from LDS each element rather than held in 2m registers: more
+        // waves per SIMD to hide the loads
+        asm volatile("" ::: "memory");
         double sv[MB], bv[MB];
 #pragma unroll
         for (int j = 0; j < MB; ++j)
