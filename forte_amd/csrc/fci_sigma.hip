@@ -1390,7 +1390,10 @@ __global__ __launch_bounds__(64 * WM * WC) void k_ab3_mixed(const float* __restr
 // FP64: 16x16x4 MFMA, KS k-steps of 4 single excitations. The A operand rows are loaded in the
 // permuted order m -> rs = 4 (m % 4) + m / 4 inside each tile so that the accumulator element v of
 // lane (g = lane / 16) is row rs = 4 g + v, the layout of the 16-bit target table.
-template <int KS, int TPW>
+// SPLIT (nmt = 4 (TPW - 1) + 1): the last row tile is shared by the four row-tile waves, each
+// taking a quarter of its k-steps and scattering its partial product, so every wave runs
+// 2 (TPW - 1) + KS / 4 MFMA chains' worth of k-steps instead of one wave running TPW full tiles.
+template <int KS, int TPW, bool SPLIT>
 __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C, double* __restrict__ S, int64_t Na,
                                                     int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
                                                     const uint32_t* __restrict__ str_a,
@@ -1418,10 +1421,28 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
         vdg[r] = d;
     }
     __syncthreads();
-    double va[TPW][KS];
-    double vd[TPW][4];
+    constexpr int TF = SPLIT ? TPW - 1 : TPW;  // full row tiles per wave
+    constexpr int KQ = KS / 4;                  // k-steps of the shared tile per wave
+    static_assert(!SPLIT || KS % 4 == 0, "shared tile needs KS divisible by 4");
+    double va[TF][KS];
+    double vd[TF][4];
+    double vs[SPLIT ? KQ : 1], vds[4];
+    const int mts = WM * (TPW - 1);  // the shared tile
+    if constexpr (SPLIT) {
+        const int rs = mts * 16 + 4 * (cl % 4) + cl / 4;
 #pragma unroll
-    for (int j = 0; j < TPW; ++j) {
+        for (int q = 0; q < KQ; ++q) {
+            const int e = (wm * KQ + q) * 4 + g;
+            const uint32_t en = ent[e];
+            const bool use = mts < nmt && rs < np && e < ns && en != kInvalid;
+            const double t = vpk[use ? size_t((en >> 21) & 0x3FFu) * np + rs : 0];
+            vs[q] = use ? ((en >> 31) ? -t : t) : 0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) vds[v] = wm == 0 && mts < nmt ? vdg[mts * 16 + 4 * g + v] : 0.0;  // once
+    }
+#pragma unroll
+    for (int j = 0; j < TF; ++j) {
         const int mtile = wm + WM * j;
         const int rs = mtile * 16 + 4 * (cl % 4) + cl / 4;  // permuted A row
 #pragma unroll
@@ -1444,8 +1465,8 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
         uint4 tg[TPW];
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            const int mtile = wm + WM * j;  // the table row c0 / 32 exists for all lanes; tiles past nmt
-            const int mtc = mtile < nmt ? mtile : 0;  // are skipped below (wave-uniform)
+            const int mtile = SPLIT && j == TPW - 1 ? mts : wm + WM * j;  // the table row c0 / 32 exists
+            const int mtc = mtile < nmt ? mtile : 0;  // for all lanes; tiles past nmt are skipped below
             tg[j] = *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtc) * 64 + lane) * 8);
         }
         double2 xv[KS];
@@ -1453,8 +1474,20 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
         for (int ks = 0; ks < KS; ++ks)
             xv[ks] = *reinterpret_cast<const double2*>(C + gather_row(ent[ks * 4 + g]) * Nb + Ibc);
         const double2 cd = *reinterpret_cast<const double2*>(Crow + Ibc);
+        auto scatter = [&](const uint4& t4, const d4& a0, const d4& a1, const double (&vdj)[4]) {
+            const uint32_t tw[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
-        for (int j = 0; j < TPW; ++j) {
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
+                    if (!ok || (tt & 0x7FFFu) == kInv16) continue;
+                    const double a = (p ? a1[v] : a0[v]) + vdj[v] * (p ? cd.y : cd.x);
+                    atomicAdd(&srow64[tt & 0x7FFFu], (tt >> 15) ? -a : a);
+                }
+        };
+#pragma unroll
+        for (int j = 0; j < TF; ++j) {
             if (wm + WM * j >= nmt) continue;  // wave-uniform
             d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1462,16 +1495,22 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
                 a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].x, a0, 0, 0, 0);
                 a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].y, a1, 0, 0, 0);
             }
-            const uint32_t tw[4] = {tg[j].x, tg[j].y, tg[j].z, tg[j].w};
+            scatter(tg[j], a0, a1, vd[j]);
+        }
+        if constexpr (SPLIT) {
+            if (mts < nmt) {  // this wave's quarter of the shared tile's k-steps (compile-time indices)
+                d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+                for (int w = 0; w < WM; ++w)
+                    if (wm == w) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
-                    if (!ok || (tt & 0x7FFFu) == kInv16) continue;
-                    const double a = (p ? a1[v] : a0[v]) + vd[j][v] * (p ? cd.y : cd.x);
-                    atomicAdd(&srow64[tt & 0x7FFFu], (tt >> 15) ? -a : a);
-                }
+                        for (int q = 0; q < KQ; ++q) {
+                            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].x, a0, 0, 0, 0);
+                            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].y, a1, 0, 0, 0);
+                        }
+                    }
+                scatter(tg[TPW - 1], a0, a1, vds);
+            }
         }
     }
     __syncthreads();
@@ -1738,11 +1777,11 @@ static void ab_mixed_launch(const float* C, float* S, const SpinTables& a, const
     HIPCHK(hipGetLastError());
 }
 
-template <int KS, int TPW>
+template <int KS, int TPW, bool SPLIT>
 static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, const SpinTables& b,
                            const DevInts& ints, hipStream_t st) {
     const size_t sh = size_t(b.nstr) * sizeof(double);
-    auto kern = k_ab2_f64<KS, TPW>;
+    auto kern = k_ab2_f64<KS, TPW, SPLIT>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
                                             ints.vpk.p, (b.np + 15) / 16);
@@ -1784,8 +1823,12 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 15) / 16) * 4;  // k-steps of 4, in buckets of 16 excitations
         const int tpw = ((b.np + 15) / 16 + 3) / 4;
-#define F64CASE(K, T) \
-    if (ks == K && tpw == T) return ab2_f64_launch<K, T>(C, S, a, b, ints, st);
+        // nmt = 4 (tpw - 1) + 1: the last row tile shared by the four row-tile waves
+        const bool split = tpw > 1 && (b.np + 15) / 16 == 4 * (tpw - 1) + 1;
+#define F64CASE(K, T)                                                                      \
+    if (ks == K && tpw == T)                                                                \
+        return split ? ab2_f64_launch<K, T, (T > 1)>(C, S, a, b, ints, st)                  \
+                     : ab2_f64_launch<K, T, false>(C, S, a, b, ints, st);
 #define F64ROW(K) F64CASE(K, 1) F64CASE(K, 2) F64CASE(K, 3)
         F64ROW(4) F64ROW(8) F64ROW(12) F64ROW(16) F64ROW(20) F64ROW(24)
 #undef F64ROW
