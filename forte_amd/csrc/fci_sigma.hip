@@ -837,6 +837,91 @@ __global__ __launch_bounds__(512) void k_clique3_f64(const double* __restrict__ 
     compute(xn, c0 + 48 + cl);
 }
 
+// FP64 pass 1 for even row lengths with paired columns: lane (g, c) loads columns 2c and 2c + 1
+// of its gathered row with one 16-byte load (four rows x 256 bytes per instruction), which feed
+// two MFMA column tiles, and stores both results with one 16-byte store: a 64-column chunk is two
+// 32-column halves, half the memory instructions of k_clique3_f64.
+template <int NK4>
+__global__ __launch_bounds__(512) void k_clique3p_f64(const double* __restrict__ X, double* __restrict__ E, int64_t L,
+                                                      const uint32_t* __restrict__ clq, int M, int64_t nK,
+                                                      const double* __restrict__ G, int gdim, int nchunk,
+                                                      int64_t cbeg, int64_t Ls) {
+    constexpr int NMT = (NK4 + 3) / 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int64_t nwork = nK * nchunk;
+    const int64_t nblk = (nwork + 7) / 8;
+    const int64_t w = xcd_remap(blockIdx.x, nblk) * 8 + wave;
+    if (w >= nwork) return;
+    const int64_t K = w % nK;
+    const int chunk = int(w / nK);
+    const uint32_t* ent = clq + K * M;
+    uint32_t mb[NK4];
+#pragma unroll
+    for (int ks = 0; ks < NK4; ++ks) {
+        const int m = ks * 4 + g;
+        const uint32_t t = ent[m < M ? m : 0];  // unconditional loads (M >= 1): all in flight at once
+        mb[ks] = m < M ? t : kInvalid;
+    }
+    double va[NMT][NK4];
+    uint32_t neg[NMT];
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) {
+        const int r = mt * 16 + cl;
+        const int pr = int((ent[r < M ? r : 0] >> 21) & 0x3FFu);  // rows r >= M get zero below
+#pragma unroll
+        for (int ks = 0; ks < NK4; ++ks) {
+            const uint32_t e = mb[ks];
+            const double t = G[pr * gdim + (e != kInvalid ? int((e >> 21) & 0x3FFu) : 0)];
+            const double v = (r < M && e != kInvalid) ? t : 0.0;
+            va[mt][ks] = (e >> 31) ? -v : v;  // sign of the gathered row folded into A
+        }
+        uint32_t nb = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ro = mt * 16 + 4 * v + g;  // f64 output row
+            const uint32_t t = ent[ro < M ? ro : 0];
+            nb |= (ro < M ? t >> 31 : 0u) << v;
+        }
+        neg[mt] = nb;
+    }
+    const int64_t cend = cbeg + Ls < L ? cbeg + Ls : L;  // even
+    const int64_t c0 = cbeg + int64_t(chunk) * CQ3_CW;
+    double2 xv[NK4], xn[NK4];
+    auto load = [&](int64_t col, double2 (&xx)[NK4]) {
+        const int64_t cc = col < cend ? col : cbeg;  // col, cbeg even; past the end: not stored
+#pragma unroll
+        for (int ks = 0; ks < NK4; ++ks) xx[ks] = *reinterpret_cast<const double2*>(X + gather_row(mb[ks]) * L + cc);
+    };
+    auto compute = [&](const double2 (&xx)[NK4], int64_t col) {
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt) {
+            d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < NK4; ++ks) {
+                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt][ks], xx[ks].x, a0, 0, 0, 0);
+                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[mt][ks], xx[ks].y, a1, 0, 0, 0);
+            }
+            if (col < cend) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mt * 16 + 4 * v + g;
+                    if (r < M) {
+                        const bool ng = (neg[mt] >> v) & 1u;
+                        *reinterpret_cast<double2*>(E + ((int64_t(chunk) * nK + K) * M + r) * CQ3_CW + (col - c0)) =
+                            ng ? make_double2(-a0[v], -a1[v]) : make_double2(a0[v], a1[v]);
+                    }
+                }
+            }
+        }
+    };
+    // the two 32-column halves of the chunk, the second one's loads in flight during the first
+    load(c0 + 2 * cl, xv);
+    load(c0 + 32 + 2 * cl, xn);
+    compute(xv, c0 + 2 * cl);
+    compute(xn, c0 + 32 + 2 * cl);
+}
+
 // Pass 2: Y[I][cbeg + c] (= or +=) sum_j E[own[I][j]][c] for c < ncol: every hole string K
 // inside I hands I its row of s_m (W_K s X_K)_m. E is either row-major with rows of stride Ls
 // (CHUNKED = false) or blocked by 64-column chunk, E[(chunk * nrowE + row) * 64 + c % 64]
@@ -1650,9 +1735,26 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
             // and measured slower at CAS(16,16) in both precisions
             const size_t gsh = size_t(gd) * gd * sizeof(T);
             const bool gl = false;
+            static const bool p64 = [] {
+                const char* e = std::getenv("FCI_CQ_P64");
+                return !e || std::atoi(e) != 0;
+            }();
             if constexpr (sizeof(T) == 8) {
                 const int64_t nblk3 = (sp.nK * nchunk + 7) / 8;
                 const int nk4 = (sp.M + 3) / 4;
+                if (p64 && L % 2 == 0 && nk4 <= 16) {  // paired columns
+                    switch (nk4) {
+#define C3P(K)                                                                                            \
+    case K:                                                                                               \
+        k_clique3p_f64<K><<<xcd_grid(nblk3), 512, 0, st>>>(X, E, L, sp.clq.p, sp.M, sp.nK, ints.g2[spin].p, \
+                                                           gd, nchunk, cbeg, Ls);                         \
+        break;
+                        C3P(1) C3P(2) C3P(3) C3P(4) C3P(5) C3P(6) C3P(7) C3P(8) C3P(9) C3P(10) C3P(11) C3P(12) C3P(13)
+                        C3P(14) C3P(15) C3P(16)
+#undef C3P
+                        default: throw Error(-1, "clique kernel: no instance for this size");
+                    }
+                } else
                 switch (nk4) {
 #define C3(K)                                                                                                   \
     case K:                                                                                                     \
