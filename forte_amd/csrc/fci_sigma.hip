@@ -1955,7 +1955,7 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 31) / 32);
         const int nmt = (b.np + 15) / 16;
-        // Default for 7..10 row tiles: 8 waves each on its own 32 columns and all row tiles, the A
+        // Default for 3..10 row tiles: 8 waves each on its own 32 columns and all row tiles, the A
         // operand in LDS, the next tile's loads in flight (k_ab3_mixed<K, T, 1, 8>: 16.5 ms at
         // CAS(16,16)); otherwise 8 waves as 2 row-tile groups x 4 column groups with A in
         // registers (k_ab2_mixed<K, T, 2, 4>: 19.7 ms there, each X tile loaded by two waves).
@@ -1984,7 +1984,8 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
         if (layout == 1 && sh3 <= 160 * 1024) {
 #define MXCASE(K, T) \
     if (ks == K && nmt == T) return ab3_mixed_launch<K, T, 1, 8>(C, S, a, b, ints, st);
-            MXCASE(1, 7) MXCASE(1, 8) MXCASE(1, 9) MXCASE(1, 10) MXCASE(2, 7) MXCASE(2, 8) MXCASE(2, 9) MXCASE(2, 10)
+            MXCASE(1, 3) MXCASE(1, 4) MXCASE(1, 5) MXCASE(1, 6) MXCASE(1, 7) MXCASE(1, 8) MXCASE(1, 9) MXCASE(1, 10)
+            MXCASE(2, 3) MXCASE(2, 4) MXCASE(2, 5) MXCASE(2, 6) MXCASE(2, 7) MXCASE(2, 8) MXCASE(2, 9) MXCASE(2, 10)
 #undef MXCASE
         }
         const int tpw = (nmt + 1) / 2;
