@@ -137,6 +137,7 @@ struct orc_ctx {
     double* CR; /* Forte's static temporaries, max_str x max_str */
     double* CL;
     int64_t maxstr;
+    int lists; /* 0: strings and addresses only (orc_create_strings) */
     double timers[6];
 };
 
@@ -259,7 +260,7 @@ static void make_vvoo(orc_ctx* ctx, spin_lists* s, int p, int q, int r, int sidx
     } while (next_perm(b, n));
 }
 
-static int build_spin(orc_ctx* ctx, spin_lists* s, int k) {
+static int build_spin(orc_ctx* ctx, spin_lists* s, int k, int with_lists) {
     int n = ctx->n;
     s->k = k;
     s->nstr = (int64_t)orc_binom(n, k);
@@ -268,6 +269,7 @@ static int build_spin(orc_ctx* ctx, spin_lists* s, int k) {
     s->addr = (int32_t*)malloc(sizeof(int32_t) << n);
     for (int64_t i = 0; i < (1LL << n); ++i) s->addr[i] = -1;
     for (int64_t i = 0; i < s->nstr; ++i) s->addr[s->str[i]] = (int32_t)i; /* fci_string_address.cc:45 */
+    if (!with_lists) return 0;
     s->vo = (sublist*)calloc((size_t)(n * n), sizeof(sublist));
     s->oo = (sublist*)calloc((size_t)(ctx->npair > 0 ? ctx->npair : 1), sizeof(sublist));
     s->vvoo = (sublist*)calloc((size_t)(ctx->npair > 0 ? ctx->npair * ctx->npair : 1),
@@ -285,9 +287,9 @@ static int build_spin(orc_ctx* ctx, spin_lists* s, int k) {
     return 0;
 }
 
-orc_ctx* orc_create(int n, int na, int nb, const double* oei_a, const double* oei_b,
-                    const double* tei_aa, const double* tei_ab, const double* tei_bb,
-                    double e_core) {
+static orc_ctx* create_ctx(int n, int na, int nb, const double* oei_a, const double* oei_b,
+                           const double* tei_aa, const double* tei_ab, const double* tei_bb,
+                           double e_core, int with_lists) {
     if (n < 1 || n > 24 || na < 0 || nb < 0 || na > n || nb > n) {
         set_err("orc_create: unsupported sizes (1 <= n <= 24, 0 <= na,nb <= n)");
         return NULL;
@@ -323,22 +325,42 @@ orc_ctx* orc_create(int n, int na, int nb, const double* oei_a, const double* oe
     memcpy(ctx->tei_ab, tei_ab, n4 * sizeof(double));
     memcpy(ctx->tei_bb, tei_bb, n4 * sizeof(double));
     ctx->e_core = e_core;
-    build_spin(ctx, &ctx->sp[0], na);
-    build_spin(ctx, &ctx->sp[1], nb);
-    /* FCIVector::allocate_temp_space, fci_vector.cc:54-79 */
+    ctx->lists = with_lists;
+    build_spin(ctx, &ctx->sp[0], na, with_lists);
+    build_spin(ctx, &ctx->sp[1], nb, with_lists);
     ctx->maxstr = ctx->sp[0].nstr > ctx->sp[1].nstr ? ctx->sp[0].nstr : ctx->sp[1].nstr;
-    ctx->CR = (double*)calloc((size_t)(ctx->maxstr * ctx->maxstr), sizeof(double));
-    ctx->CL = (double*)calloc((size_t)(ctx->maxstr * ctx->maxstr), sizeof(double));
+    if (with_lists) {
+        /* FCIVector::allocate_temp_space, fci_vector.cc:54-79 */
+        ctx->CR = (double*)calloc((size_t)(ctx->maxstr * ctx->maxstr), sizeof(double));
+        ctx->CL = (double*)calloc((size_t)(ctx->maxstr * ctx->maxstr), sizeof(double));
+    }
     return ctx;
 }
+
+orc_ctx* orc_create(int n, int na, int nb, const double* oei_a, const double* oei_b,
+                    const double* tei_aa, const double* tei_ab, const double* tei_bb,
+                    double e_core) {
+    return create_ctx(n, na, nb, oei_a, oei_b, tei_aa, tei_ab, tei_bb, e_core, 1);
+}
+
+orc_ctx* orc_create_strings(int n, int na, int nb, const double* oei_a, const double* oei_b,
+                            const double* tei_aa, const double* tei_ab, const double* tei_bb,
+                            double e_core) {
+    return create_ctx(n, na, nb, oei_a, oei_b, tei_aa, tei_ab, tei_bb, e_core, 0);
+}
+
+int orc_has_lists(const orc_ctx* ctx) { return ctx->lists; }
 
 void orc_destroy(orc_ctx* ctx) {
     if (!ctx) return;
     for (int sp = 0; sp < 2; ++sp) {
         spin_lists* s = &ctx->sp[sp];
-        for (int i = 0; i < ctx->n * ctx->n; ++i) sl_free(&s->vo[i]);
-        for (int64_t i = 0; i < ctx->npair; ++i) sl_free(&s->oo[i]);
-        for (int64_t i = 0; i < ctx->npair * ctx->npair; ++i) sl_free(&s->vvoo[i]);
+        if (s->vo)
+            for (int i = 0; i < ctx->n * ctx->n; ++i) sl_free(&s->vo[i]);
+        if (s->oo)
+            for (int64_t i = 0; i < ctx->npair; ++i) sl_free(&s->oo[i]);
+        if (s->vvoo)
+            for (int64_t i = 0; i < ctx->npair * ctx->npair; ++i) sl_free(&s->vvoo[i]);
         free(s->vo);
         free(s->oo);
         free(s->vvoo);
@@ -668,6 +690,7 @@ double orc_spin2(const orc_ctx* ctx, const double* C) {
     int n = ctx->n;
     int64_t nb = ctx->sp[1].nstr;
     double spin2 = 0.0;
+#pragma omp parallel for collapse(2) schedule(dynamic) reduction(+ : spin2)
     for (int r = 0; r < n; ++r)
         for (int s = 0; s < n; ++s) {
             const sublist* va = &ctx->sp[0].vo[s * n + r];
@@ -703,6 +726,7 @@ void orc_rdms(const orc_ctx* ctx, const double* CLv, const double* CRv, int leve
             }
             int64_t maxL = alfa ? nb : na;
             const spin_lists* s = &ctx->sp[alfa ? 0 : 1];
+#pragma omp parallel for collapse(2) schedule(dynamic)
             for (int p = 0; p < n; ++p)
                 for (int q = 0; q < n; ++q) {
                     const sublist* l = &s->vo[p * n + q];
@@ -728,6 +752,9 @@ void orc_rdms(const orc_ctx* ctx, const double* CLv, const double* CRv, int leve
             }
             int64_t maxL = alfa ? nb : na;
             const spin_lists* s = &ctx->sp[alfa ? 0 : 1];
+            /* every (pq) iteration writes its own elements (the blocks pq and the transposed
+             * ones), so the pair loops run in parallel without races */
+#pragma omp parallel for schedule(dynamic)
             for (int64_t pq = 0; pq < ctx->npair; ++pq) {
                 int p = ctx->pair_p[pq], q = ctx->pair_q[pq];
                 const sublist* l = &s->oo[pq];
@@ -740,6 +767,7 @@ void orc_rdms(const orc_ctx* ctx, const double* CLv, const double* CRv, int leve
                 rdm[IDX4(q, p, p, q, n)] -= el;
                 rdm[IDX4(q, p, q, p, n)] += el;
             }
+#pragma omp parallel for schedule(dynamic)
             for (int64_t pq = 0; pq < ctx->npair; ++pq) {
                 int p = ctx->pair_p[pq], q = ctx->pair_q[pq];
                 for (int64_t rs = 0; rs < pq; ++rs) {
@@ -762,6 +790,8 @@ void orc_rdms(const orc_ctx* ctx, const double* CLv, const double* CRv, int leve
         }
         memset(g2ab, 0, n4 * sizeof(double));
         if (ctx->na >= 1 && ctx->nb >= 1) {
+            /* each (r, s) writes the elements [p, r, q, s] of its own: parallel without races */
+#pragma omp parallel for collapse(2) schedule(dynamic)
             for (int r = 0; r < n; ++r)
                 for (int s = 0; s < n; ++s) {
                     const sublist* vb = &ctx->sp[1].vo[r * n + s];
@@ -895,6 +925,79 @@ void orc_dense_hamiltonian(const orc_ctx* ctx, double* H) {
         }
 }
 
+/* The strings reached from w by at most two excitations (same electron count): level 0 is w
+ * itself, level 1 the single, level 2 the double excitations. */
+typedef struct {
+    uint64_t* str;
+    int* level;
+    int64_t size;
+} reach_t;
+
+static reach_t reach_within_two(uint64_t w, int n) {
+    uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1);
+    uint64_t occ = w, vir = full & ~w;
+    int no = popc(occ), nv = popc(vir);
+    int64_t cap = 1 + (int64_t)no * nv + (int64_t)(no * (no - 1) / 2) * (nv * (nv - 1) / 2);
+    reach_t r;
+    r.str = (uint64_t*)malloc((size_t)cap * sizeof(uint64_t));
+    r.level = (int*)malloc((size_t)cap * sizeof(int));
+    r.size = 0;
+    r.str[r.size] = w;
+    r.level[r.size++] = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!((occ >> i) & 1)) continue;
+        for (int a = 0; a < n; ++a) {
+            if (!((vir >> a) & 1)) continue;
+            r.str[r.size] = w ^ (1ull << i) ^ (1ull << a);
+            r.level[r.size++] = 1;
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            if (!((occ >> i) & 1) || !((occ >> j) & 1)) continue;
+            for (int a = 0; a < n; ++a)
+                for (int b = a + 1; b < n; ++b) {
+                    if (!((vir >> a) & 1) || !((vir >> b) & 1)) continue;
+                    r.str[r.size] = w ^ (1ull << i) ^ (1ull << j) ^ (1ull << a) ^ (1ull << b);
+                    r.level[r.size++] = 2;
+                }
+        }
+    return r;
+}
+
+/* Selected rows of sigma = H C by the Slater-Condon rules (slater_rules above,
+ * active_space_integrals.cc:247-411), summed over every determinant connected to the row: the
+ * diagonal, alpha and beta single excitations and alpha-alpha, beta-beta and alpha-beta double
+ * excitations. Independent of the string lists, so it also checks sigma at sizes whose lists
+ * or full sigma the oracle cannot afford. out[r] = (H C)[rows[r]]. */
+void orc_sigma_rows(const orc_ctx* ctx, const double* C, int64_t nrows, const int64_t* rows,
+                    double* out) {
+    const int64_t nb = ctx->sp[1].nstr;
+    const int n = ctx->n;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int64_t I = rows[r];
+        const uint64_t la = ctx->sp[0].str[I / nb], lb = ctx->sp[1].str[I % nb];
+        reach_t ra = reach_within_two(la, n), rb = reach_within_two(lb, n);
+        double acc = 0.0;
+        for (int64_t x = 0; x < ra.size; ++x) {
+            const int64_t Ja = ctx->sp[0].addr[ra.str[x]];
+            for (int64_t y = 0; y < rb.size; ++y) {
+                if (ra.level[x] + rb.level[y] > 2) continue;
+                const int64_t J = Ja * nb + ctx->sp[1].addr[rb.str[y]];
+                double h = slater_rules(ctx, la, lb, ra.str[x], rb.str[y]);
+                if (J == I) h += ctx->e_core;
+                acc += h * C[J];
+            }
+        }
+        out[r] = acc;
+        free(ra.str);
+        free(ra.level);
+        free(rb.str);
+        free(rb.level);
+    }
+}
+
 /* ---- explicit second quantization on spin-orbital strings ------------------------------ */
 /* Spin orbital index: alpha p -> p, beta p -> n + p (Determinant: alpha word first). */
 typedef struct {
@@ -1012,6 +1115,71 @@ void orc_brute_rdms(const orc_ctx* ctx, const double* CL, const double* CR, doub
                             g[IDX4(p, q, r, s, n)] += sg * CL[det_index(ctx, d)] * CR[I];
                         }
                     }
+    }
+}
+
+/* one operator (create or annihilate spin orbital so) on d; returns its sign, 0 if it vanishes */
+static double apply1(det2* d, int n, int so, int create) {
+    if (create == so_occ(*d, n, so)) return 0.0;
+    double sign = (so_count_below(*d, n, so) & 1) ? -1.0 : 1.0;
+    so_flip(d, n, so);
+    return sign;
+}
+
+/* Brute-force 3-RDMs <CL| a+p a+q a+r a_u a_t a_s |CR> at [p,q,r,s,t,u] (n^6 each), the
+ * definition the reference's own check uses (FCIVector::test_rdms, fci_vector_rdm.cc:808-845:
+ * destroy s, t, u, then create r, q, p on the right determinant). Spin patterns: aaa; aab =
+ * (p,q,s,t alpha; r,u beta); abb = (p,s alpha; q,r,t,u beta); bbb. Operators are applied one at a
+ * time so that vanishing strings are pruned early. */
+void orc_brute_rdm3(const orc_ctx* ctx, const double* CL, const double* CR, double* g3aaa, double* g3aab,
+                    double* g3abb, double* g3bbb) {
+    const int n = ctx->n;
+    const int64_t nb = ctx->sp[1].nstr, nd = ctx->sp[0].nstr * nb;
+    const size_t n6 = (size_t)n * n * n * n * n * n;
+    double* g[4] = {g3aaa, g3aab, g3abb, g3bbb};
+    /* spin of p, q, r, s, t, u (0 alpha, 1 beta) per pattern */
+    static const int spin[4][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 1, 0, 0, 1}, {0, 1, 1, 0, 1, 1}, {1, 1, 1, 1, 1, 1}};
+    for (int k = 0; k < 4; ++k) memset(g[k], 0, n6 * sizeof(double));
+    for (int64_t I = 0; I < nd; ++I) {
+        if (CR[I] == 0.0) continue;
+        const det2 d0 = {ctx->sp[0].str[I / nb], ctx->sp[1].str[I % nb]};
+        for (int k = 0; k < 4; ++k) {
+            const int* sp = spin[k];
+            for (int ss = 0; ss < n; ++ss) {
+                det2 d1 = d0;
+                double g1 = apply1(&d1, n, ss + sp[3] * n, 0);
+                if (g1 == 0.0) continue;
+                for (int t = 0; t < n; ++t) {
+                    det2 d2 = d1;
+                    double g2 = g1 * apply1(&d2, n, t + sp[4] * n, 0);
+                    if (g2 == 0.0) continue;
+                    for (int u = 0; u < n; ++u) {
+                        det2 d3 = d2;
+                        double g3 = g2 * apply1(&d3, n, u + sp[5] * n, 0);
+                        if (g3 == 0.0) continue;
+                        for (int r = 0; r < n; ++r) {
+                            det2 d4 = d3;
+                            double g4 = g3 * apply1(&d4, n, r + sp[2] * n, 1);
+                            if (g4 == 0.0) continue;
+                            for (int q = 0; q < n; ++q) {
+                                det2 d5 = d4;
+                                double g5 = g4 * apply1(&d5, n, q + sp[1] * n, 1);
+                                if (g5 == 0.0) continue;
+                                for (int pp = 0; pp < n; ++pp) {
+                                    det2 d6 = d5;
+                                    double g6 = g5 * apply1(&d6, n, pp + sp[0] * n, 1);
+                                    if (g6 == 0.0) continue;
+                                    const int64_t J = det_index(ctx, d6);
+                                    if (J < 0) continue;
+                                    const size_t idx = ((((((size_t)pp * n + q) * n + r) * n + ss) * n + t) * n + u);
+                                    g[k][idx] += g6 * CL[J] * CR[I];
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
     }
 }
 

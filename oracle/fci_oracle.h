@@ -52,6 +52,12 @@ int64_t orc_make_strings(int n, int k, uint64_t* out);
 orc_ctx* orc_create(int n, int na, int nb, const double* oei_a, const double* oei_b,
                     const double* tei_aa, const double* tei_ab, const double* tei_bb,
                     double e_core);
+/* Strings and addresses only, no excitation lists: for orc_sigma_rows, orc_hdiag and
+ * orc_dense_hamiltonian at sizes whose lists the oracle cannot afford. */
+orc_ctx* orc_create_strings(int n, int na, int nb, const double* oei_a, const double* oei_b,
+                            const double* tei_aa, const double* tei_ab, const double* tei_bb,
+                            double e_core);
+int orc_has_lists(const orc_ctx* ctx);
 void orc_destroy(orc_ctx* ctx);
 const char* orc_last_error(void);
 
@@ -105,12 +111,22 @@ void orc_rdms(const orc_ctx* ctx, const double* CL, const double* CR, int level,
 /* ---- brute-force (independent) checks ---------------------------------------------------- */
 /* Dense H in determinant order via Slater rules (active_space_integrals.cc:247-411). */
 void orc_dense_hamiltonian(const orc_ctx* ctx, double* H);
+/* Rows of sigma = H C by the Slater-Condon rules over the determinants connected to each row
+ * (singles and doubles of both spins); needs only strings and addresses. */
+void orc_sigma_rows(const orc_ctx* ctx, const double* C, int64_t nrows, const int64_t* rows,
+                    double* out);
 /* Brute-force sigma from explicit second quantization of
  * H = e_core + sum h_pq a+p a_q + 1/4 sum <pq||rs> (same spin) + sum <pq|rs> (alpha-beta). */
 void orc_brute_sigma(const orc_ctx* ctx, const double* C, double* sigma);
 /* Brute-force RDMs from explicit operator strings on determinants. */
 void orc_brute_rdms(const orc_ctx* ctx, const double* CL, const double* CR, double* g1a,
                     double* g1b, double* g2aa, double* g2ab, double* g2bb);
+
+/* Brute-force 3-RDMs <CL| a+p a+q a+r a_u a_t a_s |CR> at [p,q,r,s,t,u] (n^6 each) for the spin
+ * patterns aaa, aab (r, u beta), abb (q, r, t, u beta) and bbb, from explicit operator strings
+ * (the definition of FCIVector::test_rdms, fci_vector_rdm.cc:808-845). */
+void orc_brute_rdm3(const orc_ctx* ctx, const double* CL, const double* CR, double* g3aaa, double* g3aab,
+                    double* g3abb, double* g3bbb);
 
 /* ---- dense symmetric eigensolver (cyclic Jacobi), ascending eigenvalues ---------------- */
 void orc_jacobi(int n, const double* A, double* evals, double* evecs /* column-major: v_k = evecs[k*n..] */);

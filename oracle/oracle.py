@@ -38,6 +38,11 @@ def lib():
         L.orc_make_strings.argtypes = [ctypes.c_int, ctypes.c_int, _u64]
         L.orc_create.restype = ctypes.c_void_p
         L.orc_create.argtypes = [ctypes.c_int] * 3 + [_d] * 5 + [ctypes.c_double]
+        L.orc_create_strings.restype = ctypes.c_void_p
+        L.orc_create_strings.argtypes = [ctypes.c_int] * 3 + [_d] * 5 + [ctypes.c_double]
+        L.orc_has_lists.restype = ctypes.c_int
+        L.orc_has_lists.argtypes = [ctypes.c_void_p]
+        L.orc_sigma_rows.argtypes = [ctypes.c_void_p, _d, ctypes.c_int64, _i64, _d]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_last_error.restype = ctypes.c_char_p
         for f in ("orc_nstr",):
@@ -71,6 +76,7 @@ def lib():
         L.orc_dense_hamiltonian.argtypes = [ctypes.c_void_p, _d]
         L.orc_brute_sigma.argtypes = [ctypes.c_void_p, _d, _d]
         L.orc_brute_rdms.argtypes = [ctypes.c_void_p, _d, _d] + [_d] * 5
+        L.orc_brute_rdm3.argtypes = [ctypes.c_void_p, _d, _d] + [_d] * 4
         L.orc_jacobi.argtypes = [ctypes.c_int, _d, _d, _d]
         L.orc_dl_dense.restype = ctypes.c_int
         L.orc_dl_dense.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, _d, _d,
@@ -174,21 +180,37 @@ def dl_dense(M, nroot, hdiag=None, guesses=None, project_out=None, collapse_per_
 class Oracle:
     """One FCI problem (strings, lists, sigma, RDMs) in Forte's layout."""
 
-    def __init__(self, n, na, nb, oei_a, oei_b, tei_aa, tei_ab, tei_bb, e_core=0.0):
+    def __init__(self, n, na, nb, oei_a, oei_b, tei_aa, tei_ab, tei_bb, e_core=0.0, lists=True):
+        """lists=False builds strings and addresses only: sigma_rows, hdiag and
+        dense_hamiltonian work, the string-list functions do not."""
         self.n, self.na, self.nb = n, na, nb
         self._keep = [np.ascontiguousarray(x, dtype=np.float64)
                       for x in (oei_a, oei_b, tei_aa, tei_ab, tei_bb)]
-        self.h = lib().orc_create(n, na, nb, *[_p(x) for x in self._keep], float(e_core))
+        create = lib().orc_create if lists else lib().orc_create_strings
+        self.h = create(n, na, nb, *[_p(x) for x in self._keep], float(e_core))
         if not self.h:
             raise RuntimeError(lib().orc_last_error().decode())
+        self.lists = bool(lists)
         self.nas = int(lib().orc_nstr(self.h, 0))
         self.nbs = int(lib().orc_nstr(self.h, 1))
         self.ndet = self.nas * self.nbs
 
     @classmethod
-    def from_chem(cls, h, eri, na, nb, e_core=0.0):
+    def from_chem(cls, h, eri, na, nb, e_core=0.0, lists=True):
         aa, ab = chem_to_forte(eri)
-        return cls(h.shape[0], na, nb, h, h, aa, ab, aa, e_core)
+        return cls(h.shape[0], na, nb, h, h, aa, ab, aa, e_core, lists)
+
+    def sigma_rows(self, C, rows):
+        """(H C)[rows] by the Slater-Condon rules over the determinants each row connects to."""
+        C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
+        if C.size != self.ndet:
+            raise ValueError("C has the wrong length")
+        rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1)
+        if rows.size and (rows.min() < 0 or rows.max() >= self.ndet):
+            raise ValueError("row index out of range")
+        out = np.zeros(rows.size)
+        lib().orc_sigma_rows(self.h, _p(C), rows.size, _p(rows, ctypes.c_int64), _p(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -203,7 +225,12 @@ class Oracle:
     def address(self, spin, s):
         return int(lib().orc_address(self.h, spin, int(s)))
 
+    def _need_lists(self):
+        if not self.lists:
+            raise RuntimeError("this oracle context was built without excitation lists")
+
     def _list(self, size, getter, *args):
+        self._need_lists()
         sign = np.zeros(max(size, 1))
         I = np.zeros(max(size, 1), dtype=np.uint32)
         J = np.zeros(max(size, 1), dtype=np.uint32)
@@ -211,6 +238,7 @@ class Oracle:
         return sign[:size], I[:size], J[:size]
 
     def vo(self, spin, p, q):
+        self._need_lists()
         return self._list(int(lib().orc_vo_size(self.h, spin, p, q)), lib().orc_vo_get, spin, p, q)
 
     def npair(self):
@@ -222,14 +250,17 @@ class Oracle:
         return p.value, q.value
 
     def oo(self, spin, pq):
+        self._need_lists()
         return self._list(int(lib().orc_oo_size(self.h, spin, pq)), lib().orc_oo_get, spin,
                           ctypes.c_int64(pq))
 
     def vvoo(self, spin, p, q, r, s):
+        self._need_lists()
         return self._list(int(lib().orc_vvoo_size(self.h, spin, p, q, r, s)), lib().orc_vvoo_get,
                           spin, p, q, r, s)
 
     def sigma(self, C, mask=63):
+        self._need_lists()
         C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
         out = np.zeros(self.ndet)
         lib().orc_sigma(self.h, _p(C), _p(out), mask)
@@ -242,6 +273,7 @@ class Oracle:
 
     def sigma_sample(self, C, stride):
         """Strided timing sample of the Forte-structured sigma: (seconds[6], fraction[6])."""
+        self._need_lists()
         C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
         out = np.zeros(self.ndet)
         t, w = np.zeros(6), np.zeros(6)
@@ -249,6 +281,7 @@ class Oracle:
         return t, w
 
     def sigma_aabb_range(self, C, rs_begin, rs_end, out=None):
+        self._need_lists()
         C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
         if out is None:
             out = np.zeros(self.ndet)
@@ -261,10 +294,12 @@ class Oracle:
         return out
 
     def spin2(self, C):
+        self._need_lists()
         C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
         return float(lib().orc_spin2(self.h, _p(C)))
 
     def rdms(self, CL, CR=None, level=2):
+        self._need_lists()
         CL = np.ascontiguousarray(CL, dtype=np.float64).reshape(-1)
         CR = CL if CR is None else np.ascontiguousarray(CR, dtype=np.float64).reshape(-1)
         n = self.n
@@ -293,9 +328,19 @@ class Oracle:
         lib().orc_brute_rdms(self.h, _p(CL), _p(CR), _p(g1a), _p(g1b), *[_p(g) for g in g2])
         return g1a, g1b, g2[0], g2[1], g2[2]
 
+    def brute_rdm3(self, CL, CR=None):
+        """(g3aaa, g3aab, g3abb, g3bbb): <CL| a+p a+q a+r a_u a_t a_s |CR> at [p,q,r,s,t,u]."""
+        CL = np.ascontiguousarray(CL, dtype=np.float64).reshape(-1)
+        CR = CL if CR is None else np.ascontiguousarray(CR, dtype=np.float64).reshape(-1)
+        n = self.n
+        g = [np.zeros((n,) * 6) for _ in range(4)]
+        lib().orc_brute_rdm3(self.h, _p(CL), _p(CR), *[_p(x) for x in g])
+        return tuple(g)
+
     def solve(self, nroot=1, multiplicity=1, guess_per_root=1, ndets_per_guess=50,
               collapse_per_root=2, subspace_per_root=10, maxiter=100, e_conv=1e-10,
               r_conv=1e-6, want_evecs=True):
+        self._need_lists()
         o = FCIOpts(nroot, multiplicity, guess_per_root, ndets_per_guess, collapse_per_root,
                     subspace_per_root, maxiter, e_conv, r_conv)
         e = np.zeros(nroot)

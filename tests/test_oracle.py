@@ -129,6 +129,48 @@ def test_sigma_against_brute_force(oracle, n, na, nb, seed):
     assert np.abs(H @ C - sb).max() < 1e-11 * max(1.0, np.abs(sb).max())
 
 
+@pytest.mark.parametrize("n,na,nb,seed", [(4, 2, 2, 1), (5, 3, 2, 2), (6, 3, 3, 3), (6, 2, 4, 4),
+                                          (5, 1, 1, 6), (6, 5, 1, 7), (4, 4, 0, 8), (7, 3, 3, 9)])
+def test_sigma_rows_against_brute_force(oracle, n, na, nb, seed):
+    """The Slater-Condon row sigma (the checker at sizes beyond a full oracle sigma) against the
+    explicit second-quantized sigma, on a context with lists and on a strings-only one."""
+    p = seeded_problem(n, na, nb, seed, e_core=0.37)
+    C = np.random.default_rng(seed).standard_normal(oracle.binom(n, na) * oracle.binom(n, nb))
+    for lists in (True, False):
+        o = oracle.Oracle(n, na, nb, p["h"], p["h"], p["aa"], p["ab"], p["aa"], 0.37, lists=lists)
+        sb = o.brute_sigma(C)
+        rows = np.arange(o.ndet)[::-1].copy()
+        got = o.sigma_rows(C, rows)
+        assert np.abs(got - sb[rows]).max() < 1e-11 * max(1.0, np.abs(sb).max())
+
+
+@pytest.mark.parametrize("n,na,nb,seed", [(4, 2, 2, 1), (6, 3, 3, 3), (6, 2, 4, 4), (6, 5, 1, 7),
+                                          (4, 4, 0, 8), (8, 4, 4, 5)])
+def test_hdiag_is_the_hamiltonian_diagonal(oracle, n, na, nb, seed):
+    """orc_hdiag (the checker of the GPU diagonal, bit-exact there) equals the diagonal of the
+    Slater-rules Hamiltonian and the brute-force sigma of unit vectors."""
+    p = seeded_problem(n, na, nb, seed, e_core=0.7)
+    o = oracle.Oracle(n, na, nb, p["h"], p["h"], p["aa"], p["ab"], p["aa"], 0.7)
+    d = o.hdiag()
+    H = o.dense_hamiltonian()
+    assert np.abs(d - np.diag(H)).max() < 1e-12 * max(1.0, np.abs(d).max())
+    for I in np.random.default_rng(seed).choice(o.ndet, size=min(5, o.ndet), replace=False):
+        e = np.zeros(o.ndet)
+        e[I] = 1.0
+        assert abs(o.brute_sigma(e)[I] - d[I]) < 1e-12 * max(1.0, abs(d[I]))
+    lite = oracle.Oracle(n, na, nb, p["h"], p["h"], p["aa"], p["ab"], p["aa"], 0.7, lists=False)
+    assert np.array_equal(lite.hdiag(), d)
+
+
+def test_strings_only_context_refuses_list_functions(oracle):
+    p = seeded_problem(6, 3, 3, 1)
+    o = oracle.Oracle(6, 3, 3, p["h"], p["h"], p["aa"], p["ab"], p["aa"], lists=False)
+    for call in (lambda: o.sigma(np.zeros(o.ndet)), lambda: o.vo(0, 1, 2), lambda: o.rdms(np.zeros(o.ndet)),
+                 lambda: o.solve()):
+        with pytest.raises(RuntimeError):
+            call()
+
+
 def test_sigma_terms_sum(oracle):
     p = seeded_problem(6, 3, 3, 11, e_core=1.5)
     o = oracle.Oracle(6, 3, 3, p["h"], p["h"], p["aa"], p["ab"], p["aa"], 1.5)
@@ -146,6 +188,27 @@ def test_rdms_against_brute_force(oracle, n, na, nb):
     C = np.random.default_rng(2).standard_normal(o.ndet)
     for a, b in zip(o.rdms(C), o.brute_rdms(C)):
         assert np.abs(a - b).max() < 1e-12 * max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (5, 3, 2), (5, 1, 3)])
+def test_transition_rdms_against_brute_force(oracle, n, na, nb):
+    """<L| ... |R> with two different vectors (the transition RDMs' checker)."""
+    p = seeded_problem(n, na, nb, 23)
+    o = oracle.Oracle(n, na, nb, p["h"], p["h"], p["aa"], p["ab"], p["aa"])
+    rng = np.random.default_rng(5)
+    L, R = rng.standard_normal(o.ndet), rng.standard_normal(o.ndet)
+    got, ref = o.rdms(L, R), o.brute_rdms(L, R)
+    for k in (0, 1, 3):  # g1a, g1b, g2ab
+        assert np.abs(got[k] - ref[k]).max() < 1e-12 * max(1.0, np.abs(ref[k]).max())
+    # same spin: Forte computes the elements with pair(p,q) >= pair(r,s) and copies each to its
+    # transpose (fci_vector_rdm.cc:240-285), so only that half equals <L|a+p a+q a_s a_r|R>
+    pidx = lambda a, b: a * (a - 1) // 2 + b  # noqa: E731
+    for k in (2, 4):
+        for pp, q, r, t in itertools.product(range(n), repeat=4):
+            if pp > q and r > t:
+                A, B = pidx(pp, q), pidx(r, t)
+                want = ref[k][pp, q, r, t] if A >= B else ref[k][r, t, pp, q]
+                assert abs(got[k][pp, q, r, t] - want) < 1e-12 * max(1.0, np.abs(ref[k]).max())
 
 
 def test_energy_from_rdms_and_traces(oracle):
