@@ -154,11 +154,18 @@ class SigmaEngine {
     // sigma = H C on device buffers, FP64 or FP32 storage
     void apply(const double* C, double* S, hipStream_t st);
     void apply(const float* C, float* S, hipStream_t st);
+    // the same with caller-provided scratch: the transposed vectors and the same-spin intermediate
+    // are carved out of `ext` (ext_bytes, 256-byte aligned); nothing is allocated and nothing is
+    // waited for. Throws before launching anything if ext_bytes < scratch_min_bytes(sizeof(T)).
+    void apply_ext(const double* C, double* S, hipStream_t st, void* ext, size_t ext_bytes);
+    void apply_ext(const float* C, float* S, hipStream_t st, void* ext, size_t ext_bytes);
+    // smallest external scratch one sigma needs with w-byte elements
+    size_t scratch_min_bytes(size_t w) const;
     void timings(double* ms3) const;
     void set_timing(bool on) { timing_ = on; }
 
   private:
-    template <class T> void apply_t(const T* C, T* S, hipStream_t st);
+    template <class T> void apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, size_t ext_bytes);
     const SpinTables* a_;
     const SpinTables* b_;
     const DevInts* ints_;

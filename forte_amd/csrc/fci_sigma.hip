@@ -1674,9 +1674,26 @@ static size_t clique_scratch_bytes() {
 // Same-spin product Y (+)= sum_K P_K^T S_K W_K S_K P_K X over the pair cliques of one spin, X and
 // Y with L columns. Pass 1 (k_clique2_*<STORE>) writes E_K = S_K W_K S_K X_K for a column chunk,
 // pass 2 (k_clique_gather) sums, for every string, the C(k,d) rows of E that address it.
+// True when launch_clique runs the two-pass path (and so needs the intermediate E).
+template <class T> static bool clique_two_pass(const SpinTables& sp, const DevInts& ints, int spin) {
+    if (sp.d == 0 || sp.nK == 0) return false;
+    const int gd = ints.gdim[spin];
+    const int Mp = (sp.M + 15) & ~15, Mk = (sp.M + 31) & ~31;
+    const size_t sh = sizeof(T) == 8 ? size_t(8) * Mp * 16 * 8 + size_t(8) * Mp * 4
+                                     : ((size_t(gd) * gd + 3) & ~size_t(3)) * 4 +
+                                           size_t(8) * 2 * 32 * (Mk + 8) * 2 + size_t(8) * Mk * 4;
+    return !(sh > 150 * 1024 || sp.M > 1023);
+}
+// Column width of one pass-1 work item (the slab width is a multiple of it).
+template <class T> static int clique_tile_width(const SpinTables& sp) {
+    return sp.M <= 64 ? CQ3_CW : (sizeof(T) == 8 ? 16 : 32);
+}
+
+// ext != nullptr: E comes from the caller's ext_bytes instead of SigmaWork (never allocated).
 template <class T>
 static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, const DevInts& ints, int spin,
-                          bool accumulate, SigmaWork& w, hipStream_t st) {
+                          bool accumulate, SigmaWork& w, hipStream_t st, unsigned char* ext = nullptr,
+                          size_t ext_bytes = 0) {
     if (sp.d == 0 || sp.nK == 0) {
         if (!accumulate) k_fill<T><<<nblk(sp.nstr * L, 256), 256, 0, st>>>(Y, sp.nstr * L, T(0));
         return;
@@ -1687,7 +1704,7 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
     const size_t sh = sizeof(T) == 8 ? size_t(WV64) * Mp * TW64 * 8 + size_t(WV64) * Mp * 4
                                      : ((size_t(gd) * gd + 3) & ~size_t(3)) * 4 +
                                            size_t(WV32) * 2 * TW32 * (Mk + 8) * 2 + size_t(WV32) * Mk * 4;
-    if (sh > 150 * 1024 || sp.M > 1023) {
+    if (!clique_two_pass<T>(sp, ints, spin)) {
         // large cliques: one-pass kernel with global atomics
         if (sp.M > CQ_MAXM) throw Error(-1, "clique size exceeds kernel limit");
         if (!accumulate) k_fill<T><<<nblk(sp.nstr * L, 256), 256, 0, st>>>(Y, sp.nstr * L, T(0));
@@ -1703,19 +1720,26 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
         return;
     }
     const bool reg = sp.M <= 64;  // register-resident pass 1
-    const int tw = reg ? CQ3_CW : (sizeof(T) == 8 ? TW64 : TW32);
+    const int tw = clique_tile_width<T>(sp);
     const int wv = sizeof(T) == 8 ? WV64 : WV32;
     const int64_t rowsE = sp.nK * sp.M;
     const int64_t Lr = (L + tw - 1) / tw * tw;
-    int64_t Ls = int64_t(clique_scratch_bytes() / (size_t(rowsE) * sizeof(T))) / tw * tw;
+    const size_t cap = ext ? std::min(ext_bytes, clique_scratch_bytes()) : clique_scratch_bytes();
+    int64_t Ls = int64_t(cap / (size_t(rowsE) * sizeof(T))) / tw * tw;
     Ls = std::max<int64_t>(tw, std::min<int64_t>(Ls, Lr));
-    DBuf<unsigned char>& eb = w.clique;
     const size_t need = size_t(rowsE) * size_t(Ls) * sizeof(T);
-    if (eb.n < need) {
-        HIPCHK(hipStreamSynchronize(st));
-        eb.alloc(need);
+    T* E;
+    if (ext) {
+        if (need > ext_bytes) throw Error(-1, "sigma: external scratch too small for the same-spin intermediate");
+        E = reinterpret_cast<T*>(ext);
+    } else {
+        DBuf<unsigned char>& eb = w.clique;
+        if (eb.n < need) {
+            HIPCHK(hipStreamSynchronize(st));
+            eb.alloc(need);
+        }
+        E = reinterpret_cast<T*>(eb.p);
     }
-    T* E = reinterpret_cast<T*>(eb.p);
     const int nct = int(Ls / tw);
     int kch = 64;
     while (kch > wv && int64_t(nct) * ((sp.nK + kch - 1) / kch) < 2048) kch /= 2;
@@ -2009,26 +2033,65 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
     }
 }
 
-template <class T> void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st) {
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t SigmaEngine::scratch_min_bytes(size_t w) const {
+    const SpinTables& a = *a_;
+    const SpinTables& b = *b_;
+    const size_t nd = size_t(a.nstr) * size_t(b.nstr);
+    const size_t vec = b.d > 0 ? 2 * align256(nd * w) : 0;
+    size_t e = 0;
+    for (int spin = 0; spin < 2; ++spin) {
+        const SpinTables& sp = spin ? b : a;
+        if (sp.d == 0) continue;
+        const bool two = w == 8 ? clique_two_pass<double>(sp, *ints_, spin) : clique_two_pass<float>(sp, *ints_, spin);
+        if (!two) continue;
+        const int tw = w == 8 ? clique_tile_width<double>(sp) : clique_tile_width<float>(sp);
+        e = std::max(e, size_t(sp.nK) * size_t(sp.M) * size_t(tw) * w);
+    }
+    return vec + e;
+}
+
+template <class T>
+void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, size_t ext_bytes) {
     const SpinTables& a = *a_;
     const SpinTables& b = *b_;
     const int64_t Na = a.nstr, Nb = b.nstr, nd = Na * Nb;
     const T ecore = T(ints_->e_core);
+    unsigned char* eext = nullptr;  // same-spin intermediate inside the external scratch
+    size_t eext_bytes = 0;
+    T *ct = nullptr, *yt = nullptr;
+    if (ext) {
+        if (ext_bytes < scratch_min_bytes(sizeof(T)))
+            throw Error(-1, "sigma: this problem needs more scratch than the external region holds");
+        size_t off = 0;
+        if (b.d > 0) {
+            ct = reinterpret_cast<T*>(ext);
+            yt = reinterpret_cast<T*>(ext + align256(size_t(nd) * sizeof(T)));
+            off = 2 * align256(size_t(nd) * sizeof(T));
+        }
+        eext = ext + off;
+        eext_bytes = ext_bytes - off;
+    }
     if (timing_) HIPCHK(hipEventRecord(w_->ev[0], st));
     // beta same-spin on C^T, then sigma = (Y^T) + e_core C   (H0, fci_vector_hamiltonian.cc:84-91)
     if (b.d > 0) {
-        DBuf<T>& Ct = scratch<T>(*w_, 0);
-        DBuf<T>& Yt = scratch<T>(*w_, 1);
-        if (Ct.n < size_t(nd)) Ct.alloc(nd);
-        if (Yt.n < size_t(nd)) Yt.alloc(nd);
-        transpose_device<T>(C, Ct.p, Na, Nb, st);
-        launch_clique<T>(Ct.p, Yt.p, Na, b, *ints_, 1, false, *w_, st);
-        transpose_add_device<T>(Yt.p, S, Nb, Na, C, ecore, st);
+        if (!ext) {
+            DBuf<T>& Ct = scratch<T>(*w_, 0);
+            DBuf<T>& Yt = scratch<T>(*w_, 1);
+            if (Ct.n < size_t(nd)) Ct.alloc(nd);
+            if (Yt.n < size_t(nd)) Yt.alloc(nd);
+            ct = Ct.p;
+            yt = Yt.p;
+        }
+        transpose_device<T>(C, ct, Na, Nb, st);
+        launch_clique<T>(ct, yt, Na, b, *ints_, 1, false, *w_, st, eext, eext_bytes);
+        transpose_add_device<T>(yt, S, Nb, Na, C, ecore, st);
     } else {
         k_scale_copy<T><<<nblk(nd, 256), 256, 0, st>>>(C, S, nd, ecore);
     }
     if (timing_) HIPCHK(hipEventRecord(w_->ev[1], st));
-    if (a.d > 0) launch_clique<T>(C, S, Nb, a, *ints_, 0, true, *w_, st);
+    if (a.d > 0) launch_clique<T>(C, S, Nb, a, *ints_, 0, true, *w_, st, eext, eext_bytes);
     if (timing_) HIPCHK(hipEventRecord(w_->ev[2], st));
     if (a.k > 0 && b.k > 0) launch_ab(C, S, a, b, *ints_, st);
     if (timing_) {
@@ -2042,7 +2105,15 @@ template <class T> void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st) {
     }
 }
 
-void SigmaEngine::apply(const double* C, double* S, hipStream_t st) { apply_t<double>(C, S, st); }
-void SigmaEngine::apply(const float* C, float* S, hipStream_t st) { apply_t<float>(C, S, st); }
+void SigmaEngine::apply(const double* C, double* S, hipStream_t st) { apply_t<double>(C, S, st, nullptr, 0); }
+void SigmaEngine::apply(const float* C, float* S, hipStream_t st) { apply_t<float>(C, S, st, nullptr, 0); }
+void SigmaEngine::apply_ext(const double* C, double* S, hipStream_t st, void* ext, size_t ext_bytes) {
+    if (!ext) throw Error(-1, "sigma: null external scratch");
+    apply_t<double>(C, S, st, static_cast<unsigned char*>(ext), ext_bytes);
+}
+void SigmaEngine::apply_ext(const float* C, float* S, hipStream_t st, void* ext, size_t ext_bytes) {
+    if (!ext) throw Error(-1, "sigma: null external scratch");
+    apply_t<float>(C, S, st, static_cast<unsigned char*>(ext), ext_bytes);
+}
 
 }  // namespace fci

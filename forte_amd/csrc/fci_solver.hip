@@ -716,7 +716,38 @@ using fci::Error;
 struct fci_solver {
     std::unique_ptr<fci::Solver> s;
     std::string err;
+    int dev = 0;  // device current when the handle was created
 };
+
+// ---- scratch of fci_sigma_device_noalloc ----------------------------------------------------
+// One static device array per GPU, held until the process ends. In host code `scratch` is a
+// host-side stand-in, not a device address: host code uses only the per-device address that
+// hipGetSymbolAddress returns (looked up once per device by fci_create, with that device
+// current). Every no-alloc call on a GPU shares that GPU's array; overlapping calls race.
+static __device__ __align__(256) unsigned char scratch[FCI_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void setup_noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw Error(FCI_ERR_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void* ptr = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !ptr)
+        throw Error(FCI_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+    g_scratch_ptr[dev] = ptr;
+}
+
+void* noalloc_scratch(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+}
+}  // namespace
+
 struct fci_dl {
     std::unique_ptr<fci::DavidsonLiu> d;
     fci::DBuf<double> M;
@@ -771,6 +802,8 @@ int fci_create(int n, int na, int nb, const double* oa, const double* ob, const 
         if (!oa || !ob || !taa || !tab || !tbb) throw Error(FCI_ERR_ARG, "null integral pointer");
         h->s.reset(new fci::Solver(n, na, nb));
         h->s->set_integrals(oa, ob, taa, tab, tbb, e_core);
+        HIPCHK(hipGetDevice(&h->dev));
+        setup_noalloc_scratch(h->dev);
     });
     if (st == FCI_OK) *out = h.release();
     return st;
@@ -927,6 +960,33 @@ int fci_sigma_device(fci_solver* s, const void* C, void* sigma, int dtype, void*
         else
             throw Error(FCI_ERR_ARG, "dtype must be 0 (FP64) or 1 (FP32)");
         S.eng_->set_timing(false);
+    });
+}
+
+int fci_sigma_device_noalloc(fci_solver* s, const void* C, void* sigma, int dtype, void* stream) {
+    if (!s || !C || !sigma) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        if (dtype != 0 && dtype != 1) throw Error(FCI_ERR_ARG, "dtype must be 0 (FP64) or 1 (FP32)");
+        int cur = -1;
+        HIPCHK(hipGetDevice(&cur));
+        if (cur != s->dev) throw Error(FCI_ERR_STATE, "fci_sigma_device_noalloc: the solver was created on another device");
+        void* ptr = noalloc_scratch(s->dev);
+        if (!ptr) throw Error(FCI_ERR_STATE, "fci_sigma_device_noalloc: scratch not set up on this device");
+        fci::Solver& S = *s->s;
+        const size_t need = S.eng_->scratch_min_bytes(dtype == 0 ? 8 : 4);
+        if (need > size_t(FCI_NOALLOC_SCRATCH_BYTES))
+            throw Error(FCI_ERR_ARG, "fci_sigma_device_noalloc: this problem needs " + std::to_string(need) +
+                                         " bytes of scratch, more than the " +
+                                         std::to_string(size_t(FCI_NOALLOC_SCRATCH_BYTES)) + "-byte maximum");
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        S.eng_->set_timing(false);
+        if (dtype == 0)
+            S.eng_->apply_ext(static_cast<const double*>(C), static_cast<double*>(sigma), st, ptr,
+                              size_t(FCI_NOALLOC_SCRATCH_BYTES));
+        else
+            S.eng_->apply_ext(static_cast<const float*>(C), static_cast<float*>(sigma), st, ptr,
+                              size_t(FCI_NOALLOC_SCRATCH_BYTES));
+        HIPCHK(hipGetLastError());
     });
 }
 

@@ -112,6 +112,17 @@ int fci_sigma(fci_solver* s, const double* C, double* sigma);
 /* Same on device buffers already resident in HBM (no copies). dtype 0 = FP64, 1 = FP32
  * (mixed-precision kernels). stream may be NULL (default stream). */
 int fci_sigma_device(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, void* stream);
+/* No counterpart in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). fci_sigma_device with
+ * its scratch (the transposed C and its product, the same-spin intermediate) taken from one
+ * static device array of FCI_NOALLOC_SCRATCH_BYTES per GPU instead of buffers the solver
+ * allocates: no allocation, no host synchronization, no timings. Problems that need more scratch
+ * are refused with FCI_ERR_ARG before anything is launched (CAS(12,12) and smaller fit in both
+ * precisions, CAS(14,14) does not). The solver must have been created on the current device.
+ * Every call on a GPU shares that GPU's array and leaves its contents behind: calls that
+ * overlap on one GPU (other streams or threads) race. */
+#define FCI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int fci_sigma_device_noalloc(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, void* stream);
 /* Per-kernel device time (ms) of the last fci_sigma_device call measured with HIP events:
  * [0] same-spin beta (incl. transposes), [1] same-spin alpha, [2] alpha-beta. */
 int fci_sigma_timings(const fci_solver* s, double* ms3);
