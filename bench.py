@@ -9,17 +9,23 @@ contractions accumulated in FP32). `value` is the sigma throughput in GB/s, defi
 9 * N_det * 4 bytes / t_sigma (C read by the two same-spin and the alpha-beta passes, sigma
 written and re-read), inputs resident in HBM. The line also carries:
   roofline      MFMA roofline of the dominant kernel (k_ab3_mixed, the alpha-beta contraction),
-                timed with HIP events on the stream it runs on; algorithmic flops per launch =
-                3 * 2 * N_det * P^2 with P = n(n+1)/2 packed pairs (the packed-pair GEMM
-                formulation the benchmark fixes; BF16x3 = 3 BF16 products), against the
-                2.5 PFLOP/s dense BF16 peak. The kernel itself executes 3 * 2 * N_det * P * E
-                products (E = k(n-k) alpha single excitations; the diagonal alpha term is a rank-1
-                VALU update), reported as executed_flops/executed_tflops;
+                timed with HIP events on the stream it runs on; flops per launch = the BF16 MFMA
+                flops the kernel executes, 3 * 2 * N_det * P * E (P = n(n+1)/2 packed beta pairs,
+                E = k(n-k) alpha single excitations, BF16x3 = 3 BF16 products; the diagonal alpha
+                term is a rank-1 VALU update), against the 2.5 PFLOP/s dense BF16 peak. The
+                packed-pair GEMM count 3 * 2 * N_det * P^2 is reported beside it (packed_pair_*),
+                not as the fraction: the kernel never runs those flops.
                 traffic = HBM bytes per launch from the committed rocprofv3 counter summary;
+  kernel_rooflines  every sigma kernel group, mixed and FP64, against its bound (same-spin:
+                    2 N_det (s + oo + vvoo) flops on the FP32 / FP64 VALU; alpha-beta: executed
+                    MFMA flops);
   fp64_sigma    the FP64 sigma of the plain solver on the same vector;
-  time_to_solution  full Davidson-Liu solves to the same thresholds: FP64 with the diagonal
-                    preconditioner, mixed precision (FP32/BF16x3 sigma + FP64 refinement) with
-                    the learned preconditioner (shipped weights) and with the diagonal one;
+  time_to_solution  full Davidson-Liu solves to e_conv 1e-9 at r_conv 1e-4 and at Forte's
+                    default 1e-6: FP64 with the diagonal preconditioner, mixed precision
+                    (FP32/BF16x3 sigma + FP64 refinement) with the learned preconditioner (shipped
+                    weights) and with the diagonal one;
+  configs       CAS(8,8) FP64 and CAS(14,14) FP64 3-root solves (with root-0 RDM time) and their
+                CPU baselines;
   cpu_baseline  the CPU restatement of Forte's sigma (oracle/fci_oracle.c, one core) on a
                 strided sample of its loops, extrapolated to one full sigma.
 """
@@ -33,6 +39,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
+
+
+def host_cpu():
+    """Model name and core counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=usable)
 
 
 def traffic_from_profiles(n):
@@ -57,6 +84,7 @@ def cpu_baseline(n, h, eri, sample_stride):
     t, w = o.sigma_sample(C, sample_stride)
     est = float(sum(ti / wi for ti, wi in zip(t, w) if wi > 0))
     return dict(value=round(9 * o.ndet * 8 / est / 1e9, 4), unit="GB/s", cores=1, kind="port",
+                host=host_cpu(),
                 seconds_per_sigma_extrapolated=round(est, 2), sampled_seconds=round(float(t.sum()), 2),
                 list_setup_seconds=round(setup, 2),
                 sample=(f"FP64 Forte-structured sigma (oracle/fci_oracle.c) of the same CAS({n},{n}) "
@@ -94,9 +122,69 @@ def batch_cpu_baseline(probs, gpu_results):
         counts[n_] = counts.get(n_, 0) + 1
     total = sum(counts[k] * per_n_time[k] for k in counts if k in per_n_time)
     return dict(value=round(len(probs) / total, 4), unit="problems/s", cores=1, kind="port",
+                host=host_cpu(),
                 seconds_extrapolated=round(total, 1),
                 sample="oracle FP64 solves of 2 problems each for n = 6, 8, 10; n = 12 from one strided "
                        "sigma sample x 2 x the GPU iteration count; scaled by the problem counts")
+
+
+def other_configs(forte_amd, with_cpu):
+    """CAS(8,8) FP64 (seed 1, e_conv 1e-10) and CAS(14,14) FP64 3 roots (seed 2, subspace 8 per
+    root, e_conv 1e-9) solved on the GPU, with root-0 RDM time; CPU baselines from the oracle: a
+    full CAS(8,8) solve, and for CAS(14,14) a strided sigma sample extrapolated to one sigma times
+    the GPU's sigma count."""
+    import numpy as np
+    out = {}
+    for key, n, seed, nroot, e_conv, spr in (("cas8_fp64_1root", 8, 1, 1, 1e-10, 10),
+                                             ("cas14_fp64_3roots", 14, 2, 3, 1e-9, 8)):
+        h, eri = forte_amd.seeded_integrals(n, seed)
+        ints = forte_amd.make_active_space_integrals(h, eri)
+        s = forte_amd.FCISolver(n // 2, n // 2, 1, nroot, ints)
+        s.set_e_convergence(e_conv)
+        s.set_r_convergence(1e-6)
+        s.set_subspace_per_root(spr)
+        s.set_maxiter(500)
+        forte_amd.device_synchronize()
+        t0 = time.perf_counter()
+        s.compute_energy()
+        forte_amd.device_synchronize()
+        dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        s.rdms([(0, 0)], 2)
+        forte_amd.device_synchronize()
+        t_rdm = time.perf_counter() - t0
+        st = s.stats()
+        d = dict(workload=f"CAS({n},{n}) seed {seed}, {nroot} root(s), FP64, e_conv {e_conv:g}, r_conv 1e-6, "
+                          f"DL_SUBSPACE_PER_ROOT {spr}",
+                 seconds=round(dt, 4), rdm_seconds=round(t_rdm, 4), energies=list(s.energies()),
+                 **{k: st[k] for k in ("iterations", "sigma_builds", "converged")})
+        print(f"# {key}: {d}", file=sys.stderr, flush=True)
+        if with_cpu:
+            from oracle import oracle as O
+            o = O.Oracle.from_chem(h, eri, n // 2, n // 2)
+            if n <= 10:
+                t0 = time.perf_counter()
+                r = o.solve(nroot=nroot, e_conv=e_conv, r_conv=1e-6, subspace_per_root=spr, maxiter=500,
+                            want_evecs=False)
+                tc = time.perf_counter() - t0
+                d["cpu_baseline"] = dict(seconds=round(tc, 3), cores=1, kind="port", host=host_cpu(),
+                                         sample="full oracle FCISolver solve (Forte-structured FP64 sigma, "
+                                                "Davidson-Liu), same thresholds",
+                                         abs_dE=float(np.abs(np.array(r["energies"]) - np.array(s.energies())).max()),
+                                         iterations=r["niter"])
+            else:
+                C = np.random.default_rng(7).standard_normal(o.ndet)
+                t, w = o.sigma_sample(C, 16)
+                est = float(sum(ti / wi for ti, wi in zip(t, w) if wi > 0))
+                d["cpu_baseline"] = dict(seconds=round(est * st["sigma_builds"], 1), cores=1, kind="port",
+                                         host=host_cpu(), sigma_seconds=round(est, 2),
+                                         sample="oracle Forte-structured FP64 sigma on outer-loop indices "
+                                                "i % 16 == 0, extrapolated to one sigma, times the GPU "
+                                                "solve's sigma count (Davidson-Liu and RDM time not included)")
+            print(f"# {key} cpu: {d['cpu_baseline']}", file=sys.stderr, flush=True)
+        out[key] = d
+        del s
+    return out
 
 
 def main():
@@ -114,6 +202,8 @@ def main():
                     help="also solve this many seeded small problems (n in {6,8,10,12}, 2 roots, FP64) "
                          "with concurrent solvers; 0 to skip")
     ap.add_argument("--batch-threads", type=int, default=16)
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also solve CAS(8,8) (1 root) and CAS(14,14) (3 roots, RDMs) in FP64")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
@@ -171,7 +261,11 @@ def main():
     ab_flops = 3 * 2 * nd * np_ * np_
     ab_exec = 3 * 2 * nd * np_ * E
     ab_useful = 2 * nd * s_ab * s_ab
-    achieved = ab_flops / (t3[2] * 1e-3) / 1e12
+    achieved = ab_exec / (t3[2] * 1e-3) / 1e12
+    # same-spin term of one spin (H1 folded in): F = 2 N_det (s + oo + vvoo), VALU-bound
+    kk = na
+    oo = kk * (kk - 1) // 2
+    same_flops = 2 * nd * (kk * (n - kk + 1) + oo + oo * ((n - kk + 2) * (n - kk + 1) // 2 - 1))
     del C, S
 
     if args.calib_copy_mb > 0:
@@ -221,10 +315,13 @@ def main():
                      "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": traffic_from_profiles(n),
                      "kernel_ms": round(t3[2], 3),
-                     "algorithmic_flops": ab_flops,
-                     "formulation": "3 x 2 N_det P^2, P = n(n+1)/2 (packed pair GEMM, BF16x3)",
-                     "executed_flops": ab_exec,
-                     "executed_tflops": round(ab_exec / (t3[2] * 1e-3) / 1e12, 2),
+                     "algorithmic_flops": ab_exec,
+                     "formulation": "3 x 2 N_det P E BF16 MFMA flops the kernel executes (P = n(n+1)/2 packed "
+                                    "beta pairs, E = k(n-k) alpha single excitations, BF16x3 = 3 products)",
+                     "packed_pair_flops": ab_flops,
+                     "packed_pair_tflops": round(ab_flops / (t3[2] * 1e-3) / 1e12, 2),
+                     "packed_pair_formulation": "3 x 2 N_det P^2 (dense packed-pair GEMM; counts flops the "
+                                                "kernel never runs, so its fraction is not a roofline)",
                      "useful_flops": ab_useful,
                      "algorithmic_bytes": 3 * nd * 4},
         "sigma_split_ms": {"beta_same_spin_with_transposes": round(t3[0], 3),
@@ -232,6 +329,28 @@ def main():
         "fp64_sigma": {"ms": round(ms64, 3), "GBps": round(9 * nd * 8 / (ms64 * 1e-3) / 1e9, 3),
                        "split_ms": [round(t, 3) for t in t3_64],
                        "host_buffers_pcie_inclusive_ms": None if pcie_ms is None else round(pcie_ms, 1)},
+    }
+
+    # every sigma kernel group against the roofline that bounds it
+    ab64_exec = 2 * nd * np_ * E
+    result["kernel_rooflines"] = {
+        "mixed_alpha_same_spin": dict(ms=round(t3[1], 3), bound="valu-fp32", flops=same_flops,
+                                      peak_tflops=FP32_PEAK_TFLOPS,
+                                      frac=round(same_flops / FP32_PEAK_TFLOPS / 1e9 / t3[1], 4)),
+        "mixed_beta_same_spin_with_transposes": dict(ms=round(t3[0], 3), bound="valu-fp32", flops=same_flops,
+                                                     peak_tflops=FP32_PEAK_TFLOPS,
+                                                     frac=round(same_flops / FP32_PEAK_TFLOPS / 1e9 / t3[0], 4)),
+        "mixed_alpha_beta": dict(ms=round(t3[2], 3), bound="mfma-bf16", flops=ab_exec, peak_tflops=BF16_PEAK_TFLOPS,
+                                 frac=round(ab_exec / BF16_PEAK_TFLOPS / 1e9 / t3[2], 4)),
+        "fp64_alpha_same_spin": dict(ms=round(t3_64[1], 3), bound="valu-fp64", flops=same_flops,
+                                     peak_tflops=FP64_PEAK_TFLOPS,
+                                     frac=round(same_flops / FP64_PEAK_TFLOPS / 1e9 / t3_64[1], 4)),
+        "fp64_beta_same_spin_with_transposes": dict(ms=round(t3_64[0], 3), bound="valu-fp64", flops=same_flops,
+                                                    peak_tflops=FP64_PEAK_TFLOPS,
+                                                    frac=round(same_flops / FP64_PEAK_TFLOPS / 1e9 / t3_64[0], 4)),
+        "fp64_alpha_beta": dict(ms=round(t3_64[2], 3), bound="mfma-fp64", flops=ab64_exec,
+                                peak_tflops=FP64_PEAK_TFLOPS,
+                                frac=round(ab64_exec / FP64_PEAK_TFLOPS / 1e9 / t3_64[2], 4)),
     }
 
     # ---- time to solution ------------------------------------------------------------------------
@@ -242,11 +361,12 @@ def main():
             lp_w = forte_amd.learned.load_weights()
         else:
             legs = [leg for leg in legs if leg[1] != "learned"]
-        tts = {}
-        for prec, pc in legs:
+        ms_mixed, ms_fp64 = ms_per_step, ms64
+
+        def solve_leg(prec, pc, r_conv):
             s = forte_amd.FCISolver(na, nb, 1, 1, ints)
             s.set_e_convergence(1e-9)
-            s.set_r_convergence(1e-4)
+            s.set_r_convergence(r_conv)
             s.set_maxiter(args.maxiter)
             s.set_die_if_not_converged(False)
             s.set_precision(prec)
@@ -259,29 +379,40 @@ def main():
             forte_amd.device_synchronize()
             dt = time.perf_counter() - t0
             st = s.stats()
-            print(f"# solve {prec}+{pc}: E={e:.12f} {dt:.2f}s {st}", file=sys.stderr, flush=True)
-            tts[f"{prec}+{pc}"] = dict(seconds=round(dt, 3), energy=e,
-                                       **{k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
-                                                             "converged", "fell_back")})
-            del s
-        ref = tts["fp64+diagonal"]
-        best = tts.get("mixed+learned", tts["mixed+diagonal"])
-        result["time_to_solution"] = {
-            "runs": tts,
-            "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
-            "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
-            "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
-            "thresholds": {"e_conv": 1e-9, "r_conv": 1e-4, "maxiter": args.maxiter},
-            "learned_weights": forte_amd.learned.WEIGHTS if lp_w is not None else None}
+            print(f"# solve {prec}+{pc} r_conv {r_conv:g}: E={e:.12f} {dt:.2f}s {st}", file=sys.stderr, flush=True)
+            # time outside the sigma builds (Davidson-Liu vector passes, guess, host steps) per iteration
+            n32 = st["sigma_builds"] - st["sigma_builds_fp64"]
+            rest = dt - 1e-3 * (n32 * ms_mixed + st["sigma_builds_fp64"] * ms_fp64)
+            return dict(seconds=round(dt, 3), energy=e,
+                        non_sigma_ms_per_iteration=round(1e3 * rest / max(1, st["iterations"]), 2),
+                        **{k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
+                                              "converged", "fell_back")})
+
+        result["time_to_solution"] = {}
+        for r_conv in (1e-4, 1e-6):
+            tts = {f"{prec}+{pc}": solve_leg(prec, pc, r_conv) for prec, pc in legs}
+            ref = tts["fp64+diagonal"]
+            best = tts.get("mixed+learned", tts["mixed+diagonal"])
+            result["time_to_solution"][f"r_conv_{r_conv:g}"] = {
+                "runs": tts,
+                "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
+                "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
+                "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
+                "thresholds": {"e_conv": 1e-9, "r_conv": r_conv, "maxiter": args.maxiter},
+                "learned_weights": os.path.relpath(forte_amd.learned.WEIGHTS, ROOT) if lp_w is not None else None}
+
+    # ---- the other configurations: CAS(8,8) and CAS(14,14) FP64 solves ---------------------------
+    if args.configs:
+        result["configs"] = other_configs(forte_amd, args.cpu)
 
     # ---- batched small problems -----------------------------------------------------------------
     if args.batch > 0:
-        from forte_amd.batch import seeded_batch, solve_batch
+        from forte_amd.batch import seeded_batch, solve_batch, solve_batch_threads
         probs = seeded_batch(args.batch)
         bopt = {"E_CONVERGENCE": 1e-9, "R_CONVERGENCE": 1e-5}
-        solve_batch(probs[:8], nroot=2, threads=args.batch_threads, options=bopt)  # warm-up
+        solve_batch(probs[:8], nroot=2, options=bopt)  # warm-up (code objects, allocator)
         forte_amd.device_synchronize()
-        res, wall = solve_batch(probs, nroot=2, threads=args.batch_threads, options=bopt)
+        res, wall, bst = solve_batch(probs, nroot=2, options=bopt)
         forte_amd.device_synchronize()
         by_n = {}
         for r in res:
@@ -290,11 +421,23 @@ def main():
             d["iterations"] += r["iterations"]
         result["batched"] = {"workload": f"{args.batch} seeded CAS(n,n), n in {{6,8,10,12}}, 2 roots, FP64, "
                                          f"e_conv 1e-9, r_conv 1e-5",
+                             "schedule": "one batched call: problems of one size advance together, one captured "
+                                         "graph replay per Davidson-Liu iteration, autotuned chunk and sigma "
+                                         "sub-batch per size",
                              "problems_per_s": round(len(res) / wall, 2), "seconds": round(wall, 3),
-                             "threads": args.batch_threads, "all_converged": all(r["converged"] for r in res),
-                             "per_n": by_n}
+                             "all_converged": all(r["converged"] for r in res), "per_n": by_n,
+                             "setup_seconds": round(bst["setup_seconds"], 3),
+                             "iterate_seconds": round(bst["iterate_seconds"], 3), "classes": bst["classes"]}
         print(f"# batch: {len(res)} problems in {wall:.2f}s ({len(res) / wall:.1f}/s), "
-              f"converged {result['batched']['all_converged']}", file=sys.stderr, flush=True)
+              f"converged {result['batched']['all_converged']} {bst}", file=sys.stderr, flush=True)
+        if args.batch_threads > 0:
+            rt, wt = solve_batch_threads(probs, nroot=2, threads=args.batch_threads, options=bopt)
+            forte_amd.device_synchronize()
+            de = max(abs(a - b) for r1, r2 in zip(res, rt) for a, b in zip(r1["energies"], r2["energies"]))
+            result["batched"]["threaded_schedule"] = {
+                "problems_per_s": round(len(rt) / wt, 2), "seconds": round(wt, 3), "threads": args.batch_threads,
+                "all_converged": all(r["converged"] for r in rt), "max_abs_dE_vs_batched": de}
+            print(f"# threaded batch: {wt:.2f}s, max |dE| vs batched {de:.2e}", file=sys.stderr, flush=True)
         if args.cpu:
             result["batched"]["cpu_baseline"] = batch_cpu_baseline(probs, res)
             print(f"# batch cpu baseline {result['batched']['cpu_baseline']}", file=sys.stderr, flush=True)

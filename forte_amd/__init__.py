@@ -47,6 +47,18 @@ def make_active_space_integrals(h, eri, scalar_energy=0.0, frozen_core_energy=0.
                                 nuclear_repulsion_energy)
 
 
+def fci_problem_from_fcidump(fname, frozen_docc=None, restricted_docc=None, active=None, nel=None,
+                             multiplicity=None, ms=None, root_sym=None):
+    """The active-space FCI problem of an FCIDUMP file under Forte's MO-space options (point-group
+    irreps, FROZEN_DOCC / RESTRICTED_DOCC folded into the core, ACTIVE orbitals per irrep):
+    dict(ints, na, nb, multiplicity, irrep). See forte_amd.active_space."""
+    from .active_space import active_space_from_fcidump
+    a = active_space_from_fcidump(fname, frozen_docc, restricted_docc, active, nel, multiplicity, ms, root_sym)
+    ints = make_active_space_integrals(a["h"], a["eri"], scalar_energy=a["e_core"])
+    ints.set_active_mo_symmetry(a["orbsym"])
+    return dict(ints=ints, na=a["na"], nb=a["nb"], multiplicity=a["multiplicity"], irrep=a["irrep"])
+
+
 def integrals_from_fcidump(fname):
     """(ActiveSpaceIntegrals, na, nb, multiplicity) for an FCIDUMP file (C1, all orbitals active)."""
     d = read_fcidump(fname)
@@ -78,5 +90,6 @@ __all__ = [
     "ActiveSpaceIntegrals", "FCISolver", "DavidsonLiuSolver", "RDMs", "RDMsType",
     "seeded_integrals", "DeviceArray", "device_synchronize", "make_active_space_integrals", "integrals_from_fcidump",
     "make_active_space_method", "read_fcidump", "chem_to_forte", "LIBRARY_PATH", "lp_forward", "lp_loss_grad",
+    "fci_problem_from_fcidump",
     "lp_train", "LP_NPARAM",
 ]

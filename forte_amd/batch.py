@@ -1,10 +1,14 @@
 """Many independent small FCI problems on one GPU (the batched workload: 512 seeded problems with
 n drawn from {6, 8, 10, 12}, two roots each, FP64).
 
-Each problem is a full `FCISolver` (Forte's `FCISolver::compute_energy`,
-forte/fci/fci_solver.cc:177-314). The library is built with per-thread default streams, so
-solvers driven from different host threads run on different HIP streams and their kernels
-overlap on the device; `compute_energy` releases the GIL. Results come back in input order.
+`solve_batch` hands all problems to the library at once (fci_batch_solve): problems of one shape
+share string tables and advance together through Davidson-Liu iterations that run entirely on the
+device, one captured graph replay per iteration (Forte's repeated FCISolver::compute_energy,
+forte/fci/fci_solver.cc:177-314, as in mcscf/mcscf_2step.cc:187-198).
+
+`solve_batch_threads` is the other schedule: one full `FCISolver` per problem on concurrent host
+threads (the library uses per-thread default streams, so their kernels overlap on the device).
+Results come back in input order.
 """
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -41,7 +45,33 @@ def _solve_one(n, seed, nroot, options):
                 converged=st["converged"])
 
 
-def solve_batch(problems, nroot=2, threads=16, options=None):
+def solve_batch(problems, nroot=2, options=None, want_evecs=False):
+    """Solve [(n, seed)] seeded CAS(n, n) problems (Ms = 0, singlets) in one batched call.
+    Returns (results in input order, wall seconds, batch statistics)."""
+    from . import make_active_space_integrals
+    t0 = time.perf_counter()
+    ints, na, nb = [], [], []
+    for n, seed in problems:
+        h, eri = _C.seeded_integrals(n, seed)
+        ints.append(make_active_space_integrals(h, eri))
+        na.append(n // 2)
+        nb.append(n // 2)
+    opts = {"DL_MAXITER": 600}
+    opts.update(options or {})
+    out = _C.batch_solve(ints, na, nb, nroot, 1, opts, want_evecs)
+    wall = time.perf_counter() - t0
+    res = []
+    for i, (n, seed) in enumerate(problems):
+        e = [float(x) for x in out["energies"][i]]
+        r = dict(n=n, seed=seed, energies=e, energy=e[0], iterations=out["iterations"][i],
+                 converged=out["converged"][i])
+        if want_evecs:
+            r["evecs"] = out["evecs"][i]
+        res.append(r)
+    return res, wall, out["stats"]
+
+
+def solve_batch_threads(problems, nroot=2, threads=16, options=None):
     """Solve [(n, seed)] seeded CAS(n, n) problems (Ms = 0, singlets) with `threads` concurrent
     solvers. Returns (results in input order, wall seconds)."""
     t0 = time.perf_counter()

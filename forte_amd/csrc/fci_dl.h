@@ -13,6 +13,16 @@ using SparseVec = std::vector<std::pair<int64_t, double>>;
 // Symmetric eigensolver (cyclic Jacobi), ascending; V column-major: eigenvector k at V[k*n..].
 void jacobi_eigh(int n, const std::vector<double>& A, std::vector<double>& evals, std::vector<double>& V);
 
+// FCISolver's determinant guess (fci_solver_initial_guess.cc:88-101, sparse_initial_guess.cc:71-264)
+// from a device H diagonal: {guess vectors, other-multiplicity roots to project out}.
+// d_hdiag: the diagonal used for the selection (determinants that may not be chosen hold +inf);
+// nsel: how many determinants may be chosen
+std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_dets(const double* d_hdiag, int64_t ndet,
+                                                                             int64_t nsel, int64_t nbs, int n, int na,
+                                                                             int nb, const HostInts& hi,
+                                                                             int multiplicity, int ndets_per_guess,
+                                                                             int num_guess_states, hipStream_t st);
+
 class DavidsonLiu {
   public:
     // sigma(b, sigma, nvec): nvec consecutive vectors of length size (device pointers)
@@ -35,6 +45,8 @@ class DavidsonLiu {
     using Sigma32Fn = std::function<void(const double* b, float* sigma)>;
     void add_sigma_builder32(Sigma32Fn f);
     void set_preconditioner(PrecondFn f) { precond_ = std::move(f); }
+    // determinants allowed to be nonzero (point-group sector); random vectors are zeroed elsewhere
+    void set_mask(const uint8_t* d_mask) { mask_ = d_mask; }
     // called after the residual norms of an iteration are known; returning true ends the solve
     // as converged (used to end a reduced-precision phase once it stalls)
     using IterFn = std::function<bool(int iter, const std::vector<double>& resid, const std::vector<double>& lambda)>;
@@ -72,6 +84,7 @@ class DavidsonLiu {
     DBuf<float> sigma32_v_;
     PrecondFn precond_;
     IterFn observe_;
+    const uint8_t* mask_ = nullptr;
     bool have_hdiag_ = false;
     DBuf<double> hdiag_, b_, sigma_v_, r_, temp_;
     DBuf<double> part_;  // reduction partials

@@ -738,10 +738,16 @@ void DavidsonLiu::project_out(double* v, int nvec) {
     HIPCHK(hipStreamSynchronize(0));
 }
 
+__global__ void k_apply_mask(double* __restrict__ v, const uint8_t* __restrict__ m, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && !m[i]) v[i] = 0.0;
+}
+
 void DavidsonLiu::fill_random(double* v, int nvec) {
     for (int k = 0; k < nvec; ++k) {
         rng_ += 0x9E3779B97F4A7C15ULL;
         k_random<<<nblk(size_, 256), 256>>>(v + size_t(k) * size_, size_, rng_);
+        if (mask_) k_apply_mask<<<nblk(size_, 256), 256>>>(v + size_t(k) * size_, mask_, size_);
     }
     HIPCHK(hipGetLastError());
 }

@@ -105,6 +105,10 @@ struct SpinTables {
 };
 
 void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* d_binom, hipStream_t st);
+// point-group classes of the strings: irrep (XOR of the occupied orbitals' irreps, orbsym in
+// [0, 8)) and address within that irrep in Forte's order; count[h] = strings of irrep h
+void string_classes_device(const SpinTables& t, const std::vector<int>& orbsym, uint8_t* d_irrep, uint32_t* d_rank,
+                           std::vector<int64_t>& count, hipStream_t st);
 
 // Forte-order lists built on the device (parity checks): returns count, fills host arrays.
 int64_t forte_vo_list(const SpinTables& t, const uint32_t* d_binom, int p, int q,
@@ -142,6 +146,25 @@ struct DevInts {
     int gdim[2] = {0, 0};
 };
 
+// Host copy of the integrals in Forte's layout (the guess and packing work on the host).
+struct HostInts {
+    int n = 0;
+    std::vector<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;
+    double e_core = 0.0;
+    size_t i4(int p, int q, int r, int s) const { return ((size_t(p) * n + q) * n + r) * n + s; }
+};
+
+// Integrals packed for the sigma kernels: vpk[P * np + Q] = (pq|rs) over packed pairs
+// (np = n(n+1)/2), and per spin the same-spin clique operator g' (gd x gd, one-body term folded
+// in). Throws if tei_ab lacks the permutational symmetry of real orbitals.
+struct PackedInts {
+    int np = 0;
+    std::vector<double> vpk;
+    std::vector<double> g2[2];
+    int gd[2] = {0, 0};
+};
+PackedInts pack_integrals(const HostInts& hi, const SpinTables& sa, const SpinTables& sb);
+
 // ---- sigma engine -----------------------------------------------------------------------------
 enum class Prec { FP64 = 0, MIXED = 1 };
 
@@ -174,6 +197,31 @@ class SigmaEngine {
     double ms_[3] = {0, 0, 0};
 };
 
+// ---- batched FP64 sigma ------------------------------------------------------------------------
+// Many problems of one size (shared string tables, own integrals) in one launch per term; item i
+// of a launch reads C = c[i] and writes sigma = s[i], and is idle when act[i] is null (act holds
+// the same pointers as c). Each item has its own scratch: ct (C^T), yt (Y^T) and e (same-spin
+// intermediate), sized by batch_sigma_scratch_elems. No e_core term (the caller adds it).
+struct BatchSigmaTabs {
+    int nitem = 0;
+    const void* const* act = nullptr;
+    const double* const* c = nullptr;
+    double* const* s = nullptr;
+    double* const* ct = nullptr;
+    double* const* yt = nullptr;
+    double* const* e = nullptr;
+    const double* const* g2a = nullptr;
+    const double* const* g2b = nullptr;
+    const double* const* vpk = nullptr;
+};
+bool batch_sigma_supported(const SpinTables& a, const SpinTables& b, std::string* why);
+// elements of scratch per item; *off_yt and *off_e: offsets of Y^T and E inside it (C^T at 0)
+size_t batch_sigma_scratch_elems(const SpinTables& a, const SpinTables& b, size_t* off_yt, size_t* off_e);
+// sets the kernel attributes batch_sigma_f64 needs (call before capturing it in a graph)
+void batch_sigma_prepare(const SpinTables& a, const SpinTables& b);
+void batch_sigma_f64(const SpinTables& a, const SpinTables& b, int gd_a, int gd_b, const BatchSigmaTabs& t,
+                     hipStream_t st);
+
 // per-string diagonal of the same-spin clique operator: out[I] = sum over the d-subsets P of
 // occ(I) of g[P][P] (d = 2: pairs a > b, index a(a-1)/2 + b; d = 1: orbitals)
 void clique_diagonal_device(const SpinTables& sp, const double* d_g, int gdim, float* d_out, hipStream_t st);
@@ -194,5 +242,10 @@ void rdm2_ab_device(const SpinTables& a, const SpinTables& b, const double* L, c
                     double* g2ab, hipStream_t st);
 double spin2_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,
                     hipStream_t st);
+// compact 3-RDM blocks, rows of L/R = strings of the first spin (na1 electrons), columns = strings
+// of the second spin (na2): c3 [C(n,3)][C(n,3)] over sorted triples of the first spin, c21
+// [C(n,2) n][C(n,2) n] over (sorted pair of the first spin, orbital of the second); NULL to skip
+void rdm3_compact_device(int n, int na1, int na2, const double* L, const double* R, int64_t nrow, int64_t ncol,
+                         double* c3, double* c21, hipStream_t st);
 
 }  // namespace fci

@@ -293,6 +293,177 @@ __global__ __launch_bounds__(RD_BS) void k_spin2(const double* __restrict__ L, c
     }
 }
 
+// ---- 3-RDMs through hole strings ----------------------------------------------------------------
+// D[x][y] = sum_c XL[x][c] XR[y][c] per hole string K, accumulated into a compact output.
+// MODE 0 (three electrons of one spin): x = member triple T of K (rows I = K + T with the sign of
+//   a+T0 a+T1 a+T2 |K>), c = column (the other spin's string), out index = rank of T.
+// MODE 1 (two electrons of the first spin, one of the second): x = (member pair P of K, orbital r),
+//   c = (k2-1)-electron string Lb of the second spin, XL[x][c] = s_P s_{Lb,r} L[K + P][Lb + r]
+//   (zero when r is in Lb), out index = rank(P) * n + r.
+constexpr int R3_T = 64, R3_C = 32;
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rdm3_hole(const double* __restrict__ L, const double* __restrict__ R,
+                                                   int64_t ncol, const uint32_t* __restrict__ mem,
+                                                   const uint32_t* __restrict__ midx, int M,
+                                                   const uint32_t* __restrict__ tab1, int64_t nLb, int n,
+                                                   int nxt, double* __restrict__ out, int64_t ldo) {
+    __shared__ double XL[R3_T][R3_C + 1], XR[R3_T][R3_C + 1];
+    const int64_t g = blockIdx.x;
+    const int xt = blockIdx.y % nxt, yt = blockIdx.y / nxt;
+    const int X = MODE == 0 ? M : M * n;
+    const int64_t nc = MODE == 0 ? ncol : nLb;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const uint32_t* gm = mem + g * M;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    // the element (x, c) of the gathered operand V (L or R)
+    auto elem = [&](const double* V, int x, int64_t c) -> double {
+        if (x >= X || c >= nc) return 0.0;
+        const int m = MODE == 0 ? x : x / n;
+        const uint32_t e = gm[m];
+        const double sm = (e >> 31) ? -1.0 : 1.0;
+        const int64_t row = int64_t(e & 0x7FFFFFFFu);
+        if (MODE == 0) return sm * V[row * ncol + c];
+        const uint32_t t = tab1[c * n + (x % n)];
+        if (t == kInvalid) return 0.0;
+        return ((t >> 31) ? -sm : sm) * V[row * ncol + int64_t(t & 0x7FFFFFFFu)];
+    };
+    for (int64_t c0 = 0; c0 < nc; c0 += R3_C) {
+        for (int i = tid; i < R3_T * R3_C; i += 256) {
+            const int xr = i / R3_C, cc = i % R3_C;
+            XL[xr][cc] = elem(L, xt * R3_T + xr, c0 + cc);
+            XR[xr][cc] = elem(R, yt * R3_T + xr, c0 + cc);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int cc = 0; cc < R3_C; ++cc) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = XL[tx * 4 + i][cc];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = XR[ty * 4 + j][cc];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = xt * R3_T + tx * 4 + i;
+        if (x >= X) continue;
+        const int64_t ox = MODE == 0 ? midx[g * M + x] : int64_t(midx[g * M + x / n]) * n + x % n;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int y = yt * R3_T + ty * 4 + j;
+            if (y >= X) continue;
+            const int64_t oy = MODE == 0 ? midx[g * M + y] : int64_t(midx[g * M + y / n]) * n + y % n;
+            atomicAdd(&out[ox * ldo + oy], acc[i][j]);
+        }
+    }
+}
+
+namespace {
+// (-1)^{#bits of s below orbital p}
+inline int sign_below_h(uint32_t s, int p) { return (__builtin_popcount(s & ((1u << p) - 1u)) & 1) ? -1 : 1; }
+inline int64_t rank_h(uint32_t s, int n) {
+    int64_t r = 0;
+    int t = 0;
+    for (int i = n - 1; i >= 0; --i)
+        if ((s >> i) & 1u) {
+            ++t;
+            r += int64_t(binom_host(n - 1 - i, t));
+        }
+    return r;
+}
+// all k-subsets of n orbitals as bit strings
+std::vector<uint32_t> subsets(int n, int k) {
+    std::vector<uint32_t> out;
+    if (k < 0 || k > n) return out;
+    for (uint32_t s = 0; s < (1u << n); ++s)
+        if (__builtin_popcount(s) == k) out.push_back(s);
+    return out;
+}
+// hole-string members: for each (k - d)-string K and each d-subset D of vir(K): entry
+// addr(K + D) | (sign of a+D0 ... a+D(d-1) |K> < 0) << 31, and the colex rank of D (0-based)
+void hole_members(int n, int k, int d, std::vector<uint32_t>& mem, std::vector<uint32_t>& idx, int& M, int64_t& nK) {
+    mem.clear();
+    idx.clear();
+    auto Ks = subsets(n, k - d);
+    nK = int64_t(Ks.size());
+    M = int(binom_host(n - (k - d), d));
+    for (uint32_t K : Ks) {
+        int cnt = 0;
+        for (uint32_t D : subsets(n, d)) {
+            if (D & K) continue;
+            int sg = 1;
+            int64_t rk = 0;
+            int t = 0;
+            for (int p = 0; p < n; ++p)
+                if ((D >> p) & 1u) {
+                    sg *= sign_below_h(K, p);
+                    ++t;
+                    rk += int64_t(binom_host(p, t));
+                }
+            mem.push_back(uint32_t(rank_h(K | D, n)) | (sg < 0 ? 0x80000000u : 0u));
+            idx.push_back(uint32_t(rk));
+            ++cnt;
+        }
+        if (cnt != M) throw Error(-1, "rdm3: member count mismatch");
+    }
+}
+}  // namespace
+
+// Compact 3-RDM blocks of one orientation (rows of L/R are strings of the first spin, na1
+// electrons; columns strings of the second spin, na2 electrons):
+//   c3 (C(n,3)^2): <L| a+ a+ a+ a a a |R> of the first spin, sorted triples;
+//   c21 ((C(n,2) n)^2): <L| a+P a+P' a+r(2) a_u(2) a_S' a_S |R>, sorted pairs of the first spin.
+void rdm3_compact_device(int n, int na1, int na2, const double* L, const double* R, int64_t nrow, int64_t ncol,
+                         double* c3, double* c21, hipStream_t st) {
+    const int64_t n3 = int64_t(binom_host(n, 3)), n2p = int64_t(binom_host(n, 2)) * n;
+    if (c3) HIPCHK(hipMemsetAsync(c3, 0, size_t(n3 * n3) * sizeof(double), st));
+    if (c21) HIPCHK(hipMemsetAsync(c21, 0, size_t(n2p * n2p) * sizeof(double), st));
+    std::vector<uint32_t> mem, idx;
+    int M = 0;
+    int64_t nK = 0;
+    if (c3 && na1 >= 3) {
+        hole_members(n, na1, 3, mem, idx, M, nK);
+        DBuf<uint32_t> dm(mem.size()), di(idx.size());
+        dm.upload(mem.data(), mem.size());
+        di.upload(idx.data(), idx.size());
+        const int nxt = (M + R3_T - 1) / R3_T;
+        k_rdm3_hole<0><<<dim3(unsigned(nK), unsigned(nxt * nxt)), 256, 0, st>>>(L, R, ncol, dm.p, di.p, M, nullptr, 0,
+                                                                               n, nxt, c3, n3);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    if (c21 && na1 >= 2 && na2 >= 1) {
+        hole_members(n, na1, 2, mem, idx, M, nK);
+        DBuf<uint32_t> dm(mem.size()), di(idx.size());
+        dm.upload(mem.data(), mem.size());
+        di.upload(idx.data(), idx.size());
+        // second spin: (na2 - 1)-strings Lb and orbital r -> addr(Lb + r) | sign << 31
+        auto Lbs = subsets(n, na2 - 1);
+        std::vector<uint32_t> t1(Lbs.size() * size_t(n), kInvalid);
+        for (size_t i = 0; i < Lbs.size(); ++i)
+            for (int r = 0; r < n; ++r)
+                if (!((Lbs[i] >> r) & 1u))
+                    t1[i * n + r] = uint32_t(rank_h(Lbs[i] | (1u << r), n)) | (sign_below_h(Lbs[i], r) < 0 ? 0x80000000u : 0u);
+        DBuf<uint32_t> dt(t1.size());
+        dt.upload(t1.data(), t1.size());
+        const int nxt = (M * n + R3_T - 1) / R3_T;
+        k_rdm3_hole<1><<<dim3(unsigned(nK), unsigned(nxt * nxt)), 256, 0, st>>>(
+            L, R, ncol, dm.p, di.p, M, dt.p, int64_t(Lbs.size()), n, nxt, c21, n2p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    (void)nrow;
+}
+
 // ---- launchers -------------------------------------------------------------------------------
 void rdm1_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R, double* g1a,
                  double* g1b, hipStream_t st) {

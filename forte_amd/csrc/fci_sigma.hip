@@ -76,11 +76,23 @@ __global__ void k_transpose(const T* __restrict__ in, T* __restrict__ out, int64
 template <class T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
+// Batched launches (many problems of one size at once) pass per-item pointer tables indexed by
+// blockIdx.y; an item whose `act` entry is null is idle. Null tables: the plain pointers are used.
 template <class T>
 __global__ __launch_bounds__(256) void k_transpose64(const T* __restrict__ in, T* __restrict__ out, int64_t R,
-                                                     int64_t Cc, const T* __restrict__ base, T bs) {
+                                                     int64_t Cc, const T* __restrict__ base, T bs,
+                                                     const void* const* __restrict__ act = nullptr,
+                                                     const T* const* __restrict__ in_t = nullptr,
+                                                     T* const* __restrict__ out_t = nullptr,
+                                                     const T* const* __restrict__ base_t = nullptr) {
     using V2 = typename Pair<T>::type;
     __shared__ T tile[64][65];
+    if (act) {
+        if (!act[blockIdx.z]) return;
+        in = in_t[blockIdx.z];
+        out = out_t[blockIdx.z];
+        if (base_t) base = base_t[blockIdx.z];
+    }
     const int64_t c0 = int64_t(blockIdx.x) * 64, r0 = int64_t(blockIdx.y) * 64;
     const int cp = threadIdx.x & 31, rr = threadIdx.x >> 5;  // column pair, row in a group of 8
     for (int j = rr; j < 64; j += 8) {
@@ -845,8 +857,18 @@ template <int NK4>
 __global__ __launch_bounds__(512) void k_clique3p_f64(const double* __restrict__ X, double* __restrict__ E, int64_t L,
                                                       const uint32_t* __restrict__ clq, int M, int64_t nK,
                                                       const double* __restrict__ G, int gdim, int nchunk,
-                                                      int64_t cbeg, int64_t Ls) {
+                                                      int64_t cbeg, int64_t Ls,
+                                                      const void* const* __restrict__ act = nullptr,
+                                                      const double* const* __restrict__ X_t = nullptr,
+                                                      double* const* __restrict__ E_t = nullptr,
+                                                      const double* const* __restrict__ G_t = nullptr) {
     constexpr int NMT = (NK4 + 3) / 4;
+    if (act) {
+        if (!act[blockIdx.y]) return;
+        X = X_t[blockIdx.y];
+        E = E_t[blockIdx.y];
+        G = G_t[blockIdx.y];
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
     const int64_t nwork = nK * nchunk;
@@ -932,7 +954,15 @@ __global__ __launch_bounds__(256) void k_clique_gather(const T* __restrict__ E, 
                                                        int64_t L, const uint32_t* __restrict__ own, int nown,
                                                        int64_t cbeg, int64_t ncol, int64_t Ls,
                                                        const T* __restrict__ X, const float* __restrict__ gdiag,
-                                                       int64_t nrowE) {
+                                                       int64_t nrowE,
+                                                       const void* const* __restrict__ act = nullptr,
+                                                       const T* const* __restrict__ E_t = nullptr,
+                                                       T* const* __restrict__ Y_t = nullptr) {
+    if (act) {  // batched (FP64 only: no diagonal term)
+        if (!act[blockIdx.y]) return;
+        E = E_t[blockIdx.y];
+        Y = Y_t[blockIdx.y];
+    }
     // block: 4 rows x 64 columns
     const int64_t nct = (ncol + 63) / 64;
     const int64_t nwork = ((nrow + 3) / 4) * nct;
@@ -1483,8 +1513,18 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
                                                     int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
                                                     const uint32_t* __restrict__ str_a,
                                                     const uint16_t* __restrict__ outT, int npp, int np,
-                                                    const double* __restrict__ vpk, int nmt) {
+                                                    const double* __restrict__ vpk, int nmt,
+                                                    const void* const* __restrict__ act = nullptr,
+                                                    const double* const* __restrict__ C_t = nullptr,
+                                                    double* const* __restrict__ S_t = nullptr,
+                                                    const double* const* __restrict__ V_t = nullptr) {
     extern __shared__ __attribute__((aligned(16))) double srow64[];  // [Nb]
+    if (act) {
+        if (!act[blockIdx.y]) return;
+        C = C_t[blockIdx.y];
+        S = S_t[blockIdx.y];
+        vpk = V_t[blockIdx.y];
+    }
     __shared__ uint32_t ent[KS * 4];
     __shared__ double vdg[12 * 16];
     constexpr int WM = 4, WC = 2, NTH = 64 * WM * WC;
@@ -1910,7 +1950,7 @@ static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, cons
     auto kern = k_ab2_f64<KS, TPW, SPLIT>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
-                                            ints.vpk.p, (b.np + 15) / 16);
+                                            ints.vpk.p, (b.np + 15) / 16, nullptr, nullptr, nullptr, nullptr);
     HIPCHK(hipGetLastError());
 }
 template <int KS, int TPW, int WM, int WC>
@@ -2031,6 +2071,124 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
         case 5: ab_mixed_launch<5>(C, S, a, b, ints, st); break;
         default: throw Error(-1, "alpha-beta kernel: too many single excitations");
     }
+}
+
+// ---- batched FP64 sigma (many problems of one size, one launch per term) ------------------------
+// Supported: both spins with >= 2 electrons, same-spin cliques of <= 64 members with <= 16 k-steps,
+// even string counts (paired-column kernels) and the register-resident alpha-beta kernel.
+bool batch_sigma_supported(const SpinTables& a, const SpinTables& b, std::string* why) {
+    auto no = [&](const char* m) {
+        if (why) *why = m;
+        return false;
+    };
+    if (a.d != 2 || b.d != 2) return no("batched sigma needs at least 2 electrons of each spin");
+    if (a.M > 64 || b.M > 64 || (a.M + 3) / 4 > 16 || (b.M + 3) / 4 > 16)
+        return no("batched sigma: same-spin cliques too large");
+    if (a.nstr % 2 || b.nstr % 2) return no("batched sigma needs even string counts");
+    if (!ab2_ok(a, b, sizeof(double))) return no("batched sigma: alpha-beta shape not supported");
+    const int ks = std::max(1, (a.ns + 15) / 16) * 4;
+    const int tpw = ((b.np + 15) / 16 + 3) / 4;
+    if (ks > 24 || tpw > 3) return no("batched sigma: alpha-beta shape not supported");
+    return true;
+}
+
+static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+size_t batch_sigma_scratch_elems(const SpinTables& a, const SpinTables& b, size_t* off_yt, size_t* off_e) {
+    const size_t nd = size_t(a.nstr) * size_t(b.nstr);
+    const size_t v = (nd + 31) & ~size_t(31);  // 256-byte aligned in FP64
+    const size_t ea = size_t(a.nK) * a.M * size_t(round_up(b.nstr, CQ3_CW));
+    const size_t eb = size_t(b.nK) * b.M * size_t(round_up(a.nstr, CQ3_CW));
+    if (off_yt) *off_yt = v;
+    if (off_e) *off_e = 2 * v;
+    return 2 * v + std::max(ea, eb);
+}
+
+static void batch_clique_f64(const SpinTables& sp, int gd, int64_t L, const void* const* act,
+                             const double* const* X_t, double* const* E_t, const double* const* G_t,
+                             double* const* Y_t, bool acc, int nitem, hipStream_t st) {
+    const int64_t Ls = round_up(L, CQ3_CW);
+    const int nchunk = int(Ls / CQ3_CW);
+    const int64_t nblk3 = (sp.nK * nchunk + 7) / 8;
+    const dim3 g1(xcd_grid(nblk3), unsigned(nitem));
+    switch ((sp.M + 3) / 4) {
+#define C3P(K)                                                                                              \
+    case K:                                                                                                 \
+        k_clique3p_f64<K><<<g1, 512, 0, st>>>(nullptr, nullptr, L, sp.clq.p, sp.M, sp.nK, nullptr, gd, nchunk, \
+                                              0, Ls, act, X_t, E_t, G_t);                                   \
+        break;
+        C3P(1) C3P(2) C3P(3) C3P(4) C3P(5) C3P(6) C3P(7) C3P(8) C3P(9) C3P(10) C3P(11) C3P(12) C3P(13) C3P(14)
+        C3P(15) C3P(16)
+#undef C3P
+        default: throw Error(-1, "batched clique kernel: no instance for this size");
+    }
+    HIPCHK(hipGetLastError());
+    const int64_t gw = ((sp.nstr + 3) / 4) * ((L + 63) / 64);
+    const dim3 g2(xcd_grid(gw), unsigned(nitem));
+    const double* const* Ec = E_t;
+    if (acc)
+        k_clique_gather<double, true, true><<<g2, 256, 0, st>>>(nullptr, nullptr, sp.nstr, L, sp.own.p, sp.nown, 0, L,
+                                                               Ls, nullptr, nullptr, sp.nK * sp.M, act, Ec, Y_t);
+    else
+        k_clique_gather<double, false, true><<<g2, 256, 0, st>>>(nullptr, nullptr, sp.nstr, L, sp.own.p, sp.nown, 0,
+                                                                L, Ls, nullptr, nullptr, sp.nK * sp.M, act, Ec, Y_t);
+    HIPCHK(hipGetLastError());
+}
+
+template <int KS, int TPW, bool SPLIT>
+static void batch_ab2_f64(const SpinTables& a, const SpinTables& b, const BatchSigmaTabs& t, hipStream_t st) {
+    const size_t sh = size_t(b.nstr) * sizeof(double);  // attribute set by batch_sigma_prepare
+    auto kern = k_ab2_f64<KS, TPW, SPLIT>;
+    kern<<<dim3(xcd_grid(a.nstr), unsigned(t.nitem)), 512, sh, st>>>(
+        nullptr, nullptr, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np, nullptr, (b.np + 15) / 16,
+        t.act, t.c, t.s, t.vpk);
+    HIPCHK(hipGetLastError());
+}
+
+template <int KS, int TPW, bool SPLIT> static void batch_ab2_attr(const SpinTables& b) {
+    HIPCHK(hipFuncSetAttribute((const void*)k_ab2_f64<KS, TPW, SPLIT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(size_t(b.nstr) * sizeof(double))));
+}
+
+void batch_sigma_prepare(const SpinTables& a, const SpinTables& b) {
+    const int ks = std::max(1, (a.ns + 15) / 16) * 4;
+    const int tpw = ((b.np + 15) / 16 + 3) / 4;
+    const bool split = tpw > 1 && (b.np + 15) / 16 == 4 * (tpw - 1) + 1;
+#define F64CASE(K, T) \
+    if (ks == K && tpw == T) return split ? batch_ab2_attr<K, T, (T > 1)>(b) : batch_ab2_attr<K, T, false>(b);
+#define F64ROW(K) F64CASE(K, 1) F64CASE(K, 2) F64CASE(K, 3)
+    F64ROW(4) F64ROW(8) F64ROW(12) F64ROW(16) F64ROW(20) F64ROW(24)
+#undef F64ROW
+#undef F64CASE
+    throw Error(-1, "batched alpha-beta kernel: no instance for this shape");
+}
+
+void batch_sigma_f64(const SpinTables& a, const SpinTables& b, int gd_a, int gd_b, const BatchSigmaTabs& t,
+                     hipStream_t st) {
+    const int64_t Na = a.nstr, Nb = b.nstr;
+    const unsigned ni = unsigned(t.nitem);
+    // beta same-spin on C^T: C -> C^T, cliques -> Y^T, sigma = (Y^T)^T
+    k_transpose64<double><<<dim3(nblk(Nb, 64), nblk(Na, 64), ni), 256, 0, st>>>(nullptr, nullptr, Na, Nb, nullptr,
+                                                                               0.0, t.act, t.c, t.ct, nullptr);
+    HIPCHK(hipGetLastError());
+    batch_clique_f64(b, gd_b, Na, t.act, t.ct, t.e, t.g2b, t.yt, false, t.nitem, st);
+    k_transpose64<double><<<dim3(nblk(Na, 64), nblk(Nb, 64), ni), 256, 0, st>>>(nullptr, nullptr, Nb, Na, nullptr,
+                                                                               0.0, t.act, t.yt, t.s, nullptr);
+    HIPCHK(hipGetLastError());
+    // alpha same-spin on C, accumulated into sigma
+    batch_clique_f64(a, gd_a, Nb, t.act, t.c, t.e, t.g2a, t.s, true, t.nitem, st);
+    // alpha-beta
+    const int ks = std::max(1, (a.ns + 15) / 16) * 4;
+    const int tpw = ((b.np + 15) / 16 + 3) / 4;
+    const bool split = tpw > 1 && (b.np + 15) / 16 == 4 * (tpw - 1) + 1;
+#define F64CASE(K, T)                                                                              \
+    if (ks == K && tpw == T)                                                                        \
+        return split ? batch_ab2_f64<K, T, (T > 1)>(a, b, t, st) : batch_ab2_f64<K, T, false>(a, b, t, st);
+#define F64ROW(K) F64CASE(K, 1) F64CASE(K, 2) F64CASE(K, 3)
+    F64ROW(4) F64ROW(8) F64ROW(12) F64ROW(16) F64ROW(20) F64ROW(24)
+#undef F64ROW
+#undef F64CASE
+    throw Error(-1, "batched alpha-beta kernel: no instance for this shape");
 }
 
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }

@@ -19,12 +19,6 @@
 namespace fci {
 
 // ---- host integrals and determinant algebra (the guess) ---------------------------------------
-struct HostInts {
-    int n = 0;
-    std::vector<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;
-    double e_core = 0.0;
-    size_t i4(int p, int q, int r, int s) const { return ((size_t(p) * n + q) * n + r) * n + s; }
-};
 
 static inline int popc32(uint32_t x) { return __builtin_popcount(x); }
 static inline double sign_between(uint32_t x, int m, int n) {
@@ -258,6 +252,77 @@ static void check_device() {
 }
 void check_device_public() { check_device(); }
 
+PackedInts pack_integrals(const HostInts& hi, const SpinTables& sa, const SpinTables& sb) {
+    const int n = hi.n;
+    const size_t n4 = size_t(n) * n * n * n;
+    const double* tab = hi.tei_ab.data();
+    PackedInts pk;
+    // packed chemist ERIs (pq|rs) = tei_ab[p,r,q,s]; the packed form needs (pq|rs) = (qp|rs)
+    const int np = n * (n + 1) / 2;
+    pk.np = np;
+    pk.vpk.assign(size_t(np) * np, 0.0);
+    double scale = 0.0;
+    for (size_t i = 0; i < n4; ++i) scale = std::max(scale, std::fabs(tab[i]));
+    for (int p = 0; p < n; ++p)
+        for (int q = 0; q <= p; ++q)
+            for (int r = 0; r < n; ++r)
+                for (int s = 0; s <= r; ++s) {
+                    double x = tab[hi.i4(p, r, q, s)];
+                    double y = tab[hi.i4(q, r, p, s)], z = tab[hi.i4(p, s, q, r)];
+                    if (std::fabs(x - y) > 1e-10 * (1.0 + scale) || std::fabs(x - z) > 1e-10 * (1.0 + scale))
+                        throw Error(-1, "tei_ab must have the permutational symmetry of real orbitals");
+                    size_t P = size_t(p) * (p + 1) / 2 + q, Q = size_t(r) * (r + 1) / 2 + s;
+                    pk.vpk[P * np + Q] = x;
+                }
+    // same-spin clique operator per spin
+    for (int spin = 0; spin < 2; ++spin) {
+        const SpinTables& sp = spin ? sb : sa;
+        const std::vector<double>& h = spin ? hi.oei_b : hi.oei_a;
+        const std::vector<double>& t = spin ? hi.tei_bb : hi.tei_aa;
+        std::vector<double>& g = pk.g2[spin];
+        int gd = 0;
+        if (sp.d == 2) {
+            gd = n * (n - 1) / 2;
+            g.assign(size_t(gd) * gd, 0.0);
+            auto pidx = [](int p, int q) { return p > q ? p * (p - 1) / 2 + q : q * (q - 1) / 2 + p; };
+            // two-body: H2_aaaa2 uses tei(p,q,r,s) with pair(pq) >= pair(rs) for both orders
+            for (int p = 0; p < n; ++p)
+                for (int q = 0; q < p; ++q)
+                    for (int r = 0; r < n; ++r)
+                        for (int s = 0; s < r; ++s) {
+                            int P = pidx(p, q), R = pidx(r, s);
+                            g[size_t(P) * gd + R] = (P >= R) ? t[hi.i4(p, q, r, s)] : t[hi.i4(r, s, p, q)];
+                        }
+            // one-body fold: h_pq a+p a_q = 1/(k-1) sum_t h_pq a+p a+t a_t a_q
+            const double f = 1.0 / double(sp.k - 1);
+            for (int p = 0; p < n; ++p)
+                for (int q = 0; q < n; ++q)
+                    for (int tt = 0; tt < n; ++tt) {
+                        if (tt == p || tt == q) continue;
+                        double sg = (p > tt ? 1.0 : -1.0) * (q > tt ? 1.0 : -1.0);
+                        g[size_t(pidx(p, tt)) * gd + pidx(q, tt)] += f * sg * h[p * n + q];
+                    }
+        } else if (sp.d == 1) {
+            gd = n;
+            g.assign(h.begin(), h.end());
+        }
+        pk.gd[spin] = gd;
+    }
+    return pk;
+}
+
+// determinants outside the target irrep: infinite diagonal for the guess selection
+__global__ void k_sector_select(const double* __restrict__ h, const uint8_t* __restrict__ in_sector, int64_t n,
+                                double* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in_sector[i] ? h[i] : 1.7976931348623157e308;
+}
+__global__ void k_sector_mask(const uint8_t* __restrict__ sa, const uint8_t* __restrict__ sb, int64_t na, int64_t nb,
+                              int irrep, uint8_t* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < na * nb) out[i] = uint8_t((sa[i / nb] ^ sb[i % nb]) == irrep);
+}
+
 class Solver {
   public:
     Solver(int n, int na, int nb) : n_(n), na_(na), nb_(nb) {
@@ -293,58 +358,14 @@ class Solver {
         di_.tei_aa.alloc(n4); di_.tei_aa.upload(taa, n4);
         di_.tei_ab.alloc(n4); di_.tei_ab.upload(tab, n4);
         di_.tei_bb.alloc(n4); di_.tei_bb.upload(tbb, n4);
-        // packed chemist ERIs (pq|rs) = tei_ab[p,r,q,s]; the packed form needs (pq|rs) = (qp|rs)
-        const int np = n * (n + 1) / 2;
-        std::vector<double> v(size_t(np) * np);
-        std::vector<float> v32(size_t(np) * np);
-        double scale = 0.0;
-        for (size_t i = 0; i < n4; ++i) scale = std::max(scale, std::fabs(tab[i]));
-        for (int p = 0; p < n; ++p)
-            for (int q = 0; q <= p; ++q)
-                for (int r = 0; r < n; ++r)
-                    for (int s = 0; s <= r; ++s) {
-                        double x = tab[hi_.i4(p, r, q, s)];
-                        double y = tab[hi_.i4(q, r, p, s)], z = tab[hi_.i4(p, s, q, r)];
-                        if (std::fabs(x - y) > 1e-10 * (1.0 + scale) || std::fabs(x - z) > 1e-10 * (1.0 + scale))
-                            throw Error(-1, "tei_ab must have the permutational symmetry of real orbitals");
-                        size_t P = size_t(p) * (p + 1) / 2 + q, Q = size_t(r) * (r + 1) / 2 + s;
-                        v[P * np + Q] = x;
-                        v32[P * np + Q] = float(x);
-                    }
-        di_.vpk.alloc(v.size()); di_.vpk.upload(v.data(), v.size());
+        const PackedInts pk = pack_integrals(hi_, sa_, sb_);
+        std::vector<float> v32(pk.vpk.begin(), pk.vpk.end());
+        di_.vpk.alloc(pk.vpk.size()); di_.vpk.upload(pk.vpk.data(), pk.vpk.size());
         di_.vpk32.alloc(v32.size()); di_.vpk32.upload(v32.data(), v32.size());
-        // same-spin clique operator per spin
         for (int spin = 0; spin < 2; ++spin) {
             const SpinTables& sp = spin ? sb_ : sa_;
-            const std::vector<double>& h = spin ? hi_.oei_b : hi_.oei_a;
-            const std::vector<double>& t = spin ? hi_.tei_bb : hi_.tei_aa;
-            std::vector<double> g;
-            int gd = 0;
-            if (sp.d == 2) {
-                gd = n * (n - 1) / 2;
-                g.assign(size_t(gd) * gd, 0.0);
-                auto pidx = [](int p, int q) { return p > q ? p * (p - 1) / 2 + q : q * (q - 1) / 2 + p; };
-                // two-body: H2_aaaa2 uses tei(p,q,r,s) with pair(pq) >= pair(rs) for both orders
-                for (int p = 0; p < n; ++p)
-                    for (int q = 0; q < p; ++q)
-                        for (int r = 0; r < n; ++r)
-                            for (int s = 0; s < r; ++s) {
-                                int P = pidx(p, q), R = pidx(r, s);
-                                g[size_t(P) * gd + R] = (P >= R) ? t[hi_.i4(p, q, r, s)] : t[hi_.i4(r, s, p, q)];
-                            }
-                // one-body fold: h_pq a+p a_q = 1/(k-1) sum_t h_pq a+p a+t a_t a_q
-                const double f = 1.0 / double(sp.k - 1);
-                for (int p = 0; p < n; ++p)
-                    for (int q = 0; q < n; ++q)
-                        for (int tt = 0; tt < n; ++tt) {
-                            if (tt == p || tt == q) continue;
-                            double sg = (p > tt ? 1.0 : -1.0) * (q > tt ? 1.0 : -1.0);
-                            g[size_t(pidx(p, tt)) * gd + pidx(q, tt)] += f * sg * h[p * n + q];
-                        }
-            } else if (sp.d == 1) {
-                gd = n;
-                g.assign(h.begin(), h.end());
-            }
+            const std::vector<double>& g = pk.g2[spin];
+            const int gd = pk.gd[spin];
             di_.gdim[spin] = gd;
             if (gd) {
                 std::vector<float> g32(g.begin(), g.end());
@@ -363,9 +384,63 @@ class Solver {
         have_ints_ = true;
     }
 
+    // point-group symmetry (FCISolver's MOSpaceInfo irreps and StateInfo irrep): the vectors keep the
+    // C1 layout, and only determinants of the target irrep ever become nonzero
+    void set_symmetry(const int* orbsym, int irrep) {
+        std::vector<int> os(orbsym, orbsym + n_);
+        for (int x : os)
+            if (x < 0 || x > 7) throw Error(-1, "orbital irreps must be in [0, 8)");
+        if (irrep < 0 || irrep > 7) throw Error(-1, "the state irrep must be in [0, 8)");
+        orbsym_ = os;
+        irrep_ = irrep;
+        for (int spin = 0; spin < 2; ++spin) {
+            const SpinTables& t = spin ? sb_ : sa_;
+            DBuf<uint8_t>& ir = spin ? symb_ : syma_;
+            DBuf<uint32_t>& rk = spin ? relb_ : rela_;
+            ir.alloc(std::max<int64_t>(1, t.nstr));
+            rk.alloc(std::max<int64_t>(1, t.nstr));
+            string_classes_device(t, orbsym_, ir.p, rk.p, spin ? cntb_ : cnta_, 0);
+        }
+        nsector_ = 0;
+        for (int h = 0; h < 8; ++h) nsector_ += cnta_[h] * cntb_[h ^ irrep_];
+        if (nsector_ == 0) throw Error(-1, "no determinant has the requested irrep");
+        symdet_.alloc(ndet_);
+        k_sector_mask<<<unsigned((ndet_ + 255) / 256), 256>>>(syma_.p, symb_.p, sa_.nstr, sb_.nstr, irrep_, symdet_.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(0));
+        sym_ = true;
+        dl_.reset();  // the next compute_energy builds a guess in the new sector
+    }
+    // eigenvector `root` restricted to the target irrep, in Forte's block order: alpha irrep ha
+    // outer, then alpha address within ha, then beta address within ha ^ irrep (fci_vector.cc:92-115)
+    std::vector<double> evec_sector(int root) {
+        std::vector<double> full(ndet_);
+        HIPCHK(hipMemcpy(full.data(), dl_->eigenvector_device(root), ndet_ * sizeof(double), hipMemcpyDeviceToHost));
+        if (!sym_) return full;
+        const std::vector<uint8_t> ia = syma_.to_host(), ib = symb_.to_host();
+        const std::vector<uint32_t> ra = rela_.to_host(), rb = relb_.to_host();
+        std::vector<int64_t> off(9, 0);
+        for (int h = 0; h < 8; ++h) off[h + 1] = off[h] + cnta_[h] * cntb_[h ^ irrep_];
+        std::vector<double> out(nsector_, 0.0);
+        for (int64_t a = 0; a < sa_.nstr; ++a)
+            for (int64_t b = 0; b < sb_.nstr; ++b) {
+                const int ha = ia[a];
+                if ((ha ^ ib[b]) != irrep_) continue;
+                out[off[ha] + int64_t(ra[a]) * cntb_[ha ^ irrep_] + rb[b]] = full[a * sb_.nstr + b];
+            }
+        return out;
+    }
+
     fci_options opt_;
     int n_, na_, nb_;
     int64_t ndet_ = 0;
+    bool sym_ = false;
+    int irrep_ = 0;
+    int64_t nsector_ = 0;
+    std::vector<int> orbsym_;
+    std::vector<int64_t> cnta_, cntb_;
+    DBuf<uint8_t> syma_, symb_, symdet_;
+    DBuf<uint32_t> rela_, relb_;
     DBuf<uint32_t> binom_;
     SpinTables sa_, sb_;
     DevInts di_;
@@ -424,13 +499,31 @@ void Solver::sigma_mixed32(const double* C, float* S) {
     HIPCHK(hipGetLastError());
 }
 
-// FCISolver::initial_guess_det (fci_solver_initial_guess.cc:88-101) and find_initial_guess_det
 std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(int num_guess_states) {
-    const int64_t nbs = sb_.nstr;
-    int64_t ngd = std::min<int64_t>(int64_t(num_guess_states) * opt_.ndets_per_guess, ndet_);
+    if (!sym_)
+        return initial_guess_dets(hdiag_.p, ndet_, ndet_, sb_.nstr, n_, na_, nb_, hi_, opt_.multiplicity,
+                                  opt_.ndets_per_guess, num_guess_states, 0);
+    // with point-group symmetry only the determinants of the target irrep are candidates (the
+    // reference's FCIVector holds that block only): the others get an infinite diagonal here
+    DBuf<double> hsel(ndet_);
+    k_sector_select<<<unsigned((ndet_ + 255) / 256), 256>>>(hdiag_.p, symdet_.p, ndet_, hsel.p);
+    HIPCHK(hipGetLastError());
+    return initial_guess_dets(hsel.p, ndet_, nsector_, sb_.nstr, n_, na_, nb_, hi_, opt_.multiplicity,
+                              opt_.ndets_per_guess, num_guess_states, 0);
+}
+
+// FCISolver::initial_guess_det (fci_solver_initial_guess.cc:88-101) and find_initial_guess_det
+// (sparse_initial_guess.cc:71-264) from the H diagonal on the device: guess vectors of the requested
+// multiplicity and the lower-lying roots of other multiplicities to project out.
+std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_dets(const double* d_hdiag, int64_t ndet_,
+                                                                             int64_t nsel, int64_t nbs, int n_, int na_,
+                                                                             int nb_, const HostInts& hi_,
+                                                                             int multiplicity, int ndets_per_guess,
+                                                                             int num_guess_states, hipStream_t st) {
+    int64_t ngd = std::min<int64_t>(int64_t(num_guess_states) * ndets_per_guess, nsel);
     std::vector<double> vals;
     std::vector<int64_t> idx;
-    smallest_k(hdiag_.p, ndet_, int(ngd), vals, idx, 0);
+    smallest_k(d_hdiag, ndet_, int(ngd), vals, idx, st);
     std::vector<Det> dets;
     for (int64_t I : idx) dets.push_back({unrank_host(uint64_t(I / nbs), n_, na_), unrank_host(uint64_t(I % nbs), n_, nb_)});
     spin_complete(dets, n_);
@@ -482,7 +575,7 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(
             }
         info[gr.m] = std::move(inf);
     }
-    auto it = info.find(opt_.multiplicity);
+    auto it = info.find(multiplicity);
     if (it == info.end()) throw Error(-1, "No guess with the requested multiplicity was found.");
     const Info& tg = it->second;
     if (tg.bsz < num_guess_states)
@@ -498,7 +591,7 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(
         gmax = std::max(gmax, tg.e[r]);
     }
     for (const auto& [mult_, inf] : info) {
-        if (mult_ == opt_.multiplicity) continue;
+        if (mult_ == multiplicity) continue;
         for (int r = 0; r < inf.bsz; ++r)
             if (inf.e[r] < gmax) {
                 SparseVec g(G);
@@ -543,7 +636,8 @@ double Solver::compute_energy() {
     const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, ndet_));
     compute_hdiag();
     dl_->add_h_diag_device(hdiag_.p, ndet_);
-    bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= ndet_;
+    bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= (sym_ ? nsector_ : ndet_);
+    dl_->set_mask(sym_ ? symdet_.p : nullptr);
     if (first_run || use_initial_guess) {
         dl_->reset();
         auto [g, bad] = initial_guess(num_guess_states);
@@ -602,7 +696,8 @@ double Solver::compute_energy() {
     dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
     dl_->set_e_convergence(opt_.e_convergence);
     dl_->set_r_convergence(opt_.r_convergence);
-    dl_->set_maxiter(opt_.maxiter);
+    // DL_MAXITER caps the whole solve: the refinement gets what the mixed phase left
+    dl_->set_maxiter(std::max(1, opt_.maxiter - iters));
     dl_->set_iteration_observer(lp_observer);
     converged = dl_->solve();
     dl_->set_iteration_observer(nullptr);
@@ -747,7 +842,6 @@ void* noalloc_scratch(int dev) {
     return (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
 }
 }  // namespace
-
 struct fci_dl {
     std::unique_ptr<fci::DavidsonLiu> d;
     fci::DBuf<double> M;
@@ -872,6 +966,156 @@ int fci_evec(const fci_solver* s, int root, double* out) {
 
 int64_t fci_ndet(const fci_solver* s) { return s ? s->s->ndet_ : -1; }
 int64_t fci_nstr(const fci_solver* s, int spin) { return s ? (spin ? s->s->sb_.nstr : s->s->sa_.nstr) : -1; }
+
+int fci_set_symmetry(fci_solver* s, const int* orbsym, int irrep) {
+    if (!s || !orbsym) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] { s->s->set_symmetry(orbsym, irrep); });
+}
+
+int64_t fci_sector_size(const fci_solver* s) {
+    if (!s) return -1;
+    return s->s->sym_ ? s->s->nsector_ : s->s->ndet_;
+}
+
+int fci_string_classes(fci_solver* s, int spin, int* irrep, int64_t* address) {
+    if (!s || !irrep || !address || spin < 0 || spin > 1) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        const fci::SpinTables& t = spin ? S.sb_ : S.sa_;
+        if (!S.sym_) {
+            for (int64_t i = 0; i < t.nstr; ++i) {
+                irrep[i] = 0;
+                address[i] = i;
+            }
+            return;
+        }
+        const std::vector<uint8_t> ir = (spin ? S.symb_ : S.syma_).to_host();
+        const std::vector<uint32_t> rk = (spin ? S.relb_ : S.rela_).to_host();
+        for (int64_t i = 0; i < t.nstr; ++i) {
+            irrep[i] = ir[i];
+            address[i] = rk[i];
+        }
+    });
+}
+
+int fci_evec_sector(fci_solver* s, int root, double* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (!S.solved_) throw Error(FCI_ERR_STATE, "no wave function; call compute_energy first");
+        if (root < 0 || root >= S.dl_->nroot()) throw Error(FCI_ERR_ARG, "root index out of range");
+        const std::vector<double> v = S.evec_sector(root);
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    });
+}
+
+int fci_transition_rdms(fci_solver* left, int rl, fci_solver* right, int rr, int level, double* g1a,
+                        double* g1b) {
+    if (!left || !right) return FCI_ERR_ARG;
+    return guarded(&left->err, [&] {
+        fci::Solver& A = *left->s;
+        fci::Solver& B = *right->s;
+        if (!A.solved_ || !B.solved_) throw Error(FCI_ERR_STATE, "FCIVector is not assigned. Cannot compute RDMs.");
+        if (level > 3 || level < 1) throw Error(FCI_ERR_ARG, "Invalid max_rdm_level, required 1 <= max_rdm_level <= 3.");
+        if (rl < 0 || rr < 0 || rl >= A.dl_->nroot() || rr >= B.dl_->nroot())
+            throw Error(FCI_ERR_ARG, "Cannot compute RDMs <" + std::to_string(rl) + "| ... |" + std::to_string(rr) +
+                                         "> (0-based) because nroot_left = " + std::to_string(A.dl_->nroot()) +
+                                         " and nroot_right = " + std::to_string(B.dl_->nroot()));
+        if (A.n_ != B.n_)
+            throw Error(FCI_ERR_ARG, "FCI transition RDMs: The number of MOs must be the same in the two wave functions.");
+        if (A.na_ != B.na_ || A.nb_ != B.nb_)
+            throw Error(FCI_ERR_ARG,
+                        "FCI transition RDMs: The number of alfa and beta electrons must be the same in the two wave "
+                        "functions.");
+        if (level >= 2)
+            throw Error(FCI_ERR_ARG,
+                        "Transition RDMs of order 2 or higher are not implemented in FCISolver (and more generally "
+                        "in Forte).");
+        const size_t n2 = size_t(A.n_) * A.n_;
+        fci::DBuf<double> d1a(n2), d1b(n2);
+        // both solvers have the same strings (same n, na, nb): A's tables address B's vector too
+        fci::rdm1_device(A.sa_, A.sb_, A.dl_->eigenvector_device(rl), B.dl_->eigenvector_device(rr), d1a.p, d1b.p, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        if (g1a) d1a.download(g1a, n2);
+        if (g1b) d1b.download(g1b, n2);
+    });
+}
+
+int fci_rdms3(fci_solver* s, int rl, int rr, double* g3aaa, double* g3aab, double* g3abb, double* g3bbb) {
+    if (!s) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (!S.solved_) throw Error(FCI_ERR_STATE, "FCISolver::rdms: no wave function; call compute_energy first");
+        if (rl < 0 || rr < 0 || rl >= S.dl_->nroot() || rr >= S.dl_->nroot())
+            throw Error(FCI_ERR_ARG, "FCISolver::rdms: root index out of range");
+        const int n = S.n_;
+        const int64_t Na = S.sa_.nstr, Nb = S.sb_.nstr, nd = Na * Nb;
+        const size_t n6 = size_t(n) * n * n * n * n * n;
+        // the reference's hole-list loops put the right state's strings on the creation side
+        // (fci_vector_rdm.cc:428-433, 492-500): the blocks are <R| a+ a+ a+ a a a |L>
+        const double* Lp = S.dl_->eigenvector_device(rr);
+        const double* Rp = S.dl_->eigenvector_device(rl);
+        const int64_t n3 = int64_t(fci::binom_host(n, 3)), n2p = int64_t(fci::binom_host(n, 2)) * n;
+        fci::DBuf<double> c3a(size_t(std::max<int64_t>(1, n3 * n3))), c21a(size_t(std::max<int64_t>(1, n2p * n2p)));
+        fci::DBuf<double> c3b(c3a.n), c21b(c21a.n);
+        fci::rdm3_compact_device(n, S.na_, S.nb_, Lp, Rp, Na, Nb, c3a.p, c21a.p, 0);
+        {
+            fci::DBuf<double> Lt(nd), Rt(nd);
+            fci::transpose_device<double>(Lp, Lt.p, Na, Nb, 0);
+            fci::transpose_device<double>(Rp, Rt.p, Na, Nb, 0);
+            fci::rdm3_compact_device(n, S.nb_, S.na_, Lt.p, Rt.p, Nb, Na, c3b.p, c21b.p, 0);
+        }
+        HIPCHK(hipStreamSynchronize(0));
+        const std::vector<double> A3 = c3a.to_host(), B3 = c3b.to_host(), A21 = c21a.to_host(), B21 = c21b.to_host();
+        // expansion: a+p a+q a+r is antisymmetric in (p, q, r); compact rows hold sorted index sets
+        auto tri = [&](int p, int q, int r, int64_t& rank) -> int {
+            if (p == q || q == r || p == r) return 0;
+            int a[3] = {p, q, r}, sg = 1;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 2 - i; ++j)
+                    if (a[j] > a[j + 1]) {
+                        std::swap(a[j], a[j + 1]);
+                        sg = -sg;
+                    }
+            rank = int64_t(a[0]) + int64_t(a[1]) * (a[1] - 1) / 2 + int64_t(a[2]) * (a[2] - 1) * (a[2] - 2) / 6;
+            return sg;
+        };
+        auto pr = [&](int p, int q, int64_t& rank) -> int {
+            if (p == q) return 0;
+            const int hi = std::max(p, q), lo = std::min(p, q);
+            rank = int64_t(hi) * (hi - 1) / 2 + lo;
+            return p < q ? 1 : -1;
+        };
+        auto at6 = [&](double* g, int p, int q, int r, int a, int b, int c) -> double& {
+            return g[((((size_t(p) * n + q) * n + r) * n + a) * n + b) * n + c];
+        };
+        for (double* g : {g3aaa, g3aab, g3abb, g3bbb})
+            if (g) std::memset(g, 0, n6 * sizeof(double));
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q)
+                for (int r = 0; r < n; ++r)
+                    for (int a = 0; a < n; ++a)
+                        for (int b = 0; b < n; ++b)
+                            for (int c = 0; c < n; ++c) {
+                                int64_t t1 = 0, t2 = 0;
+                                const int s1 = tri(p, q, r, t1), s2 = tri(a, b, c, t2);
+                                if (s1 && s2) {
+                                    if (g3aaa && S.na_ >= 3) at6(g3aaa, p, q, r, a, b, c) = s1 * s2 * A3[t1 * n3 + t2];
+                                    if (g3bbb && S.nb_ >= 3) at6(g3bbb, p, q, r, a, b, c) = s1 * s2 * B3[t1 * n3 + t2];
+                                }
+                                int64_t u1 = 0, u2 = 0;
+                                // aab [p,q,r,a,b,c]: (p,q,a,b) alpha, (r,c) beta
+                                const int f1 = pr(p, q, u1), f2 = pr(a, b, u2);
+                                if (g3aab && f1 && f2 && S.na_ >= 2 && S.nb_ >= 1)
+                                    at6(g3aab, p, q, r, a, b, c) = f1 * f2 * A21[(u1 * n + r) * n2p + u2 * n + c];
+                                // abb [p,q,r,a,b,c] = aab of the transposed vectors at [q,r,p,b,c,a]
+                                int64_t v1 = 0, v2 = 0;
+                                const int h1 = pr(q, r, v1), h2 = pr(b, c, v2);
+                                if (g3abb && h1 && h2 && S.nb_ >= 2 && S.na_ >= 1)
+                                    at6(g3abb, p, q, r, a, b, c) = h1 * h2 * B21[(v1 * n + p) * n2p + v2 * n + a];
+                            }
+    });
+}
 
 int fci_rdms(fci_solver* s, int rl, int rr, int level, double* g1a, double* g1b, double* g2aa, double* g2ab,
              double* g2bb) {

@@ -237,6 +237,88 @@ void build_spin_tables(SpinTables& t, int n, int k, const uint32_t* B, hipStream
     HIPCHK(hipGetLastError());
 }
 
+// ---- point-group classes of the strings (FCIStringLists::make_fci_strings with symmetry) ------
+// Forte enumerates the strings in next_permutation order and files each one under its irrep, its
+// address being its position among the strings of that irrep (fci_string_lists.cc:258-293,
+// fci_string_address.cc:36-59). The C1 order is the same enumeration, so the address within an
+// irrep is the number of same-irrep strings before it: per block, a ballot per irrep and wave
+// gives each string its rank in the wave and the wave totals; block totals are prefix-summed
+// across blocks on the host (one-time table work).
+constexpr int SC_BS = 256, SC_NIRREP = 8;
+__global__ __launch_bounds__(SC_BS) void k_string_irrep(const uint32_t* __restrict__ str, int64_t nstr,
+                                                        const uint32_t* __restrict__ orbsym_bits,
+                                                        uint8_t* __restrict__ irrep, uint32_t* __restrict__ rank,
+                                                        uint32_t* __restrict__ block_count) {
+    __shared__ uint32_t wcnt[SC_BS / 64][SC_NIRREP];
+    const int64_t I = int64_t(blockIdx.x) * SC_BS + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int h = -1;
+    if (I < nstr) {
+        // irrep = XOR of the occupied orbitals' irreps; orbsym_bits[b] holds bit b of every orbital's irrep
+        const uint32_t s = str[I];
+        h = (__builtin_popcount(s & orbsym_bits[0]) & 1) | ((__builtin_popcount(s & orbsym_bits[1]) & 1) << 1) |
+            ((__builtin_popcount(s & orbsym_bits[2]) & 1) << 2);
+        irrep[I] = uint8_t(h);
+    }
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t in_wave = 0;
+    for (int g = 0; g < SC_NIRREP; ++g) {
+        const uint64_t m = __ballot(h == g);
+        if (h == g) in_wave = uint32_t(__popcll(m & below));
+        if (lane == 0) wcnt[w][g] = uint32_t(__popcll(m));
+    }
+    __syncthreads();
+    if (I < nstr) {
+        uint32_t off = 0;
+        for (int v = 0; v < w; ++v) off += wcnt[v][h];
+        rank[I] = off + in_wave;  // within the block; the block offset is added afterwards
+    }
+    if (threadIdx.x < SC_NIRREP) {
+        uint32_t t = 0;
+        for (int v = 0; v < SC_BS / 64; ++v) t += wcnt[v][threadIdx.x];
+        block_count[int64_t(blockIdx.x) * SC_NIRREP + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_string_rank_offset(int64_t nstr, const uint8_t* __restrict__ irrep,
+                                     const uint32_t* __restrict__ block_off, uint32_t* __restrict__ rank) {
+    const int64_t I = int64_t(blockIdx.x) * SC_BS + threadIdx.x;
+    if (I < nstr) rank[I] += block_off[int64_t(blockIdx.x) * SC_NIRREP + irrep[I]];
+}
+
+void string_classes_device(const SpinTables& t, const std::vector<int>& orbsym, uint8_t* d_irrep, uint32_t* d_rank,
+                           std::vector<int64_t>& count, hipStream_t st) {
+    if (int(orbsym.size()) != t.n) throw Error(-1, "orbital symmetry list has the wrong length");
+    uint32_t bits[3] = {0, 0, 0};
+    for (int p = 0; p < t.n; ++p) {
+        if (orbsym[p] < 0 || orbsym[p] >= SC_NIRREP) throw Error(-1, "orbital irreps must be in [0, 8)");
+        for (int b = 0; b < 3; ++b)
+            if ((orbsym[p] >> b) & 1) bits[b] |= 1u << p;
+    }
+    count.assign(SC_NIRREP, 0);
+    if (t.nstr == 0) return;
+    const unsigned nb = unsigned((t.nstr + SC_BS - 1) / SC_BS);
+    DBuf<uint32_t> dbits(3), bc(size_t(nb) * SC_NIRREP);
+    dbits.upload(bits, 3);
+    k_string_irrep<<<nb, SC_BS, 0, st>>>(t.str.p, t.nstr, dbits.p, d_irrep, d_rank, bc.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<uint32_t> h = bc.to_host(), off(h.size());
+    for (int g = 0; g < SC_NIRREP; ++g) {
+        uint32_t run = 0;
+        for (unsigned b = 0; b < nb; ++b) {
+            off[size_t(b) * SC_NIRREP + g] = run;
+            run += h[size_t(b) * SC_NIRREP + g];
+        }
+        count[g] = run;
+    }
+    DBuf<uint32_t> doff(off.size());
+    doff.upload(off.data(), off.size());
+    k_string_rank_offset<<<nb, SC_BS, 0, st>>>(t.nstr, d_irrep, doff.p, d_rank);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+}
+
 // ---- Forte-order lists ---------------------------------------------------------------------
 // Insert the reduced-space bits b (n' bits) into the orbitals not in `skip` (ascending).
 __device__ __forceinline__ uint32_t d_expand(uint32_t b, int n, uint32_t skip) {

@@ -36,6 +36,7 @@ struct ActiveSpaceIntegrals {
     int n;
     std::vector<double> oei_a, oei_b, tei_aa, tei_ab, tei_bb;
     double scalar_energy = 0.0, frozen_core_energy = 0.0, nuclear_repulsion_energy = 0.0;
+    std::vector<int> active_mo_symmetry;  // irreps of the active orbitals (empty: all 0)
     double e_core() const { return scalar_energy + frozen_core_energy + nuclear_repulsion_energy; }
 };
 
@@ -56,7 +57,31 @@ enum class RDMsType { spin_dependent, spin_free };
 struct RDMs {
     int level = 0, n = 0;
     RDMsType type = RDMsType::spin_dependent;
-    std::vector<double> g1a, g1b, g2aa, g2ab, g2bb;
+    std::vector<double> g1a, g1b, g2aa, g2ab, g2bb, g3aaa, g3aab, g3abb, g3bbb;
+    py::array_t<double> t6(const std::vector<double>& v) const {
+        if (v.empty()) throw std::runtime_error("RDM not available at this level");
+        return make_arr({n, n, n, n, n, n}, v);
+    }
+    // RDMsSpinFree G3 (fci_vector_rdm.cc:127-131): g3aaa + g3aab(pqrstu) + g3aab(prqsut) + g3aab(qrptus)
+    //   + g3abb(pqrstu) + g3abb(qprtsu) + g3abb(rpqust) + g3bbb
+    std::vector<double> sf_g3() const {
+        if (g3aaa.empty()) throw std::runtime_error("3-RDM not available at this level");
+        std::vector<double> g(g3aaa.size());
+        auto ix = [&](int a, int b, int c, int d, int e, int f) {
+            return ((((size_t(a) * n + b) * n + c) * n + d) * n + e) * n + f;
+        };
+        for (int p = 0; p < n; ++p)
+            for (int q = 0; q < n; ++q)
+                for (int r = 0; r < n; ++r)
+                    for (int s = 0; s < n; ++s)
+                        for (int t = 0; t < n; ++t)
+                            for (int u = 0; u < n; ++u) {
+                                const size_t i = ix(p, q, r, s, t, u);
+                                g[i] = g3aaa[i] + g3aab[i] + g3aab[ix(p, r, q, s, u, t)] + g3aab[ix(q, r, p, t, u, s)] +
+                                       g3abb[i] + g3abb[ix(q, p, r, t, s, u)] + g3abb[ix(r, p, q, u, s, t)] + g3bbb[i];
+                            }
+        return g;
+    }
     py::array_t<double> t2(const std::vector<double>& v) const {
         if (v.empty()) throw std::runtime_error("RDM not available at this level");
         return make_arr({n, n}, v);
@@ -133,7 +158,9 @@ class DeviceArray {
 
 class FCISolver {
   public:
-    FCISolver(int na, int nb, int multiplicity, int nroot, std::shared_ptr<ActiveSpaceIntegrals> ints)
+    // FCISolver(StateInfo(na, nb, multiplicity, twice_ms, irrep), nroot, MOSpaceInfo, ActiveSpaceIntegrals)
+    // (fci_solver.h:63-64): the active orbitals' irreps come with the integrals
+    FCISolver(int na, int nb, int multiplicity, int nroot, std::shared_ptr<ActiveSpaceIntegrals> ints, int irrep = 0)
         : ints_(std::move(ints)) {
         check(fci_create(ints_->n, na, nb, ints_->oei_a.data(), ints_->oei_b.data(), ints_->tei_aa.data(),
                          ints_->tei_ab.data(), ints_->tei_bb.data(), ints_->e_core(), &h_),
@@ -142,6 +169,27 @@ class FCISolver {
         o_.nroot = nroot;
         o_.multiplicity = multiplicity;
         push();
+        const auto& os = ints_->active_mo_symmetry;
+        bool sym = irrep != 0;
+        for (int x : os) sym = sym || x != 0;
+        if (sym) {
+            std::vector<int> orbsym = os.empty() ? std::vector<int>(ints_->n, 0) : os;
+            if (int(orbsym.size()) != ints_->n) throw std::runtime_error("active_mo_symmetry has the wrong length");
+            check(fci_set_symmetry(h_, orbsym.data(), irrep), fci_last_error(h_));
+        }
+    }
+    int64_t sector_size() const { return fci_sector_size(h_); }
+    py::tuple string_classes(int spin) const {
+        const int64_t ns = fci_nstr(h_, spin);
+        py::array_t<int> ir(ns);
+        py::array_t<int64_t> ad(ns);
+        check(fci_string_classes(h_, spin, ir.mutable_data(), ad.mutable_data()), fci_last_error(h_));
+        return py::make_tuple(ir, ad);
+    }
+    py::array_t<double> evec_sector(int root) const {
+        py::array_t<double> a(fci_sector_size(h_));
+        check(fci_evec_sector(h_, root, a.mutable_data()), fci_last_error(h_));
+        return a;
     }
     ~FCISolver() { fci_destroy(h_); }
     FCISolver(const FCISolver&) = delete;
@@ -174,6 +222,20 @@ class FCISolver {
     }
     // FCISolver::set_options (fci_solver.cc:156-172) from a dict of Forte option names
     void set_options(const py::dict& d) {
+        apply_options(d, o_);
+        push();
+    }
+    static void apply_options(const py::dict& d, fci_options& o_) {
+        auto set_precision = [&](const std::string& p) {
+            if (p == "fp64") o_.precision = FCI_PREC_FP64;
+            else if (p == "mixed") o_.precision = FCI_PREC_MIXED;
+            else throw std::runtime_error("precision must be 'fp64' or 'mixed'");
+        };
+        auto set_preconditioner = [&](const std::string& p) {
+            if (p == "diagonal") o_.preconditioner = FCI_PRECOND_DIAGONAL;
+            else if (p == "learned") o_.preconditioner = FCI_PRECOND_LEARNED;
+            else throw std::runtime_error("preconditioner must be 'diagonal' or 'learned'");
+        };
         for (auto item : d) {
             std::string k = py::str(item.first);
             py::handle v = item.second;
@@ -197,7 +259,6 @@ class FCISolver {
             }
             else throw std::runtime_error("unknown FCI option " + k);
         }
-        push();
     }
     void set_active_space_integrals(std::shared_ptr<ActiveSpaceIntegrals> ints) {
         ints_ = std::move(ints);
@@ -265,11 +326,50 @@ class FCISolver {
                 R->g2ab.resize(n4);
                 R->g2bb.resize(n4);
             }
+            if (level > 3) throw std::runtime_error("RDMs of level 4 and higher are not implemented");
             int st;
             {
                 py::gil_scoped_release nogil;
-                st = fci_rdms(h_, l, r, level, R->g1a.data(), R->g1b.data(), level >= 2 ? R->g2aa.data() : nullptr,
-                              level >= 2 ? R->g2ab.data() : nullptr, level >= 2 ? R->g2bb.data() : nullptr);
+                st = fci_rdms(h_, l, r, std::min(level, 2), R->g1a.data(), R->g1b.data(),
+                              level >= 2 ? R->g2aa.data() : nullptr, level >= 2 ? R->g2ab.data() : nullptr,
+                              level >= 2 ? R->g2bb.data() : nullptr);
+            }
+            check(st, fci_last_error(h_));
+            if (level >= 3) {
+                const size_t n6 = n4 * n2;
+                R->g3aaa.resize(n6);
+                R->g3aab.resize(n6);
+                R->g3abb.resize(n6);
+                R->g3bbb.resize(n6);
+                {
+                    py::gil_scoped_release nogil;
+                    st = fci_rdms3(h_, l, r, R->g3aaa.data(), R->g3aab.data(), R->g3abb.data(), R->g3bbb.data());
+                }
+                check(st, fci_last_error(h_));
+            }
+            out.push_back(R);
+        }
+        return out;
+    }
+
+    // ActiveSpaceMethod::transition_rdms(root_list, method2, max_rdm_level, type)
+    // (active_space_method.h:151-154; FCISolver: fci_solver_transition_rdms.cc:37-99)
+    std::vector<std::shared_ptr<RDMs>> transition_rdms(const std::vector<std::pair<int, int>>& roots,
+                                                       const FCISolver& method2, int level, RDMsType type) {
+        std::vector<std::shared_ptr<RDMs>> out;
+        const int n = ints_->n;
+        const size_t n2 = size_t(n) * n;
+        for (auto [l, r] : roots) {
+            auto R = std::make_shared<RDMs>();
+            R->level = level;
+            R->n = n;
+            R->type = type;
+            R->g1a.resize(n2);
+            R->g1b.resize(n2);
+            int st;
+            {
+                py::gil_scoped_release nogil;
+                st = fci_transition_rdms(h_, l, method2.h_, r, level, R->g1a.data(), R->g1b.data());
             }
             check(st, fci_last_error(h_));
             out.push_back(R);
@@ -498,6 +598,13 @@ PYBIND11_MODULE(_forte_amd, m) {
              py::arg("tei_bb"), py::arg("scalar_energy") = 0.0, py::arg("frozen_core_energy") = 0.0,
              py::arg("nuclear_repulsion_energy") = 0.0)
         .def_property_readonly("nmo", [](const ActiveSpaceIntegrals& a) { return a.n; })
+        .def("active_mo_symmetry", [](const ActiveSpaceIntegrals& a) {
+            return a.active_mo_symmetry.empty() ? std::vector<int>(a.n, 0) : a.active_mo_symmetry;
+        })
+        .def("set_active_mo_symmetry", [](ActiveSpaceIntegrals& a, const std::vector<int>& s) {
+            if (!s.empty() && int(s.size()) != a.n) throw std::runtime_error("one irrep per active orbital");
+            a.active_mo_symmetry = s;
+        })
         .def("scalar_energy", [](const ActiveSpaceIntegrals& a) { return a.scalar_energy; })
         .def("frozen_core_energy", [](const ActiveSpaceIntegrals& a) { return a.frozen_core_energy; })
         .def("nuclear_repulsion_energy", [](const ActiveSpaceIntegrals& a) { return a.nuclear_repulsion_energy; })
@@ -516,11 +623,19 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("g2ab", [](const RDMs& r) { return r.t4(r.g2ab); })
         .def("g2bb", [](const RDMs& r) { return r.t4(r.g2bb); })
         .def("SF_G1", [](const RDMs& r) { return r.t2(r.sf_g1()); })
-        .def("SF_G2", [](const RDMs& r) { return r.t4(r.sf_g2()); });
+        .def("SF_G2", [](const RDMs& r) { return r.t4(r.sf_g2()); })
+        .def("g3aaa", [](const RDMs& r) { return r.t6(r.g3aaa); })
+        .def("g3aab", [](const RDMs& r) { return r.t6(r.g3aab); })
+        .def("g3abb", [](const RDMs& r) { return r.t6(r.g3abb); })
+        .def("g3bbb", [](const RDMs& r) { return r.t6(r.g3bbb); })
+        .def("SF_G3", [](const RDMs& r) { return r.t6(r.sf_g3()); });
 
     py::class_<FCISolver, std::shared_ptr<FCISolver>>(m, "FCISolver")
-        .def(py::init<int, int, int, int, std::shared_ptr<ActiveSpaceIntegrals>>(), py::arg("na"), py::arg("nb"),
-             py::arg("multiplicity"), py::arg("nroot"), py::arg("as_ints"))
+        .def(py::init<int, int, int, int, std::shared_ptr<ActiveSpaceIntegrals>, int>(), py::arg("na"), py::arg("nb"),
+             py::arg("multiplicity"), py::arg("nroot"), py::arg("as_ints"), py::arg("irrep") = 0)
+        .def("sector_size", &FCISolver::sector_size)
+        .def("string_classes", &FCISolver::string_classes)
+        .def("evec_sector", &FCISolver::evec_sector)
         .def("compute_energy", &FCISolver::compute_energy)
         .def("energies", &FCISolver::energies)
         .def("spin2", &FCISolver::spin2)
@@ -530,6 +645,8 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("nstr", &FCISolver::nstr)
         .def("rdms", &FCISolver::rdms, py::arg("root_list"), py::arg("max_rdm_level"),
              py::arg("rdm_type") = RDMsType::spin_dependent)
+        .def("transition_rdms", &FCISolver::transition_rdms, py::arg("root_list"), py::arg("method2"),
+             py::arg("max_rdm_level"), py::arg("rdm_type") = RDMsType::spin_dependent)
         .def("sigma", &FCISolver::sigma)
         .def("sigma_device", &FCISolver::sigma_device, py::arg("c_ptr"), py::arg("sigma_ptr"), py::arg("dtype"),
              py::arg("stream") = 0)
@@ -628,6 +745,75 @@ PYBIND11_MODULE(_forte_amd, m) {
         return py::make_tuple(out, losses);
     }, py::arg("params"), py::arg("X"), py::arg("d0"), py::arg("e"), py::arg("group"), py::arg("epochs"),
        py::arg("lr") = 1e-3f, py::arg("reg") = 1e-4f, py::arg("seed") = 0);
+    // many independent problems in one call (fci_batch_solve): problem i = (ints[i], na[i], nb[i])
+    m.def("batch_solve", [](const std::vector<std::shared_ptr<ActiveSpaceIntegrals>>& ints, const std::vector<int>& na,
+                            const std::vector<int>& nb, int nroot, int multiplicity, const py::dict& options,
+                            bool want_evecs) {
+        const size_t P = ints.size();
+        if (na.size() != P || nb.size() != P) throw std::runtime_error("batch_solve: ints, na and nb differ in length");
+        fci_options o;
+        fci_default_options(&o);
+        o.nroot = nroot;
+        o.multiplicity = multiplicity;
+        o.die_if_not_converged = 0;
+        FCISolver::apply_options(options, o);
+        std::vector<int> norb(P), iters(P), conv(P);
+        std::vector<const double*> oa(P), ob(P), taa(P), tab(P), tbb(P);
+        std::vector<double> ec(P);
+        std::vector<py::array_t<double>> ev;
+        std::vector<double*> evp(P, nullptr);
+        for (size_t i = 0; i < P; ++i) {
+            const ActiveSpaceIntegrals& a = *ints[i];
+            norb[i] = a.n;
+            oa[i] = a.oei_a.data(); ob[i] = a.oei_b.data();
+            taa[i] = a.tei_aa.data(); tab[i] = a.tei_ab.data(); tbb[i] = a.tei_bb.data();
+            ec[i] = a.e_core();
+            if (want_evecs) {
+                int64_t ns_a = 1, ns_b = 1;
+                for (int t = 0; t < na[i]; ++t) ns_a = ns_a * (a.n - t) / (t + 1);
+                for (int t = 0; t < nb[i]; ++t) ns_b = ns_b * (a.n - t) / (t + 1);
+                ev.emplace_back(std::vector<py::ssize_t>{nroot, py::ssize_t(ns_a * ns_b)});
+                evp[i] = ev.back().mutable_data();
+            }
+        }
+        py::array_t<double> e(std::vector<py::ssize_t>{py::ssize_t(P), nroot});
+        fci_batch_stats st;
+        std::memset(&st, 0, sizeof(st));
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = fci_batch_solve(int(P), norb.data(), na.data(), nb.data(), oa.data(), ob.data(), taa.data(),
+                                 tab.data(), tbb.data(), ec.data(), &o, e.mutable_data(), iters.data(), conv.data(),
+                                 want_evecs ? evp.data() : nullptr, &st);
+        }
+        if (rc != FCI_OK) throw std::runtime_error(fci_batch_last_error());
+        py::dict out, stats;
+        out["energies"] = e;
+        out["iterations"] = iters;
+        std::vector<bool> cb(conv.begin(), conv.end());
+        out["converged"] = cb;
+        if (want_evecs) out["evecs"] = ev;
+        stats["setup_seconds"] = st.setup_seconds;
+        stats["iterate_seconds"] = st.iterate_seconds;
+        stats["converged"] = st.converged;
+        stats["iterations"] = st.iterations;
+        stats["sigma_builds"] = st.sigma_builds;
+        py::list cls;
+        for (int k = 0; k < std::min(st.nclasses, FCI_BATCH_MAX_CLASSES); ++k) {
+            py::dict c;
+            c["norb"] = st.class_norb[k];
+            c["problems"] = st.class_nprob[k];
+            c["chunk"] = st.class_chunk[k];
+            c["subbatch"] = st.class_subbatch[k];
+            c["graph_launches"] = st.class_graph_launches[k];
+            c["ms_per_iteration"] = st.class_ms_per_iteration[k];
+            cls.append(c);
+        }
+        stats["classes"] = cls;
+        out["stats"] = stats;
+        return out;
+    }, py::arg("ints"), py::arg("na"), py::arg("nb"), py::arg("nroot") = 1, py::arg("multiplicity") = 1,
+       py::arg("options") = py::dict(), py::arg("want_evecs") = false);
     m.def("seeded_integrals", [](int n, uint64_t seed) {
         std::vector<double> h(size_t(n) * n), eri(size_t(n) * n * n * n);
         fci_seeded_integrals(n, seed, h.data(), eri.data());

@@ -7,13 +7,18 @@ their position in the file: four zero indices give the scalar (nuclear + frozen 
 i>0 and j=k=l=0 an orbital energy, i,j>0 and k=l=0 a core Hamiltonian element, anything else an
 ERI (ij|kl).
 """
+import gzip
+
 import numpy as np
 
 
 def read_fcidump(fname):
-    """Return dict(norb, nelec, ms2, isym, orbsym, uhf, enuc, hcore, eri[, epsilon])."""
+    """Return dict(norb, nelec, ms2, isym, orbsym, pntgrp, uhf, enuc, hcore, eri[, epsilon]).
+    orbsym and isym are as written in the file (the FCIDUMP convention, 1-based); a gzip-compressed
+    file (name ending in .gz) is read the same way."""
     out = {}
-    with open(fname) as f:
+    opener = gzip.open if str(fname).endswith(".gz") else open
+    with opener(fname, "rt") as f:
         lines = f.read().splitlines()
     if lines[0].strip() != "&FCI":
         raise ValueError(f"{fname}: not an FCIDUMP file (first line is not &FCI)")
@@ -45,6 +50,7 @@ def read_fcidump(fname):
     out["uhf"] = "TRUE" in "".join(keys.get("UHF", ["FALSE"])).upper()
     out["orbsym"] = [int(x) for x in keys.get("ORBSYM", ["1"] * norb)]
     out["isym"] = int(keys.get("ISYM", ["1"])[0])
+    out["pntgrp"] = keys["PNTGRP"][0] if keys.get("PNTGRP") else None
     hcore = np.zeros((norb, norb))
     eri = np.zeros((norb, norb, norb, norb))
     eps = None

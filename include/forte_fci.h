@@ -8,7 +8,10 @@
  * a negative status on failure; fci_last_error() then holds the message. No C++ exception
  * crosses this boundary. Output buffers are owned by the caller.
  *
- * Scope: one irrep (C1), all active orbitals, n <= 24, any na / nb. Integrals use Forte's
+ * Scope: one irrep (C1), all active orbitals, any na / nb; n <= 22 orbitals for the string tables
+ * and n <= 19 for sigma (the alpha-beta kernels hold at most 12 row tiles of 16 packed pairs).
+ * The same-spin 2-RDM needs cliques of at most 96 members, C(n - k + 2, 2) <= 96 for k electrons
+ * of a spin (so CAS(16,16) works, but 2 electrons in 16 orbitals do not). Integrals use Forte's
  * layout (forte/integrals/active_space_integrals.h:103-127): oei n*n row-major, tei n^4
  * row-major physicist, tei_aa / tei_bb antisymmetrized <pq||rs>, tei_ab = <pq|rs>.
  * Determinants are ordered alpha-major, C[Ia*Nb + Ib] (forte/fci/fci_vector.cc:108-114).
@@ -77,6 +80,24 @@ int fci_create(int n, int na, int nb, const double* oei_a, const double* oei_b,
 void fci_destroy(fci_solver* s);
 const char* fci_last_error(const fci_solver* s); /* also valid for s == NULL (global error) */
 
+/* Point-group symmetry (FCISolver's StateInfo irrep and the active orbitals' irreps from
+ * MOSpaceInfo, fci_solver.cc:55-65; determinant symmetry = XOR of the occupied orbitals' irreps,
+ * fci_string_lists.cc:52-205): orbsym[n] in [0, 8) (Cotton order), irrep = target irrep. The solver
+ * then works in the determinants of that irrep (the reference's FCIVector blocks,
+ * fci_vector.cc:92-115); vectors passed to and returned by the C1 entry points (fci_evec,
+ * fci_sigma*) keep the C1 layout and are zero outside the sector. */
+int fci_set_symmetry(fci_solver* s, const int* orbsym, int irrep);
+/* number of determinants of the target irrep (fci_ndet without symmetry) */
+int64_t fci_sector_size(const fci_solver* s);
+/* FCIStringLists string classes (fci_string_lists.cc:258-293, fci_string_address.cc:36-59): for
+ * every string in C1 order, its irrep and its address within that irrep (all 0 / the C1 address
+ * without symmetry); nstr entries each */
+int fci_string_classes(fci_solver* s, int spin, int* irrep, int64_t* address);
+/* eigenvector `root` in the reference's symmetry-blocked order (fci_sector_size doubles): alpha
+ * irrep ha outer, then alpha address within ha, then beta address within ha ^ irrep
+ * (FCIVector::copy_to, fci_vector.cc:108-169) */
+int fci_evec_sector(fci_solver* s, int root, double* out);
+
 /* FCISolver::set_options, fci_solver.cc:156-172 */
 int fci_set_options(fci_solver* s, const fci_options* opts);
 /* ActiveSpaceMethod::set_active_space_integrals, active_space_method.h:255 (keeps the
@@ -103,6 +124,22 @@ int64_t fci_nstr(const fci_solver* s, int spin);
  * <a+p a+q a_s a_r> stored at [p,q,r,s]. Pass NULL for arrays not wanted. */
 int fci_rdms(fci_solver* s, int root_left, int root_right, int level, double* g1a, double* g1b,
              double* g2aa, double* g2ab, double* g2bb);
+/* The 3-RDMs of FCIVector::compute_rdms at max_rdm_level 3 (fci_vector_rdm.cc:70-103, 395-565):
+ * g3aaa, g3aab, g3abb, g3bbb (n^6 each, NULL for blocks not wanted), element [p,q,r,s,t,u] =
+ * <a+p a+q a+r a_u a_t a_s> with spins aaa / (p,q,s,t alpha, r,u beta) / (p,s alpha, q,r,t,u beta) /
+ * bbb. For root_left != root_right the reference's hole-string loops place the right state on the
+ * creation side, i.e. <right| ... |left>; this follows them. */
+int fci_rdms3(fci_solver* s, int root_left, int root_right, double* g3aaa, double* g3aab, double* g3abb,
+              double* g3bbb);
+/* ActiveSpaceMethod::transition_rdms (active_space_method.h:151-154) as FCISolver implements it
+ * (fci_solver_transition_rdms.cc:37-99, fci_vector_transition_rdm.cc:43-163): the 1-RDMs
+ * <left root_left| a+p a_q |right root_right> of each spin between the eigenvectors of two solved
+ * solvers with the same orbitals and electron counts. level must be 1: levels 2 and 3 return
+ * FCI_ERR_ARG with the reference's message. (The reference's beta block reads the alpha electron
+ * count, fci_vector_transition_rdm.cc:112-113, and so is zero whenever na == 0; here it is
+ * computed.) */
+int fci_transition_rdms(fci_solver* left, int root_left, fci_solver* right, int root_right, int level,
+                        double* g1a, double* g1b);
 /* FCIVector::compute_spin2 (fci_vector_spin2.cc:38-75) of a host vector */
 int fci_spin2_of(fci_solver* s, const double* C, double* spin2);
 
@@ -210,6 +247,35 @@ int fci_lp_loss_grad(const float* params, const float* X, const float* d0, const
  * mean loss of each epoch. */
 int fci_lp_train(float* params, const float* X, const float* d0, const float* e, int64_t ngroups, int64_t group,
                  int epochs, float lr, float reg, uint64_t seed, float* epoch_loss);
+
+/* ---- many independent problems in one call ------------------------------------------------
+ * The repeated-solve pattern of CASSCF micro-iterations and geometry scans (mcscf_2step.cc:187-198:
+ * set integrals, FCISolver::compute_energy, fci_solver.cc:177-314), for nprob independent problems
+ * at once. Problem i has norb[i] orbitals, nalpha[i] / nbeta[i] electrons and integrals in Forte's
+ * layout (as fci_create). Problems of one shape share string tables and advance together through
+ * Davidson-Liu iterations (davidson_liu_solver.cc:180-273) that run entirely on the device: one
+ * captured graph replay per iteration, no host decision inside an iteration. opts: nroot (<= 4),
+ * multiplicity, guess/collapse/subspace per root (subspace * nroot <= 32), maxiter, e/r
+ * convergence; FP64, diagonal preconditioner. Outputs: energies[i * nroot + k]; optional
+ * iterations[i], converged[i] (1/0), evecs[i] (host buffers of nroot * ndet_i doubles, or NULL),
+ * stats. Errors: negative status, message from fci_batch_last_error(). */
+#define FCI_BATCH_MAX_CLASSES 16
+typedef struct fci_batch_stats {
+    double setup_seconds;    /* tables, integrals, diagonals, guesses, graph capture */
+    double iterate_seconds;  /* Davidson-Liu iterations (graph replays), autotuning included */
+    int nclasses, converged, iterations, sigma_builds;
+    int class_norb[FCI_BATCH_MAX_CLASSES], class_nprob[FCI_BATCH_MAX_CLASSES];
+    int class_chunk[FCI_BATCH_MAX_CLASSES];          /* autotuned streaming chunk (elements) */
+    int class_subbatch[FCI_BATCH_MAX_CLASSES];       /* autotuned sigma sub-batch (items) */
+    int class_graph_launches[FCI_BATCH_MAX_CLASSES];
+    double class_ms_per_iteration[FCI_BATCH_MAX_CLASSES]; /* timed during autotuning */
+} fci_batch_stats;
+int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nbeta,
+                    const double* const* oei_a, const double* const* oei_b, const double* const* tei_aa,
+                    const double* const* tei_ab, const double* const* tei_bb, const double* e_core,
+                    const fci_options* opts, double* energies, int* iterations, int* converged,
+                    double* const* evecs, fci_batch_stats* stats);
+const char* fci_batch_last_error(void);
 
 /* ---- seeded integrals of the benchmark configurations (host) ----------------------------- */
 /* h (n*n) and chemist ERIs (pq|rs) (n^4) from a SplitMix64 stream; see fci_solver.hip. */

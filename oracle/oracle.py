@@ -94,6 +94,39 @@ def _p(a, t=ctypes.c_double):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
+def string_classes(n, k, orbsym):
+    """FCIStringLists::make_fci_strings with symmetry (fci_string_lists.cc:258-293,
+    fci_string_address.cc:36-59): the strings in next_permutation order, each filed under its irrep
+    (XOR of its orbitals' irreps) at the next address of that irrep. Returns (irrep, address) per
+    string in enumeration order (the C1 order)."""
+    strs = make_strings(n, k)
+    irrep = np.zeros(len(strs), dtype=np.int64)
+    addr = np.zeros(len(strs), dtype=np.int64)
+    seen = {}
+    for i, x in enumerate(strs):
+        h = 0
+        for p in range(n):
+            if (int(x) >> p) & 1:
+                h ^= orbsym[p]
+        irrep[i] = h
+        addr[i] = seen.get(h, 0)
+        seen[h] = addr[i] + 1
+    return irrep, addr
+
+
+def sector_mask(o, orbsym, irrep):
+    """Determinants (C1 order) whose irrep is `irrep`."""
+    def sym(s):
+        h = 0
+        for p in range(o.n):
+            if (int(s) >> p) & 1:
+                h ^= orbsym[p]
+        return h
+    sa = np.array([sym(x) for x in o.strings(0)], dtype=np.int64)
+    sb = np.array([sym(x) for x in o.strings(1)], dtype=np.int64)
+    return ((sa[:, None] ^ sb[None, :]) == irrep).ravel()
+
+
 class DLOpts(ctypes.Structure):
     _fields_ = [("collapse_per_root", ctypes.c_int), ("subspace_per_root", ctypes.c_int),
                 ("maxiter", ctypes.c_int), ("e_convergence", ctypes.c_double),

@@ -305,3 +305,47 @@ def test_dl_errors(oracle):
         oracle.dl_dense(M4, 1, hdiag=np.diag(M4), guesses=[[(0, 0.1)]], no_sigma=True)
     with pytest.raises(RuntimeError):
         oracle.dl_dense(M4, 1, hdiag=None)
+
+
+# ---- point-group symmetry and doubly occupied cores (the reference's D2h FCIDUMP tests) -----
+
+KAT_SYM = {  # tests/methods/integrals-fcidump-1 and -6 (input.dat; the same INTDUMP file)
+    1: (dict(frozen_docc=[2, 0, 0, 0], restricted_docc=[2, 0, 0, 0], active=[2, 2, 2, 2]), -112.74446815362198, 784),
+    6: (dict(frozen_docc=[2, 0, 0, 0], restricted_docc=[2, 0, 0, 0], active=[2, 2, 2, 2], nel=13, multiplicity=2),
+        -112.181397694724, 392),
+}
+
+
+@pytest.mark.parametrize("k", sorted(KAT_SYM))
+def test_kat_symmetry_dense(oracle, k):
+    """Core folding, MO spaces and the irrep sector against the reference's known answers, by a
+    dense diagonalization of the Slater-rules Hamiltonian restricted to the sector."""
+    import os
+    from forte_amd.active_space import active_space_from_fcidump, forte_layout
+    from conftest import GOLDEN
+    kw, ref, ndet = KAT_SYM[k]
+    a = active_space_from_fcidump(os.path.join(GOLDEN, "integrals-fcidump-1.INTDUMP.gz"), **kw)
+    h, aa, ab = forte_layout(a["h"], a["eri"])
+    o = oracle.Oracle(h.shape[0], a["na"], a["nb"], h, h, aa, ab, aa, a["e_core"])
+    mask = oracle.sector_mask(o, a["orbsym"], a["irrep"])
+    assert mask.sum() == ndet  # "Number of determinants" of the reference output
+    ev = np.linalg.eigvalsh(o.dense_hamiltonian()[np.ix_(mask, mask)])
+    assert abs(ev[0] - ref) < 1e-9
+
+
+@pytest.mark.parametrize("n,k,orbsym", [(6, 3, [0, 1, 2, 3, 0, 1]), (8, 4, [0, 0, 1, 1, 2, 2, 3, 3]),
+                                        (7, 2, [0, 4, 1, 5, 2, 6, 3]), (5, 5, [1, 1, 0, 2, 3])])
+def test_string_classes(oracle, n, k, orbsym):
+    """Per-irrep string addresses: consecutive within each irrep, in enumeration order, with
+    class sizes equal to an independent count over all k-subsets."""
+    irrep, addr = oracle.string_classes(n, k, orbsym)
+    for h in set(irrep.tolist()):
+        assert np.array_equal(addr[irrep == h], np.arange(np.sum(irrep == h)))
+    count = {}
+    for c in itertools.combinations(range(n), k):
+        h = 0
+        for p in c:
+            h ^= orbsym[p]
+        count[h] = count.get(h, 0) + 1
+    assert {h: int(np.sum(irrep == h)) for h in count} == count
+
