@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <map>
 
@@ -29,9 +30,14 @@ void check_device_public();
 
 constexpr int BMS = 32;         // max subspace size (subspace_per_root * nroot)
 constexpr int BNR = 4;          // max roots
-constexpr int BGS_PASSES = 6;   // Gram-Schmidt passes per added vector inside one iteration
+// Gram-Schmidt passes per added vector inside one iteration: a correction needs a measuring pass,
+// one or two subtractions and the scaling pass; a collapsed vector is orthonormal up to rounding
+// (a measuring and a scaling pass); a vector not accepted within its passes is discarded
+constexpr int BGS_PASSES = 6;   // guesses and final Ritz vectors (outside the graph)
+constexpr int BGS_PASSES_CORR = 5, BGS_PASSES_COLL = 4;
 constexpr int BBS = 256;        // threads of the streaming kernels
-constexpr int BPS = BMS * BMS;  // partial-sum stride per chunk (largest: the overlap matrix)
+constexpr int BPS = BMS * BMS;  // partial-sum stride per block (largest: the overlap matrix)
+constexpr int BNPB = 128;       // at most this many blocks per problem in a streaming kernel
 
 struct BState {
     int bs, ss, gv;      // basis size, vectors with sigma, overlap entries valid
@@ -52,13 +58,14 @@ struct BState {
 struct BDev {
     int P, NR, MS, CS, NSL;
     int64_t nd;
-    int nch;
+    int nch;     // chunks of chl elements per vector
+    int npb;     // blocks per problem in the streaming kernels (each loops over its chunks)
     int64_t chl;
     BState* st;
     double *b, *sg, *r, *tmp, *tms;  // [P][MS][nd], [P][MS][nd], [P][NR][nd], [P][TM][nd], [P][CS][nd]
     int TM;
     const double* hd;                // [P][nd]
-    double* part;                    // [P][nch][BPS]
+    double* part;                    // [P][npb][BPS]: one partial per block
     const int* pvo;                  // bad roots of problem p: vectors [pvo[p], pvo[p+1])
     const int64_t* pvoff;            // entries of bad vector v: [pvoff[v], pvoff[v+1])
     const int64_t* pidx;
@@ -139,31 +146,70 @@ __global__ void k_b_after_sigma(BDev d) {
 }
 
 // ---- Rayleigh-Ritz ------------------------------------------------------------------------------
-// partial sums of G[i][j] = b_i . sigma_j for the entries not yet known (max(i, j) >= gv)
+// partial sums of G[i][j] = b_i . sigma_j for the entries not yet known (max(i, j) >= gv): the
+// chunk's elements are staged 32 at a time for every basis and sigma vector in LDS, so each
+// element is read from memory once, and each thread accumulates up to 4 of the pairs
+constexpr int GD_TE = 32;
 __global__ __launch_bounds__(BBS) void k_b_gdots(BDev d) {
-    __shared__ double red[BBS / 64];
-    const int p = blockIdx.y, c = blockIdx.x;
+    __shared__ double Bt[BMS][GD_TE + 1], St[BMS][GD_TE + 1];
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (s.done) return;
     const int bs = s.bs, gv = s.gv;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
+    const int npair = bs * bs - gv * gv;
     const double* B = d.b + int64_t(p) * d.MS * d.nd;
     const double* S = d.sg + int64_t(p) * d.MS * d.nd;
-    double* out = d.part + (int64_t(p) * d.nch + c) * BPS;
-    for (int i = 0; i < bs; ++i)
-        for (int j = 0; j < bs; ++j) {
-            if (i < gv && j < gv) continue;
-            const double* bi = B + int64_t(i) * d.nd;
-            const double* sj = S + int64_t(j) * d.nd;
-            double acc = 0.0;
-            for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) acc += bi[e] * sj[e];
-            acc = block_sum(acc, red);
-            if (threadIdx.x == 0) out[i * BMS + j] = acc;
+    // pair index t -> (i, j): the rows i >= gv (all columns), then rows i < gv with columns j >= gv
+    auto pair_of = [&](int t, int& i, int& j) {
+        const int nhi = (bs - gv) * bs;
+        if (t < nhi) {
+            i = gv + t / bs;
+            j = t % bs;
+        } else {
+            t -= nhi;
+            i = t / (bs - gv);
+            j = gv + t % (bs - gv);
         }
+    };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int pi[4], pj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int t = threadIdx.x + u * BBS;
+        pi[u] = pj[u] = 0;
+        if (t < npair) pair_of(t, pi[u], pj[u]);
+    }
+    for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+    for (int64_t t0 = int64_t(c) * d.chl, e1 = min(t0 + d.chl, d.nd); t0 < e1; t0 += GD_TE) {
+        for (int x = threadIdx.x; x < bs * GD_TE; x += BBS) {
+            const int v = x / GD_TE, e = x % GD_TE;
+            const int64_t el = t0 + e;
+            const bool ok = el < e1;
+            Bt[v][e] = ok ? B[int64_t(v) * d.nd + el] : 0.0;
+            St[v][e] = ok ? S[int64_t(v) * d.nd + el] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (threadIdx.x + u * BBS >= npair) break;
+            double a = acc[u];
+#pragma unroll 8
+            for (int e = 0; e < GD_TE; ++e) a += Bt[pi[u]][e] * St[pj[u]][e];
+            acc[u] = a;
+        }
+        __syncthreads();
+    }
+    double* out = d.part + (int64_t(p) * d.npb + blockIdx.x) * BPS;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int t = threadIdx.x + u * BBS;
+        if (t < npair) out[pi[u] * BMS + pj[u]] = acc[u];
+    }
 }
 
-// One wave per problem: complete G, then Jacobi on sym(G) (the same rotations, sweep rule and
-// ascending stable order as the host solver), lambda and alpha.
+// One wave per problem: complete G, then Jacobi on sym(G) (the host solver's rotation formula,
+// sweep rule and ascending stable order, with the rotations of a sweep applied in parallel
+// rounds), lambda and alpha.
 __global__ __launch_bounds__(64) void k_b_eigh(BDev d) {
     __shared__ double A[BMS][BMS + 1], V[BMS][BMS + 1];
     __shared__ double ev[BMS];
@@ -177,7 +223,7 @@ __global__ __launch_bounds__(64) void k_b_eigh(BDev d) {
         const int i = idx / n, j = idx % n;
         if (i < gv && j < gv) continue;
         double acc = 0.0;
-        for (int c = 0; c < d.nch; ++c) acc += d.part[(int64_t(p) * d.nch + c) * BPS + i * BMS + j];
+        for (int c = 0; c < d.npb; ++c) acc += d.part[(int64_t(p) * d.npb + c) * BPS + i * BMS + j];
         s.G[i * BMS + j] = acc;
     }
     __syncthreads();
@@ -188,6 +234,12 @@ __global__ __launch_bounds__(64) void k_b_eigh(BDev d) {
         V[i][j] = i == j ? 1.0 : 0.0;
     }
     __syncthreads();
+    // parallel-order Jacobi: each round applies m/2 disjoint rotations at once (round-robin
+    // pairing of m = n rounded up to even indices; pairs with the padding index are skipped), so a
+    // sweep is m - 1 rounds instead of n(n-1)/2 dependent rotations
+    __shared__ double rc[BMS / 2 + 1], rs[BMS / 2 + 1];
+    __shared__ int rp[BMS / 2 + 1], rq[BMS / 2 + 1];
+    const int m = n + (n & 1);
     for (int sweep = 0; sweep < 100; ++sweep) {
         double off = 0.0, tot = 0.0;
         for (int idx = l; idx < n * n; idx += 64) {
@@ -199,31 +251,59 @@ __global__ __launch_bounds__(64) void k_b_eigh(BDev d) {
         off = __shfl(wave_sum(off), 0, 64);
         tot = __shfl(wave_sum(tot), 0, 64);
         if (off == 0.0 || off <= 1e-30 * (tot > 0 ? tot : 1.0)) break;
-        for (int pp = 0; pp < n - 1; ++pp)
-            for (int q = pp + 1; q < n; ++q) {
-                const double apq = A[pp][q];
-                if (fabs(apq) < 1e-300) continue;
-                const double app = A[pp][pp], aqq = A[q][q];
-                const double theta = (aqq - app) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                const double cc = 1.0 / sqrt(t * t + 1.0), sn = t * cc;
-                __syncthreads();
-                if (l < n) {  // columns p, q of row l
+        for (int r = 0; r < m - 1; ++r) {
+            if (l < m / 2) {
+                int p0, q0;
+                if (l == 0) {
+                    p0 = r;
+                    q0 = m - 1;
+                } else {
+                    p0 = (r + l) % (m - 1);
+                    q0 = (r - l + m - 1) % (m - 1);
+                }
+                const int pp = min(p0, q0), q = max(p0, q0);
+                double cc = 1.0, sn = 0.0;
+                if (q < n) {
+                    const double apq = A[pp][q];
+                    if (fabs(apq) >= 1e-300) {
+                        const double theta = (A[q][q] - A[pp][pp]) / (2.0 * apq);
+                        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        cc = 1.0 / sqrt(t * t + 1.0);
+                        sn = t * cc;
+                    }
+                }
+                rc[l] = cc;
+                rs[l] = sn;
+                rp[l] = pp;
+                rq[l] = q < n ? q : pp;  // a pair with the padding index is the identity
+            }
+            __syncthreads();
+            if (l < n) {  // columns p, q of row l, and of V
+                for (int k = 0; k < m / 2; ++k) {
+                    const int pp = rp[k], q = rq[k];
+                    if (pp == q) continue;
+                    const double cc = rc[k], sn = rs[k];
                     const double akp = A[l][pp], akq = A[l][q];
                     A[l][pp] = cc * akp - sn * akq;
                     A[l][q] = sn * akp + cc * akq;
-                }
-                __syncthreads();
-                if (l < n) {  // rows p, q of column l
-                    const double apk = A[pp][l], aqk = A[q][l];
-                    A[pp][l] = cc * apk - sn * aqk;
-                    A[q][l] = sn * apk + cc * aqk;
                     const double vkp = V[l][pp], vkq = V[l][q];
                     V[l][pp] = cc * vkp - sn * vkq;
                     V[l][q] = sn * vkp + cc * vkq;
                 }
-                __syncthreads();
             }
+            __syncthreads();
+            if (l < n) {  // rows p, q of column l
+                for (int k = 0; k < m / 2; ++k) {
+                    const int pp = rp[k], q = rq[k];
+                    if (pp == q) continue;
+                    const double cc = rc[k], sn = rs[k];
+                    const double apk = A[pp][l], aqk = A[q][l];
+                    A[pp][l] = cc * apk - sn * aqk;
+                    A[q][l] = sn * apk + cc * aqk;
+                }
+            }
+            __syncthreads();
+        }
     }
     if (l == 0) {
         for (int i = 0; i < n; ++i) {
@@ -255,7 +335,7 @@ __global__ __launch_bounds__(64) void k_b_eigh(BDev d) {
 __global__ __launch_bounds__(BBS) void k_b_residual(BDev d) {
     __shared__ double red[BBS / 64];
     __shared__ double al[BMS * BNR];
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (s.done) return;
     const int bs = s.bs, NR = d.NR;
@@ -263,13 +343,13 @@ __global__ __launch_bounds__(BBS) void k_b_residual(BDev d) {
     __syncthreads();
     double lam[BNR];
     for (int k = 0; k < BNR; ++k) lam[k] = k < NR ? s.lam[k] : 0.0;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
     const double* B = d.b + int64_t(p) * d.MS * d.nd;
     const double* S = d.sg + int64_t(p) * d.MS * d.nd;
     const double* H = d.hd + int64_t(p) * d.nd;
     double* R = d.r + int64_t(p) * NR * d.nd;
     double nrm[BNR] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) {
+    for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+    for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1; e += BBS) {
         double sa[BNR] = {0.0, 0.0, 0.0, 0.0}, ba[BNR] = {0.0, 0.0, 0.0, 0.0};
         for (int j = 0; j < bs; ++j) {
             const double sj = S[int64_t(j) * d.nd + e], bj = B[int64_t(j) * d.nd + e];
@@ -290,7 +370,7 @@ __global__ __launch_bounds__(BBS) void k_b_residual(BDev d) {
                 R[int64_t(k) * d.nd + e] = fabs(den) > 1.0e-6 ? rk / den : 0.0;
             }
     }
-    double* out = d.part + (int64_t(p) * d.nch + c) * BPS;
+    double* out = d.part + (int64_t(p) * d.npb + blockIdx.x) * BPS;
     for (int k = 0; k < NR; ++k) {
         const double t = block_sum(nrm[k], red);
         if (threadIdx.x == 0) out[k] = t;
@@ -306,7 +386,7 @@ __global__ void k_b_converge(BDev d) {
     int ne = 0, nr = 0;
     for (int k = 0; k < d.NR; ++k) {
         double acc = 0.0;
-        for (int c = 0; c < d.nch; ++c) acc += d.part[(int64_t(p) * d.nch + c) * BPS + k];
+        for (int c = 0; c < d.npb; ++c) acc += d.part[(int64_t(p) * d.npb + c) * BPS + k];
         s.res[k] = sqrt(acc);
         ne += fabs(s.lam[k] - s.lam_old[k]) < d.e_conv;
         nr += s.res[k] < d.r_conv;
@@ -353,14 +433,16 @@ __global__ __launch_bounds__(BBS) void k_b_project(BDev d, double* base, int per
 
 __global__ __launch_bounds__(BBS) void k_b_rnorm_part(BDev d) {
     __shared__ double red[BBS / 64];
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     if (d.st[p].done) return;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
-    double* out = d.part + (int64_t(p) * d.nch + c) * BPS;
+    double* out = d.part + (int64_t(p) * d.npb + blockIdx.x) * BPS;
     for (int k = 0; k < d.NR; ++k) {
         const double* R = d.r + (int64_t(p) * d.NR + k) * d.nd;
         double acc = 0.0;
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) acc += R[e] * R[e];
+        for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+            for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1;
+                 e += BBS)
+                acc += R[e] * R[e];
         acc = block_sum(acc, red);
         if (threadIdx.x == 0) out[k] = acc;
     }
@@ -373,7 +455,7 @@ __global__ void k_b_rnorm(BDev d) {
     if (s.done) return;
     for (int k = 0; k < d.NR; ++k) {
         double acc = 0.0;
-        for (int c = 0; c < d.nch; ++c) acc += d.part[(int64_t(p) * d.nch + c) * BPS + k];
+        for (int c = 0; c < d.npb; ++c) acc += d.part[(int64_t(p) * d.npb + c) * BPS + k];
         s.rnorm[k] = sqrt(acc);
     }
 }
@@ -383,23 +465,28 @@ __global__ void k_b_rnorm(BDev d) {
 __global__ __launch_bounds__(BBS) void k_b_rotate(BDev d, const double* X, int xper, double* out, int oper,
                                                   int final_) {
     __shared__ double al[BMS * BMS];
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (!final_ && (s.done || !s.collapse)) return;
     const int bs = s.bs, kout = final_ ? d.NR : s.cs;
     for (int i = threadIdx.x; i < bs * kout; i += BBS) al[i] = s.alpha[(i / kout) * BMS + i % kout];
     __syncthreads();
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
     const double* Xp = X + int64_t(p) * xper * d.nd;
     double* Op = out + int64_t(p) * oper * d.nd;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) {
+    for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+    for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1; e += BBS) {
         double acc[BMS];
-        for (int k = 0; k < kout; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < BMS; ++k) acc[k] = 0.0;
         for (int j = 0; j < bs; ++j) {
             const double x = Xp[int64_t(j) * d.nd + e];
-            for (int k = 0; k < kout; ++k) acc[k] += al[j * kout + k] * x;
+#pragma unroll
+            for (int k = 0; k < BMS; ++k)
+                if (k < kout) acc[k] += al[j * kout + k] * x;
         }
-        for (int k = 0; k < kout; ++k) Op[int64_t(k) * d.nd + e] = acc[k];
+#pragma unroll
+        for (int k = 0; k < BMS; ++k)
+            if (k < kout) Op[int64_t(k) * d.nd + e] = acc[k];
     }
 }
 
@@ -412,14 +499,16 @@ __global__ void k_b_collapse_begin(BDev d) {
 }
 
 __global__ __launch_bounds__(BBS) void k_b_copy_sigma(BDev d) {
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (s.done || !s.collapse) return;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
     for (int k = 0; k < s.cs; ++k) {
         const double* src = d.tms + (int64_t(p) * d.CS + k) * d.nd;
         double* dst = d.sg + (int64_t(p) * d.MS + k) * d.nd;
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) dst[e] = src[e];
+        for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+            for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1;
+                 e += BBS)
+                dst[e] = src[e];
     }
 }
 
@@ -475,37 +564,56 @@ __global__ void k_b_gs_begin(BDev d, int kind, int k) {
     for (int i = 0; i < BMS; ++i) s.gs_d[i] = 0.0;
 }
 
-// v = a src - sum_{i < row} d_i b_i into row `row`; partial |v|^2 and v . b_i
+// v = a src - sum_{i < row} d_i b_i into row `row`; partial |v|^2 and v . b_i, in one sweep: the
+// basis elements of a position are loaded once into registers and serve both the subtraction and
+// the overlaps of the result
 __global__ __launch_bounds__(BBS) void k_b_gs_pass(BDev d) {
-    __shared__ double red[BBS / 64];
+    __shared__ double red[BBS / 64][BMS + 1];
     __shared__ double dd[BMS];
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (!s.gs_active) return;
     const int row = s.gs_row;
-    for (int i = threadIdx.x; i < row; i += BBS) dd[i] = s.gs_d[i];
+    for (int i = threadIdx.x; i < BMS; i += BBS) dd[i] = i < row ? s.gs_d[i] : 0.0;
     __syncthreads();
     const double a = s.gs_a;
     const double* src = s.gs_src;
     double* B = d.b + int64_t(p) * d.MS * d.nd;
     double* v = B + int64_t(row) * d.nd;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
     double n2 = 0.0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) {
+    double ov[BMS];
+#pragma unroll
+    for (int i = 0; i < BMS; ++i) ov[i] = 0.0;
+    for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+    for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1; e += BBS) {
+        double bv[BMS];
         double x = a * src[e];
-        for (int i = 0; i < row; ++i) x -= dd[i] * B[int64_t(i) * d.nd + e];
+#pragma unroll
+        for (int i = 0; i < BMS; ++i) {
+            bv[i] = i < row ? B[int64_t(i) * d.nd + e] : 0.0;
+            x -= dd[i] * bv[i];
+        }
         v[e] = x;
         n2 += x * x;
+#pragma unroll
+        for (int i = 0; i < BMS; ++i) ov[i] += x * bv[i];
     }
-    double* out = d.part + (int64_t(p) * d.nch + c) * BPS;
-    const double t = block_sum(n2, red);
-    if (threadIdx.x == 0) out[0] = t;
-    for (int i = 0; i < row; ++i) {
-        const double* bi = B + int64_t(i) * d.nd;
-        double acc = 0.0;
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) acc += v[e] * bi[e];
-        acc = block_sum(acc, red);
-        if (threadIdx.x == 0) out[1 + i] = acc;
+    // block sums of |v|^2 and the row overlaps
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    n2 = wave_sum(n2);
+    if (lane == 0) red[w][0] = n2;
+#pragma unroll
+    for (int i = 0; i < BMS; ++i) {
+        if (i >= row) break;
+        const double t = wave_sum(ov[i]);
+        if (lane == 0) red[w][1 + i] = t;
+    }
+    __syncthreads();
+    double* out = d.part + (int64_t(p) * d.npb + blockIdx.x) * BPS;
+    for (int i = threadIdx.x; i <= row; i += BBS) {
+        double t = 0.0;
+        for (int x = 0; x < BBS / 64; ++x) t += red[x][i];
+        out[i] = t;
     }
 }
 
@@ -518,7 +626,7 @@ __global__ __launch_bounds__(64) void k_b_gs_decide(BDev d) {
     const int row = s.gs_row;
     if (l <= row) {
         double acc = 0.0;
-        for (int c = 0; c < d.nch; ++c) acc += d.part[(int64_t(p) * d.nch + c) * BPS + l];
+        for (int c = 0; c < d.npb; ++c) acc += d.part[(int64_t(p) * d.npb + c) * BPS + l];
         sums[l] = acc;
     }
     __syncthreads();
@@ -574,14 +682,15 @@ __global__ void k_b_missing(BDev d) {
 
 // fill_random: tmp_k for k < nmiss from a SplitMix64 stream per problem
 __global__ __launch_bounds__(BBS) void k_b_random(BDev d) {
-    const int p = blockIdx.y, c = blockIdx.x;
+    const int p = blockIdx.y;
     const BState& s = d.st[p];
     if (s.done || s.nmiss <= 0) return;
-    const int64_t e0 = int64_t(c) * d.chl, e1 = min(e0 + d.chl, d.nd);
     for (int k = 0; k < min(s.nmiss, d.TM); ++k) {
         double* t = d.tmp + (int64_t(p) * d.TM + k) * d.nd;
         const unsigned long long seed = s.rng + 0x9E3779B97F4A7C15ULL * (unsigned long long)(s.iters * BNR + k + 1);
-        for (int64_t e = e0 + threadIdx.x; e < e1; e += BBS) {
+        for (int c = blockIdx.x; c < d.nch; c += gridDim.x)
+        for (int64_t e = int64_t(c) * d.chl + threadIdx.x, e1 = min(int64_t(c) * d.chl + d.chl, d.nd); e < e1;
+             e += BBS) {
             unsigned long long z = seed + 0x9E3779B97F4A7C15ULL * (unsigned long long)(e + 1);
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
             z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
@@ -666,6 +775,13 @@ struct BatchClass {
     };
     std::vector<Variant> var;
     int chosen = 0;
+    bool graphs = true;
+    void launch(int vi) {
+        if (graphs)
+            HIPCHK(hipGraphLaunch(var[vi].exec, stream));
+        else
+            iteration(var[vi].chl, var[vi].sb);
+    }
     std::vector<double> ecore;
     int iters_run = 0;
     bool active = true;
@@ -676,6 +792,7 @@ struct BatchClass {
         d.nd = nd;
         d.chl = chl;
         d.nch = int((nd + chl - 1) / chl);
+        d.npb = std::min(d.nch, BNPB);
         d.st = st.p;
         d.b = b.p; d.sg = sg.p; d.r = r.p; d.tmp = tmp.p; d.tms = tms.p;
         d.TM = std::max(CS, std::max(NR, ng));
@@ -700,9 +817,10 @@ struct BatchClass {
     // Gram-Schmidt sequence of one vector (BGS_PASSES passes) for all problems
     void gs_vector(const BDev& d, int kind, int k) {
         const unsigned gp = nb1(P, 64);
+        const int passes = kind == 1 ? BGS_PASSES_COLL : (kind == 0 || kind == 2) ? BGS_PASSES_CORR : BGS_PASSES;
         k_b_gs_begin<<<gp, 64, 0, stream>>>(d, kind, k);
-        for (int it = 0; it < BGS_PASSES; ++it) {
-            k_b_gs_pass<<<dim3(d.nch, P), BBS, 0, stream>>>(d);
+        for (int it = 0; it < passes; ++it) {
+            k_b_gs_pass<<<dim3(d.npb, P), BBS, 0, stream>>>(d);
             k_b_gs_decide<<<P, 64, 0, stream>>>(d);
         }
         k_b_gs_end<<<gp, 64, 0, stream>>>(d);
@@ -731,7 +849,7 @@ struct BatchClass {
     void iteration(int64_t chl, int sbsz) {
         const BDev d = dev(chl);
         const unsigned gp = nb1(P, 64);
-        const dim3 gs(d.nch, P);
+        const dim3 gs(d.npb, P);
         k_b_sig_tab<<<nb1(int64_t(P) * NSL, 64), 64, 0, stream>>>(d, c_tab.p, s_tab.p);
         sigma_items(sbsz);
         k_b_after_sigma<<<gp, 64, 0, stream>>>(d);
@@ -896,8 +1014,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             for (int64_t x : chls)
                 for (int y : sbs) C.var.push_back({x, y});
             C.min_chl = chls.front();
-            const int nch_max = int((nd + C.min_chl - 1) / C.min_chl);
-            C.part.alloc(size_t(P) * nch_max * BPS);
+            C.part.alloc(size_t(P) * BNPB * BPS);
             const int slots = *std::max_element(sbs.begin(), sbs.end());
             C.scratch.alloc(scr * size_t(slots));
             // bad roots
@@ -976,8 +1093,16 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             k_b_guess_end<<<nb1(P, 64), 64, 0, C.stream>>>(d, C.ng);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(C.stream));
-            // capture one iteration per candidate (kernel attributes are set outside the capture)
+            // capture one iteration per candidate (kernel attributes are set outside the capture);
+            // FCI_BATCH_GRAPHS=0 launches the same sequence directly each iteration instead
+            // (profilers that cannot follow graph replays)
             batch_sigma_prepare(C.sa, C.sb);
+            static const bool use_graphs = [] {
+                const char* e = std::getenv("FCI_BATCH_GRAPHS");
+                return !e || std::atoi(e) != 0;
+            }();
+            C.graphs = use_graphs;
+            if (use_graphs)
             for (auto& v : C.var) {
                 hipGraph_t g;
                 HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
@@ -1003,7 +1128,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             HIPCHK(hipEventCreate(&e1));
             for (auto& v : C.var) {
                 HIPCHK(hipEventRecord(e0, C.stream));
-                for (int r = 0; r < 2; ++r) HIPCHK(hipGraphLaunch(v.exec, C.stream));
+                for (int r = 0; r < 2; ++r) C.launch(int(&v - C.var.data()));
                 HIPCHK(hipEventRecord(e1, C.stream));
                 HIPCHK(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1026,7 +1151,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 if (!C.active) continue;
                 any = true;
                 for (int r = 0; r < kPoll && C.iters_run < C.maxiter; ++r, ++C.iters_run)
-                    HIPCHK(hipGraphLaunch(C.var[C.chosen].exec, C.stream));
+                    C.launch(C.chosen);
                 HIPCHK(hipMemcpyAsync(C.done_host, C.done_dev.p, sizeof(int), hipMemcpyDeviceToHost, C.stream));
             }
             if (!any) break;
@@ -1044,7 +1169,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
         for (auto& c : classes) {
             BatchClass& C = *c;
             const BDev d = C.dev(C.min_chl);
-            const dim3 gs(d.nch, C.P);
+            const dim3 gs(d.npb, C.P);
             // get_results (davidson_liu_solver.cc:488-501): Ritz vectors, orthonormalized into b
             k_b_rotate<<<gs, BBS, 0, C.stream>>>(d, d.b, C.MS, d.tmp, d.TM, 1);
             k_b_final_begin<<<nb1(C.P, 64), 64, 0, C.stream>>>(d);
