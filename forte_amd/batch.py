@@ -45,6 +45,15 @@ def _solve_one(n, seed, nroot, options):
                 converged=st["converged"])
 
 
+def solve_many(ints, na, nb, nroot=1, multiplicity=1, options=None, want_evecs=False):
+    """Solve independent problems in one batched call (fci_batch_solve): problem i has the
+    ActiveSpaceIntegrals ints[i] and na[i] / nb[i] electrons; options take FCISolver's option
+    names (DL_*, E_CONVERGENCE, R_CONVERGENCE). Returns dict(energies [P, nroot], iterations,
+    converged, stats[, evecs])."""
+    return _C.batch_solve(list(ints), [int(x) for x in na], [int(x) for x in nb], nroot, multiplicity,
+                          dict(options or {}), want_evecs)
+
+
 def solve_batch(problems, nroot=2, options=None, want_evecs=False):
     """Solve [(n, seed)] seeded CAS(n, n) problems (Ms = 0, singlets) in one batched call.
     Returns (results in input order, wall seconds, batch statistics)."""

@@ -8,8 +8,9 @@
  * a negative status on failure; fci_last_error() then holds the message. No C++ exception
  * crosses this boundary. Output buffers are owned by the caller.
  *
- * Scope: one irrep (C1), all active orbitals, any na / nb; n <= 22 orbitals for the string tables
- * and n <= 19 for sigma (the alpha-beta kernels hold at most 12 row tiles of 16 packed pairs).
+ * Scope: all orbitals active (doubly occupied ones are folded into the integrals by the caller),
+ * any na / nb, C1 or one irrep of a point group (fci_set_symmetry); n <= 22 orbitals for the string
+ * tables and n <= 19 for sigma (the alpha-beta kernels hold at most 12 row tiles of 16 packed pairs).
  * The same-spin 2-RDM needs cliques of at most 96 members, C(n - k + 2, 2) <= 96 for k electrons
  * of a spin (so CAS(16,16) works, but 2 electrons in 16 orbitals do not). Integrals use Forte's
  * layout (forte/integrals/active_space_integrals.h:103-127): oei n*n row-major, tei n^4

@@ -349,3 +349,33 @@ def test_threaded_solves_match_oracle(oracle):
         o = oracle.Oracle.from_chem(h, eri, n // 2, n // 2)
         ref = o.solve(nroot=2, e_conv=1e-11, r_conv=1e-7, maxiter=300)["energies"]
         assert np.abs(np.array(r["energies"]) - ref).max() < 1e-9, (n, seed)
+
+
+def test_fp64_sigma_run_to_run():
+    """The FP64 alpha-beta row sums and the alpha-beta 2-RDM add in scheduling order (LDS and
+    global FP64 atomics), so repeated evaluations may differ in the last bits; bound that
+    difference (Forte's sums run in a fixed order)."""
+    import forte_amd
+    p = seeded_problem(12, 6, 6, 11)
+    g = gpu_solver(p)
+    C = np.random.default_rng(5).standard_normal(g.ndet())
+    C /= np.linalg.norm(C)
+    c = forte_amd.DeviceArray(g.ndet(), "float64")
+    s = forte_amd.DeviceArray(g.ndet(), "float64")
+    c.upload(C)
+    outs = []
+    for _ in range(3):
+        g.sigma_arrays(c, s)
+        outs.append(s.download())
+    norm = np.linalg.norm(outs[0])
+    d = max(np.abs(o - outs[0]).max() for o in outs[1:])
+    say(f"  FP64 sigma run-to-run max difference {d:.2e} (|sigma| {norm:.3e})")
+    assert d <= 1e-14 * norm
+    g.set_e_convergence(1e-10)
+    g.compute_energy()
+    r1 = g.rdms([(0, 0)], 2)[0]
+    r2 = g.rdms([(0, 0)], 2)[0]
+    dr = max(np.abs(a - b).max() for a, b in zip([r1.g1a(), r1.g2aa(), r1.g2ab(), r1.g2bb()],
+                                                 [r2.g1a(), r2.g2aa(), r2.g2ab(), r2.g2bb()]))
+    say(f"  RDM run-to-run max difference {dr:.2e}")
+    assert dr <= 1e-13
