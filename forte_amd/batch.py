@@ -88,3 +88,14 @@ def solve_batch_threads(problems, nroot=2, threads=16, options=None):
         futs = [ex.submit(_solve_one, n, seed, nroot, options) for n, seed in problems]
         res = [f.result() for f in futs]
     return res, time.perf_counter() - t0
+
+
+if __name__ == "__main__":
+    # python -m forte_amd.batch [count]: time the seeded batch (a warm-up call on 8 problems first)
+    import sys
+    probs = seeded_batch(int(sys.argv[1]) if len(sys.argv) > 1 else 512)
+    opts = {"E_CONVERGENCE": 1e-9, "R_CONVERGENCE": 1e-5}
+    solve_batch(probs[:8], nroot=2, options=opts)
+    res, wall, st = solve_batch(probs, nroot=2, options=opts)
+    print(f"{len(probs)} problems in {wall:.3f} s, all converged: {all(r['converged'] for r in res)}", flush=True)
+    print(st, flush=True)

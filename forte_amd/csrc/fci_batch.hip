@@ -912,6 +912,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
         if (MS0 > BMS) throw Error(FCI_ERR_ARG, "batched solves support subspaces of at most 32 vectors");
         if (opts->precision != FCI_PREC_FP64 || opts->preconditioner != FCI_PRECOND_DIAGONAL)
             throw Error(FCI_ERR_ARG, "batched solves run in FP64 with the diagonal preconditioner");
+        if (opts->spin_adapt) throw Error(FCI_ERR_ARG, "batched solves run in the determinant basis (no CI_SPIN_ADAPT)");
         // group problems by shape
         std::map<std::tuple<int, int, int>, std::vector<int>> groups;
         for (int i = 0; i < nprob; ++i) {

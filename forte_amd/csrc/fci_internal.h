@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -232,6 +233,57 @@ void transpose_device(const T* in, T* out, int64_t rows, int64_t cols, hipStream
 template <class T>
 void transpose_add_device(const T* in, T* out, int64_t rows, int64_t cols, const T* base,
                           T base_scale, hipStream_t st);
+
+// ---- CSF spin adaptation (fci_csf.hip) -----------------------------------------------------------
+// a determinant as its alpha / beta occupation bits
+struct Det {
+    uint32_t a, b;
+    bool operator==(const Det& o) const { return a == o.a && b == o.b; }
+};
+
+// configurations with N open shells: their masks, CSF offsets and the shared table T_N
+struct CsfClass {
+    int N = 0, ncsf = 0, ndp = 0;  // open shells, CSFs and determinant patterns per configuration
+    int64_t nconf = 0;
+    // host copies: docc (bit 31: the configuration's sign), socc, CSF offset; pattern masks
+    // (bit k: k-th lowest open shell beta); T_N by rows (CSF) and by columns (pattern)
+    std::vector<uint32_t> h_docc, h_socc, h_occ, h_rj, h_ci;
+    std::vector<int64_t> h_off;
+    std::vector<int> h_rp, h_cp;
+    std::vector<double> h_rv, h_cv;
+    DBuf<uint32_t> docc, socc, occ, rj, ci;
+    DBuf<int64_t> off;
+    DBuf<int> rp, cp;
+    DBuf<double> rv, cv;
+};
+
+// Forte's SpinAdapter (sparse_ci/ci_spin_adaptation.cc) over the C1 determinant layout
+// C[Ia * nbs + Ib]: CSFs of spin S (twoS = 2S) in the reference's order, optionally restricted to
+// the determinants of one irrep
+class CsfSpace {
+  public:
+    CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbsym, int irrep, const uint32_t* d_binom,
+             int64_t nbs);
+    int64_t ncsf() const { return ncsf_; }
+    int64_t ncoupling() const { return ncoupling_; }
+    int64_t nconf() const { return nconf_; }
+    int twoS() const { return twoS_; }
+    // det_C_to_csf_C (ci_spin_adaptation.cc:89-104); TI = TO = double or float
+    template <class TI, class TO> void to_csf(const TI* det, TO* csf, hipStream_t st) const;
+    // csf_C_to_det_C (ci_spin_adaptation.cc:106-121); determinants outside the CSFs are zeroed
+    template <class TO> void to_det(const double* csf, TO* det, int64_t ndet, hipStream_t st) const;
+    // sum_j c_j^2 H_jj per CSF
+    void hdiag(const double* hdet, double* hcsf, hipStream_t st) const;
+    // host walk over every CSF: (CSF index, its determinants with coefficients)
+    void for_each_csf(const std::function<void(int64_t, const std::vector<std::pair<Det, double>>&)>& f) const;
+
+  private:
+    int n_, na_, nb_, twoS_;
+    int64_t nbs_;
+    const uint32_t* binom_;
+    int64_t ncsf_ = 0, ncoupling_ = 0, nconf_ = 0;
+    std::vector<CsfClass> cls_;
+};
 
 // ---- RDMs ---------------------------------------------------------------------------------------
 void rdm1_device(const SpinTables& a, const SpinTables& b, const double* L, const double* R,

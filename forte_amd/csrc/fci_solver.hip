@@ -147,10 +147,6 @@ static double slater_rules(const HostInts& I, uint32_t la, uint32_t lb, uint32_t
     return me;
 }
 
-struct Det {
-    uint32_t a, b;
-    bool operator==(const Det& o) const { return a == o.a && b == o.b; }
-};
 struct DetHash {
     size_t operator()(const Det& d) const { return (size_t(d.a) << 32) ^ d.b; }
 };
@@ -410,12 +406,13 @@ class Solver {
         HIPCHK(hipStreamSynchronize(0));
         sym_ = true;
         dl_.reset();  // the next compute_energy builds a guess in the new sector
+        csf_.reset();
     }
     // eigenvector `root` restricted to the target irrep, in Forte's block order: alpha irrep ha
     // outer, then alpha address within ha, then beta address within ha ^ irrep (fci_vector.cc:92-115)
     std::vector<double> evec_sector(int root) {
         std::vector<double> full(ndet_);
-        HIPCHK(hipMemcpy(full.data(), dl_->eigenvector_device(root), ndet_ * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(full.data(), evec_device(root), ndet_ * sizeof(double), hipMemcpyDeviceToHost));
         if (!sym_) return full;
         const std::vector<uint8_t> ia = syma_.to_host(), ib = symb_.to_host();
         const std::vector<uint32_t> ra = rela_.to_host(), rb = relb_.to_host();
@@ -448,6 +445,11 @@ class Solver {
     bool have_ints_ = false;
     std::unique_ptr<SigmaEngine> eng_;
     std::unique_ptr<DavidsonLiu> dl_;
+    // CSF spin adaptation: the CSF space of the current multiplicity / irrep, det-basis work
+    // vectors for the sigma builders, and the det-basis eigenvectors of a spin-adapted solve
+    std::unique_ptr<CsfSpace> csf_;
+    bool csf_solved_ = false, dl_csf_ = false;
+    DBuf<double> csf_hd_, csf_wb_, csf_ws_, csf_ev_;
     LearnedPrecond lp_;
     DBuf<double> hdiag_;
     DBuf<float> c32_, s32_;
@@ -457,6 +459,25 @@ class Solver {
     double stat_sec_ = 0.0;
     std::string err_;
 
+    // the CSF space of the solver's multiplicity (and irrep), built on first use
+    CsfSpace& csf_space() {
+        const int twoS = opt_.multiplicity - 1;
+        if (!csf_ || csf_->twoS() != twoS) {
+            csf_.reset();
+            dl_.reset();
+            csf_.reset(new CsfSpace(n_, na_, nb_, twoS, sym_ ? &orbsym_ : nullptr, irrep_, binom_.p, sb_.nstr));
+            HIPCHK(hipStreamSynchronize(0));
+        }
+        return *csf_;
+    }
+    // FCISolver::form_Hdiag_csf (fci_solver_initial_guess.cc:111-149) into csf_hd_
+    void compute_hdiag_csf();
+    // eigenvector `root` in the determinant basis (device); spin-adapted solves convert theirs
+    const double* evec_device(int root) const {
+        if (csf_solved_) return csf_ev_.p + size_t(root) * ndet_;
+        if (!dl_ || dl_csf_) throw Error(-2, "no determinant-basis eigenvector is available");
+        return dl_->eigenvector_device(root);
+    }
     void sigma64(const double* C, double* S) { eng_->apply(C, S, 0); }
     void sigma_mixed(const double* C, double* S);
     void sigma_mixed32(const double* C, float* S);
@@ -621,28 +642,82 @@ struct LPRun {
     }
 };
 
+void Solver::compute_hdiag_csf() {
+    CsfSpace& cs = csf_space();
+    if (csf_hd_.n < size_t(cs.ncsf())) csf_hd_.alloc(cs.ncsf());
+    if (!opt_.spin_adapt_full_preconditioner) {
+        cs.hdiag(hdiag_.p, csf_hd_.p, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        return;
+    }
+    // <CSF|H|CSF> (fci_solver_initial_guess.cc:116-133): the scalar energy plus, over the CSF's
+    // determinants, c_I^2 H_II and 2 c_I c_J H_IJ (I < J) by the Slater rules (host)
+    std::vector<double> h(cs.ncsf());
+    cs.for_each_csf([&](int64_t i, const std::vector<std::pair<Det, double>>& ents) {
+        double e = hi_.e_core;
+        for (size_t I = 0; I < ents.size(); ++I) {
+            const auto& [dI, cI] = ents[I];
+            e += cI * cI * slater_rules(hi_, dI.a, dI.b, dI.a, dI.b);
+            for (size_t J = I + 1; J < ents.size(); ++J) {
+                const auto& [dJ, cJ] = ents[J];
+                e += 2.0 * cI * cJ * slater_rules(hi_, dI.a, dI.b, dJ.a, dJ.b);
+            }
+        }
+        h[i] = e;
+    });
+    csf_hd_.upload(h.data(), h.size());
+}
+
 double Solver::compute_energy() {
     if (!have_ints_) throw Error(-2, "integrals have not been set");
     const bool learned = opt_.preconditioner == FCI_PRECOND_LEARNED;
     if (learned && !lp_.ready()) throw Error(-2, "learned preconditioner weights have not been set");
+    const bool csf = opt_.spin_adapt != 0;
+    if (csf && learned)
+        throw Error(-1, "the learned preconditioner works on determinants; it cannot be combined with CI_SPIN_ADAPT");
     auto t0 = std::chrono::steady_clock::now();
+    solved_ = false;
     const int nroot = opt_.nroot;
+    // the Davidson-Liu basis: CSFs of the multiplicity (fci_solver.cc:186-201) or determinants
+    const int64_t basis = csf ? csf_space().ncsf() : ndet_;
+    if (basis == 0) throw Error(-1, "no CSF has the requested multiplicity");
     bool first_run = false;
-    if (!dl_ || dl_->nroot() != nroot) {
-        dl_.reset(new DavidsonLiu(ndet_, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
+    if (!dl_ || dl_->nroot() != nroot || dl_->size() != basis || dl_csf_ != csf) {
+        dl_.reset(new DavidsonLiu(basis, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
+        dl_csf_ = csf;
         first_run = true;
     }
+    csf_solved_ = false;
     dl_->set_print_level(opt_.print);
-    const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, ndet_));
+    const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, basis));
     compute_hdiag();
-    dl_->add_h_diag_device(hdiag_.p, ndet_);
-    bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= (sym_ ? nsector_ : ndet_);
-    dl_->set_mask(sym_ ? symdet_.p : nullptr);
-    if (first_run || use_initial_guess) {
-        dl_->reset();
-        auto [g, bad] = initial_guess(num_guess_states);
-        dl_->add_guesses(g);
-        dl_->add_project_out_vectors(bad);
+    if (csf) {
+        compute_hdiag_csf();
+        dl_->add_h_diag_device(csf_hd_.p, basis);
+        dl_->set_mask(nullptr);
+        if (csf_wb_.n < size_t(ndet_)) csf_wb_.alloc(ndet_);
+        if (csf_ws_.n < size_t(ndet_)) csf_ws_.alloc(ndet_);
+        if (first_run) {
+            // find_initial_guess_csf (sparse_initial_guess.cc:266-326): the lowest diagonal CSFs
+            std::vector<double> vals;
+            std::vector<int64_t> idx;
+            smallest_k(csf_hd_.p, basis, num_guess_states, vals, idx, 0);
+            std::vector<SparseVec> g;
+            for (int64_t i : idx) g.push_back(SparseVec{{i, 1.0}});
+            dl_->reset();
+            dl_->add_guesses(g);
+            dl_->add_project_out_vectors({});
+        }
+    } else {
+        dl_->add_h_diag_device(hdiag_.p, ndet_);
+        bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= (sym_ ? nsector_ : ndet_);
+        dl_->set_mask(sym_ ? symdet_.p : nullptr);
+        if (first_run || use_initial_guess) {
+            dl_->reset();
+            auto [g, bad] = initial_guess(num_guess_states);
+            dl_->add_guesses(g);
+            dl_->add_project_out_vectors(bad);
+        }
     }
     int nsig64 = 0, nsig = 0, iters = 0;
     bool converged = false;
@@ -667,7 +742,17 @@ double Solver::compute_energy() {
     };
     if (opt_.precision == FCI_PREC_MIXED) {
         // mixed phase: FP32 sigma until the residual reaches mixed_switch or stalls
-        dl_->add_sigma_builder32([this](const double* b, float* s) { sigma_mixed32(b, s); });
+        if (csf) {
+            if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
+            if (s32_.n < size_t(ndet_)) s32_.alloc(ndet_);
+            dl_->add_sigma_builder32([this](const double* b, float* s) {
+                csf_->to_det<float>(b, c32_.p, ndet_, 0);
+                eng_->apply(c32_.p, s32_.p, 0);
+                csf_->to_csf<float, float>(s32_.p, s, 0);
+            });
+        } else {
+            dl_->add_sigma_builder32([this](const double* b, float* s) { sigma_mixed32(b, s); });
+        }
         dl_->set_e_convergence(std::max(opt_.e_convergence, 1e-7));
         dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
         dl_->set_maxiter(opt_.maxiter);
@@ -693,7 +778,16 @@ double Solver::compute_energy() {
         iters += dl_->iterations();
     }
     // FP64 (whole solve, or refinement from the mixed-precision vectors)
-    dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
+    if (csf) {
+        // sigma in the CSF basis through the determinant sigma (fci_solver.cc:246-254)
+        dl_->add_sigma_builder([this](const double* b, double* s) {
+            csf_->to_det<double>(b, csf_wb_.p, ndet_, 0);
+            eng_->apply(csf_wb_.p, csf_ws_.p, 0);
+            csf_->to_csf<double, double>(csf_ws_.p, s, 0);
+        });
+    } else {
+        dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
+    }
     dl_->set_e_convergence(opt_.e_convergence);
     dl_->set_r_convergence(opt_.r_convergence);
     // DL_MAXITER caps the whole solve: the refinement gets what the mixed phase left
@@ -715,11 +809,19 @@ double Solver::compute_energy() {
                     "Davidson-Liu solver did not converge.\nPlease try to increase the number of Davidson-Liu "
                     "iterations (DL_MAXITER). You can also try to increase:\n - the maximum size of the subspace "
                     "(DL_SUBSPACE_PER_ROOT)\n - the number of guess states (DL_GUESS_PER_ROOT)");
+    if (csf) {
+        // the eigenvectors in the determinant basis (copy_state_into_fci_vector, fci_solver.cc:85-96)
+        if (csf_ev_.n < size_t(nroot) * ndet_) csf_ev_.alloc(size_t(nroot) * ndet_);
+        for (int r = 0; r < nroot; ++r)
+            csf_->to_det<double>(dl_->eigenvector_device(r), csf_ev_.p + size_t(r) * ndet_, ndet_, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        csf_solved_ = true;
+    }
     energies_.assign(nroot, 0.0);
     spin2_.assign(nroot, 0.0);
     for (int r = 0; r < nroot; ++r) {
         energies_[r] = dl_->eigenvalues()[r];
-        const double* v = dl_->eigenvector_device(r);
+        const double* v = evec_device(r);
         spin2_[r] = spin2_device(sa_, sb_, v, v, 0);
     }
     solved_ = true;
@@ -743,6 +845,8 @@ int64_t Solver::lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, 
     auto recs = std::make_shared<std::vector<std::unique_ptr<Rec>>>();
     const int nroot = opt_.nroot;
     dl_.reset(new DavidsonLiu(ndet_, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
+    dl_csf_ = false;
+    csf_solved_ = false;
     dl_->set_print_level(opt_.print);
     compute_hdiag();
     dl_->add_h_diag_device(hdiag_.p, ndet_);
@@ -885,6 +989,8 @@ void fci_default_options(fci_options* o) {
     o->precision = FCI_PREC_FP64;
     o->preconditioner = FCI_PRECOND_DIAGONAL;
     o->mixed_switch = 1.0e-4;
+    o->spin_adapt = 0;                      // CI_SPIN_ADAPT (options.yaml)
+    o->spin_adapt_full_preconditioner = 0;  // CI_SPIN_ADAPT_FULL_PRECONDITIONER
 }
 
 int fci_create(int n, int na, int nb, const double* oa, const double* ob, const double* taa, const double* tab,
@@ -960,7 +1066,64 @@ int fci_evec(const fci_solver* s, int root, double* out) {
     return guarded(&self->err, [&] {
         if (!s->s->solved_) throw Error(FCI_ERR_STATE, "compute_energy has not been called");
         if (root < 0 || root >= s->s->dl_->nroot()) throw Error(FCI_ERR_ARG, "root index out of range");
+        HIPCHK(hipMemcpy(out, s->s->evec_device(root), size_t(s->s->ndet_) * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int fci_evec_csf(const fci_solver* s, int root, double* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    auto* self = const_cast<fci_solver*>(s);
+    return guarded(&self->err, [&] {
+        if (!s->s->solved_) throw Error(FCI_ERR_STATE, "compute_energy has not been called");
+        if (!s->s->csf_solved_) throw Error(FCI_ERR_STATE, "the last solve was not spin adapted");
+        if (root < 0 || root >= s->s->dl_->nroot()) throw Error(FCI_ERR_ARG, "root index out of range");
         s->s->dl_->eigenvector_host(root, out);
+    });
+}
+
+int fci_csf_size(fci_solver* s, int64_t* ncsf, int64_t* ncoupling) {
+    if (!s) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::CsfSpace& cs = s->s->csf_space();
+        if (ncsf) *ncsf = cs.ncsf();
+        if (ncoupling) *ncoupling = cs.ncoupling();
+    });
+}
+
+int fci_det_to_csf(fci_solver* s, const double* det, double* csf) {
+    if (!s || !det || !csf) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        auto& S = *s->s;
+        fci::CsfSpace& cs = S.csf_space();
+        fci::DBuf<double> d(S.ndet_), c(std::max<int64_t>(1, cs.ncsf()));
+        d.upload(det, S.ndet_);
+        cs.to_csf<double, double>(d.p, c.p, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        c.download(csf, cs.ncsf());
+    });
+}
+
+int fci_csf_to_det(fci_solver* s, const double* csf, double* det) {
+    if (!s || !det || !csf) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        auto& S = *s->s;
+        fci::CsfSpace& cs = S.csf_space();
+        fci::DBuf<double> d(S.ndet_), c(std::max<int64_t>(1, cs.ncsf()));
+        c.upload(csf, cs.ncsf());
+        cs.to_det<double>(c.p, d.p, S.ndet_, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        d.download(det, S.ndet_);
+    });
+}
+
+int fci_hdiag_csf(fci_solver* s, double* out) {
+    if (!s || !out) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        auto& S = *s->s;
+        if (!S.have_ints_) throw Error(FCI_ERR_STATE, "integrals have not been set");
+        S.compute_hdiag();
+        S.compute_hdiag_csf();
+        S.csf_hd_.download(out, S.csf_space().ncsf());
     });
 }
 
@@ -1034,7 +1197,7 @@ int fci_transition_rdms(fci_solver* left, int rl, fci_solver* right, int rr, int
         const size_t n2 = size_t(A.n_) * A.n_;
         fci::DBuf<double> d1a(n2), d1b(n2);
         // both solvers have the same strings (same n, na, nb): A's tables address B's vector too
-        fci::rdm1_device(A.sa_, A.sb_, A.dl_->eigenvector_device(rl), B.dl_->eigenvector_device(rr), d1a.p, d1b.p, 0);
+        fci::rdm1_device(A.sa_, A.sb_, A.evec_device(rl), B.evec_device(rr), d1a.p, d1b.p, 0);
         HIPCHK(hipStreamSynchronize(0));
         if (g1a) d1a.download(g1a, n2);
         if (g1b) d1b.download(g1b, n2);
@@ -1053,8 +1216,8 @@ int fci_rdms3(fci_solver* s, int rl, int rr, double* g3aaa, double* g3aab, doubl
         const size_t n6 = size_t(n) * n * n * n * n * n;
         // the reference's hole-list loops put the right state's strings on the creation side
         // (fci_vector_rdm.cc:428-433, 492-500): the blocks are <R| a+ a+ a+ a a a |L>
-        const double* Lp = S.dl_->eigenvector_device(rr);
-        const double* Rp = S.dl_->eigenvector_device(rl);
+        const double* Lp = S.evec_device(rr);
+        const double* Rp = S.evec_device(rl);
         const int64_t n3 = int64_t(fci::binom_host(n, 3)), n2p = int64_t(fci::binom_host(n, 2)) * n;
         fci::DBuf<double> c3a(size_t(std::max<int64_t>(1, n3 * n3))), c21a(size_t(std::max<int64_t>(1, n2p * n2p)));
         fci::DBuf<double> c3b(c3a.n), c21b(c21a.n);
@@ -1128,8 +1291,8 @@ int fci_rdms(fci_solver* s, int rl, int rr, int level, double* g1a, double* g1b,
         if (level < 1 || level > 2) throw Error(FCI_ERR_ARG, "FCISolver::rdms: only levels 1 and 2 are implemented");
         const int n = S.n_;
         const size_t n2 = size_t(n) * n, n4 = n2 * n2;
-        const double* L = S.dl_->eigenvector_device(rl);
-        const double* R = S.dl_->eigenvector_device(rr);
+        const double* L = S.evec_device(rl);
+        const double* R = S.evec_device(rr);
         fci::DBuf<double> d1a(n2), d1b(n2);
         fci::rdm1_device(S.sa_, S.sb_, L, R, d1a.p, d1b.p, 0);
         HIPCHK(hipStreamSynchronize(0));

@@ -248,9 +248,9 @@ class FCISolver {
             else if (k == "DL_COLLAPSE_PER_ROOT") o_.collapse_per_root = v.cast<int>();
             else if (k == "DL_SUBSPACE_PER_ROOT") o_.subspace_per_root = v.cast<int>();
             else if (k == "PRINT") o_.print = v.cast<int>();
-            else if (k == "CI_SPIN_ADAPT") {
-                if (v.cast<bool>()) throw std::runtime_error("CI_SPIN_ADAPT is not supported by this solver");
-            } else if (k == "FCI_TEST_RDMS") {
+            else if (k == "CI_SPIN_ADAPT") o_.spin_adapt = v.cast<bool>() ? 1 : 0;
+            else if (k == "CI_SPIN_ADAPT_FULL_PRECONDITIONER") o_.spin_adapt_full_preconditioner = v.cast<bool>() ? 1 : 0;
+            else if (k == "FCI_TEST_RDMS") {
                 // the brute-force RDM self test is the test suite's job here
             } else if (k == "FCI_PRECISION") set_precision(v.cast<std::string>());
             else if (k == "FCI_PRECONDITIONER") set_preconditioner(v.cast<std::string>());
@@ -308,6 +308,38 @@ class FCISolver {
     }
     int64_t ndet() const { return fci_ndet(h_); }
     int64_t nstr(int spin) const { return fci_nstr(h_, spin); }
+
+    // CSF spin adaptation (CI_SPIN_ADAPT; FCISolver::set_spin_adapt, fci_solver.cc:75-79)
+    void set_spin_adapt(bool v) { o_.spin_adapt = v ? 1 : 0; push(); }
+    void set_spin_adapt_full_preconditioner(bool v) { o_.spin_adapt_full_preconditioner = v ? 1 : 0; push(); }
+    std::pair<int64_t, int64_t> csf_size() {
+        int64_t nc = 0, nz = 0;
+        check(fci_csf_size(h_, &nc, &nz), fci_last_error(h_));
+        return {nc, nz};
+    }
+    int64_t ncsf() { return csf_size().first; }
+    py::array_t<double> det_to_csf(py::array_t<double, py::array::c_style | py::array::forcecast> det) {
+        if (det.size() != fci_ndet(h_)) throw std::runtime_error("det_to_csf: vector length is not the number of determinants");
+        py::array_t<double> out(ncsf());
+        check(fci_det_to_csf(h_, det.data(), out.mutable_data()), fci_last_error(h_));
+        return out;
+    }
+    py::array_t<double> csf_to_det(py::array_t<double, py::array::c_style | py::array::forcecast> csf) {
+        if (csf.size() != ncsf()) throw std::runtime_error("csf_to_det: vector length is not the number of CSFs");
+        py::array_t<double> out(fci_ndet(h_));
+        check(fci_csf_to_det(h_, csf.data(), out.mutable_data()), fci_last_error(h_));
+        return out;
+    }
+    py::array_t<double> hdiag_csf() {
+        py::array_t<double> out(ncsf());
+        check(fci_hdiag_csf(h_, out.mutable_data()), fci_last_error(h_));
+        return out;
+    }
+    py::array_t<double> evec_csf(int root) {
+        py::array_t<double> out(ncsf());
+        check(fci_evec_csf(h_, root, out.mutable_data()), fci_last_error(h_));
+        return out;
+    }
 
     // ActiveSpaceMethod::rdms(root_list, max_rdm_level, type), active_space_method.h:134-136
     std::vector<std::shared_ptr<RDMs>> rdms(const std::vector<std::pair<int, int>>& roots, int level, RDMsType type) {
@@ -641,6 +673,14 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("spin2", &FCISolver::spin2)
         .def("stats", &FCISolver::stats)
         .def("evec", &FCISolver::evec)
+        .def("set_spin_adapt", &FCISolver::set_spin_adapt)
+        .def("set_spin_adapt_full_preconditioner", &FCISolver::set_spin_adapt_full_preconditioner)
+        .def("csf_size", &FCISolver::csf_size)
+        .def("ncsf", &FCISolver::ncsf)
+        .def("det_to_csf", &FCISolver::det_to_csf)
+        .def("csf_to_det", &FCISolver::csf_to_det)
+        .def("hdiag_csf", &FCISolver::hdiag_csf)
+        .def("evec_csf", &FCISolver::evec_csf)
         .def("ndet", &FCISolver::ndet)
         .def("nstr", &FCISolver::nstr)
         .def("rdms", &FCISolver::rdms, py::arg("root_list"), py::arg("max_rdm_level"),

@@ -68,6 +68,8 @@ typedef struct fci_options {
     int precision;           /* enum fci_precision */
     int preconditioner;      /* enum fci_preconditioner */
     double mixed_switch;     /* mixed precision: residual norm at which FP64 refinement starts */
+    int spin_adapt;          /* CI_SPIN_ADAPT: Davidson-Liu in the CSF basis of the multiplicity */
+    int spin_adapt_full_preconditioner; /* CI_SPIN_ADAPT_FULL_PRECONDITIONER: <CSF|H|CSF> diagonal */
 } fci_options;
 
 void fci_default_options(fci_options* opts);
@@ -141,6 +143,21 @@ int fci_rdms3(fci_solver* s, int root_left, int root_right, double* g3aaa, doubl
  * computed.) */
 int fci_transition_rdms(fci_solver* left, int root_left, fci_solver* right, int root_right, int level,
                         double* g1a, double* g1b);
+/* ---- CSF spin adaptation (CI_SPIN_ADAPT; sparse_ci/ci_spin_adaptation.cc, fci_solver.cc:135-141,
+ * 240-263). The CSFs are those of the solver's multiplicity (and irrep, with fci_set_symmetry) in
+ * the reference's order: configurations sorted by (open-shell mask, closed-shell mask), genealogical
+ * couplings per configuration. Determinant vectors use the C1 layout. */
+/* SpinAdapter::ncsf() and the number of nonzero CSF-determinant couplings */
+int fci_csf_size(fci_solver* s, int64_t* ncsf, int64_t* ncoupling);
+/* SpinAdapter::det_C_to_csf_C / csf_C_to_det_C (ci_spin_adaptation.cc:89-121), host buffers */
+int fci_det_to_csf(fci_solver* s, const double* det, double* csf);
+int fci_csf_to_det(fci_solver* s, const double* csf, double* det);
+/* FCISolver::form_Hdiag_csf (fci_solver_initial_guess.cc:111-149), honouring
+ * spin_adapt_full_preconditioner; ncsf doubles */
+int fci_hdiag_csf(fci_solver* s, double* out);
+/* FCISolver::evecs() row `root` of a spin-adapted solve: the eigenvector in the CSF basis */
+int fci_evec_csf(const fci_solver* s, int root, double* out);
+
 /* FCIVector::compute_spin2 (fci_vector_spin2.cc:38-75) of a host vector */
 int fci_spin2_of(fci_solver* s, const double* C, double* spin2);
 
