@@ -76,6 +76,253 @@ __global__ void k_csf_gather(const TI* __restrict__ det, TO* __restrict__ csf, c
     csf[off[c] + i] = TO((dw & kConfNeg) ? -acc : acc);
 }
 
+// det -> csf with the determinants of `cpb` consecutive configurations staged in LDS (signed by
+// their configuration factor), so each determinant address is computed once per block. Block
+// (x, y) covers configurations [x cpb, x cpb + cpb) and CSF rows [y rpb, y rpb + rpb). The row
+// table T_N is read once per row for kCsfCT configurations at a time; rows are spread by `mode`:
+//   0: (group of kCsfCT configurations, row) pairs over the threads (few CSFs per configuration)
+//   1: one row per thread
+//   2: one row per wave, its nonzeros over the lanes (thousands per row at 12+ open shells)
+constexpr int kCsfCT = 4;
+__device__ __forceinline__ double csf_wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class TI, class TO>
+__global__ __launch_bounds__(256) void k_csf_gather_lds(const TI* __restrict__ det, TO* __restrict__ csf,
+                                                        const uint32_t* __restrict__ docc,
+                                                        const uint32_t* __restrict__ socc,
+                                                        const int64_t* __restrict__ off, int64_t nconf, int cpb,
+                                                        int rpb, int mode, int ncsf, int ndp,
+                                                        const int* __restrict__ rp, const uint32_t* __restrict__ rj,
+                                                        const double* __restrict__ rv, const uint32_t* __restrict__ occ,
+                                                        int n, int64_t nbs, const uint32_t* __restrict__ B) {
+    extern __shared__ double xs[];  // [cpb][ndp]
+    const int64_t c0 = int64_t(blockIdx.x) * cpb;
+    const int nc = int(min(int64_t(cpb), nconf - c0));
+    const int r0 = blockIdx.y * rpb, r1 = min(ncsf, r0 + rpb);
+    for (int t = threadIdx.x; t < nc * ndp; t += blockDim.x) {
+        const int cc = t / ndp, j = t - cc * ndp;
+        const uint32_t dw = docc[c0 + cc];
+        uint32_t a, b;
+        csf_strings(dw & ~kConfNeg, socc[c0 + cc], occ[j], a, b);
+        const double v = double(det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)]);
+        xs[t] = (dw & kConfNeg) ? -v : v;
+    }
+    __syncthreads();
+    const int ng = (nc + kCsfCT - 1) / kCsfCT;
+    if (mode == 2) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int i = r0 + wave; i < r1; i += nw) {
+            const int e0 = rp[i], e1 = rp[i + 1];
+            for (int g = 0; g < ng; ++g) {
+                const int cb = g * kCsfCT, kk = min(kCsfCT, nc - cb);
+                const double* x0 = xs + cb * ndp;
+                double acc[kCsfCT];
+#pragma unroll
+                for (int k = 0; k < kCsfCT; ++k) acc[k] = 0.0;
+                for (int e = e0 + lane; e < e1; e += 64) {
+                    const double v = rv[e];
+                    const uint32_t jj = rj[e];
+#pragma unroll
+                    for (int k = 0; k < kCsfCT; ++k)
+                        if (k < kk) acc[k] += v * x0[k * ndp + jj];
+                }
+#pragma unroll
+                for (int k = 0; k < kCsfCT; ++k) {
+                    const double t = csf_wave_sum(acc[k]);
+                    if (lane == 0 && k < kk) csf[off[c0 + cb + k] + i] = TO(t);
+                }
+            }
+        }
+        return;
+    }
+    // modes 0 and 1: work item = (group of configurations, row)
+    const int nr = r1 - r0;
+    const int items = mode == 0 ? ng * nr : nr;
+    for (int t = threadIdx.x; t < items; t += blockDim.x) {
+        const int i = r0 + (mode == 0 ? t % nr : t);
+        const int g0 = mode == 0 ? t / nr : 0, g1 = mode == 0 ? g0 + 1 : ng;
+        const int e0 = rp[i], e1 = rp[i + 1];
+        for (int g = g0; g < g1; ++g) {
+            const int cb = g * kCsfCT, kk = min(kCsfCT, nc - cb);
+            const double* x0 = xs + cb * ndp;
+            double acc[kCsfCT];
+#pragma unroll
+            for (int k = 0; k < kCsfCT; ++k) acc[k] = 0.0;
+            for (int e = e0; e < e1; ++e) {
+                const double v = rv[e];
+                const uint32_t jj = rj[e];
+#pragma unroll
+                for (int k = 0; k < kCsfCT; ++k)
+                    if (k < kk) acc[k] += v * x0[k * ndp + jj];
+            }
+#pragma unroll
+            for (int k = 0; k < kCsfCT; ++k)
+                if (k < kk) csf[off[c0 + cb + k] + i] = TO(acc[k]);
+        }
+    }
+}
+
+// Classes whose dense T_N fits in LDS beside the determinants of `cpb` configurations (up to 10
+// open shells: 42 x 252 doubles for singlets) use dense products: no index loads, every T_N
+// element and every staged determinant read from LDS, 2 x 2 outputs per thread.
+constexpr size_t kCsfDenseBytes = 88u << 10;   // largest dense T_N
+constexpr size_t kCsfDenseTotal = 144u << 10;  // LDS of a dense-path block
+template <class TI, class TO>
+__global__ __launch_bounds__(256) void k_csf_gather_dense(const TI* __restrict__ det, TO* __restrict__ csf,
+                                                          const uint32_t* __restrict__ docc,
+                                                          const uint32_t* __restrict__ socc,
+                                                          const int64_t* __restrict__ off, int64_t nconf, int cpb,
+                                                          int ncsf, int ndp, const double* __restrict__ td,
+                                                          const uint32_t* __restrict__ occ, int n, int64_t nbs,
+                                                          const uint32_t* __restrict__ B) {
+    extern __shared__ double sh[];
+    double* T = sh;                       // [ncsf][ndp]
+    double* X = sh + size_t(ncsf) * ndp;  // [cpb][ndp]
+    const int64_t c0 = int64_t(blockIdx.x) * cpb;
+    const int nc = int(min(int64_t(cpb), nconf - c0));
+    for (int t = threadIdx.x; t < ncsf * ndp; t += blockDim.x) T[t] = td[t];
+    for (int t = threadIdx.x; t < cpb * ndp; t += blockDim.x) {
+        const int cc = t / ndp, j = t - cc * ndp;
+        double v = 0.0;
+        if (cc < nc) {
+            const uint32_t dw = docc[c0 + cc];
+            uint32_t a, b;
+            csf_strings(dw & ~kConfNeg, socc[c0 + cc], occ[j], a, b);
+            v = double(det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)]);
+            if (dw & kConfNeg) v = -v;
+        }
+        X[t] = v;
+    }
+    __syncthreads();
+    // 2 x 2 tiles: configurations (cc, cc + 1) x rows (i, i + 1)
+    const int tc = (nc + 1) / 2, ti = (ncsf + 1) / 2;
+    for (int t = threadIdx.x; t < tc * ti; t += blockDim.x) {
+        const int cc = 2 * (t / ti), i = 2 * (t % ti);
+        const bool c2 = cc + 1 < nc, i2 = i + 1 < ncsf;
+        const double* x0 = X + cc * ndp;
+        const double* x1 = X + (c2 ? cc + 1 : cc) * ndp;
+        const double* t0 = T + i * ndp;
+        const double* t1 = T + (i2 ? i + 1 : i) * ndp;
+        double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+        for (int j = 0; j < ndp; ++j) {
+            const double u0 = t0[j], u1 = t1[j], v0 = x0[j], v1 = x1[j];
+            a00 += u0 * v0;
+            a01 += u1 * v0;
+            a10 += u0 * v1;
+            a11 += u1 * v1;
+        }
+        const int64_t o0 = off[c0 + cc];
+        csf[o0 + i] = TO(a00);
+        if (i2) csf[o0 + i + 1] = TO(a01);
+        if (c2) {
+            const int64_t o1 = off[c0 + cc + 1];
+            csf[o1 + i] = TO(a10);
+            if (i2) csf[o1 + i + 1] = TO(a11);
+        }
+    }
+}
+
+// csf -> det for the same classes: det[c][j] = sum_i T[i][j] y[c][i]
+template <class TO>
+__global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restrict__ csf, TO* __restrict__ det,
+                                                           const uint32_t* __restrict__ docc,
+                                                           const uint32_t* __restrict__ socc,
+                                                           const int64_t* __restrict__ off, int64_t nconf, int cpb,
+                                                           int ncsf, int ndp, const double* __restrict__ td,
+                                                           const uint32_t* __restrict__ occ, int n, int64_t nbs,
+                                                           const uint32_t* __restrict__ B) {
+    extern __shared__ double sh[];
+    double* T = sh;                       // [ncsf][ndp]
+    double* Y = sh + size_t(ncsf) * ndp;  // [cpb][ncsf], signed by the configuration factor
+    const int64_t c0 = int64_t(blockIdx.x) * cpb;
+    const int nc = int(min(int64_t(cpb), nconf - c0));
+    for (int t = threadIdx.x; t < ncsf * ndp; t += blockDim.x) T[t] = td[t];
+    for (int t = threadIdx.x; t < cpb * ncsf; t += blockDim.x) {
+        const int cc = t / ncsf, i = t - cc * ncsf;
+        double v = 0.0;
+        if (cc < nc) {
+            v = csf[off[c0 + cc] + i];
+            if (docc[c0 + cc] & kConfNeg) v = -v;
+        }
+        Y[t] = v;
+    }
+    __syncthreads();
+    const int tc = (nc + 1) / 2, tj = (ndp + 1) / 2;
+    for (int t = threadIdx.x; t < tc * tj; t += blockDim.x) {
+        const int cc = 2 * (t / tj), j = 2 * (t % tj);
+        const bool c2 = cc + 1 < nc, j2 = j + 1 < ndp;
+        const double* y0 = Y + cc * ncsf;
+        const double* y1 = Y + (c2 ? cc + 1 : cc) * ncsf;
+        const int jb = j2 ? j + 1 : j;
+        double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+        for (int i = 0; i < ncsf; ++i) {
+            const double u0 = T[i * ndp + j], u1 = T[i * ndp + jb], v0 = y0[i], v1 = y1[i];
+            a00 += u0 * v0;
+            a01 += u1 * v0;
+            a10 += u0 * v1;
+            a11 += u1 * v1;
+        }
+        for (int q = 0; q < (c2 ? 2 : 1); ++q) {
+            const uint32_t dw = docc[c0 + cc + q], so = socc[c0 + cc + q];
+            for (int r = 0; r < (j2 ? 2 : 1); ++r) {
+                uint32_t a, b;
+                csf_strings(dw & ~kConfNeg, so, occ[j + r], a, b);
+                const double v = q == 0 ? (r == 0 ? a00 : a01) : (r == 0 ? a10 : a11);
+                det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)] = TO(v);
+            }
+        }
+    }
+}
+
+// csf -> det with the CSF coefficients of `cpb` configurations staged in LDS: work item =
+// (group of kCsfCT configurations, determinant pattern j), the column j of T_N read once per group
+template <class TO>
+__global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restrict__ csf, TO* __restrict__ det,
+                                                         const uint32_t* __restrict__ docc,
+                                                         const uint32_t* __restrict__ socc,
+                                                         const int64_t* __restrict__ off, int64_t nconf, int cpb,
+                                                         int ppb, int ncsf, int ndp, const int* __restrict__ cp,
+                                                         const uint32_t* __restrict__ ci, const double* __restrict__ cv,
+                                                         const uint32_t* __restrict__ occ, int n, int64_t nbs,
+                                                         const uint32_t* __restrict__ B) {
+    extern __shared__ double ys[];  // [cpb][ncsf], signed by the configuration factor
+    const int64_t c0 = int64_t(blockIdx.x) * cpb;
+    const int nc = int(min(int64_t(cpb), nconf - c0));
+    for (int t = threadIdx.x; t < nc * ncsf; t += blockDim.x) {
+        const int cc = t / ncsf, i = t - cc * ncsf;
+        const double v = csf[off[c0 + cc] + i];
+        ys[t] = (docc[c0 + cc] & kConfNeg) ? -v : v;
+    }
+    __syncthreads();
+    const int ng = (nc + kCsfCT - 1) / kCsfCT;
+    const int j0 = blockIdx.y * ppb, j1 = min(ndp, j0 + ppb), np = j1 - j0;
+    for (int t = threadIdx.x; t < ng * np; t += blockDim.x) {
+        const int g = t / np, j = j0 + (t - g * np);
+        const int cb = g * kCsfCT, kk = min(kCsfCT, nc - cb);
+        const double* y0 = ys + cb * ncsf;
+        double acc[kCsfCT];
+#pragma unroll
+        for (int k = 0; k < kCsfCT; ++k) acc[k] = 0.0;
+        for (int e = cp[j]; e < cp[j + 1]; ++e) {
+            const double v = cv[e];
+            const uint32_t ii = ci[e];
+#pragma unroll
+            for (int k = 0; k < kCsfCT; ++k)
+                if (k < kk) acc[k] += v * y0[k * ncsf + ii];
+        }
+        const uint32_t m = occ[j];
+#pragma unroll
+        for (int k = 0; k < kCsfCT; ++k)
+            if (k < kk) {
+                uint32_t a, b;
+                csf_strings(docc[c0 + cb + k] & ~kConfNeg, socc[c0 + cb + k], m, a, b);
+                det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)] = TO(acc[k]);
+            }
+    }
+}
+
 // csf -> det: thread per (configuration, determinant pattern); every determinant of the sector
 // belongs to exactly one configuration, so each is written once
 template <class TO>
@@ -119,6 +366,8 @@ __global__ void k_csf_hdiag(const double* __restrict__ hd, double* __restrict__ 
 }
 
 inline unsigned grid_of(int64_t work) { return unsigned((work + 255) / 256); }
+// largest staged determinant row of one configuration (12,870 doubles for 16 open shells)
+constexpr size_t kCsfLdsBytes = 128u << 10;
 
 // all 0/1 sequences of length N with `ones` ones, lexicographic (0 before 1), optionally with
 // position 0 held at 0: what std::next_permutation enumerates from the sorted start
@@ -305,6 +554,13 @@ CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbs
         C.rv.alloc(nz);
         C.ci.alloc(nz);
         C.cv.alloc(nz);
+        if (size_t(C.ncsf) * C.ndp * sizeof(double) <= kCsfDenseBytes) {
+            std::vector<double> dense(size_t(C.ncsf) * C.ndp, 0.0);
+            for (int i = 0; i < C.ncsf; ++i)
+                for (int e = C.h_rp[i]; e < C.h_rp[i + 1]; ++e) dense[size_t(i) * C.ndp + C.h_rj[e]] = C.h_rv[e];
+            C.td.alloc(dense.size());
+            C.td.upload(dense.data(), dense.size());
+        }
         if (!C.h_rj.empty()) {
             C.rj.upload(C.h_rj.data(), C.h_rj.size());
             C.rv.upload(C.h_rv.data(), C.h_rv.size());
@@ -317,8 +573,29 @@ CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbs
 template <class TI, class TO> void CsfSpace::to_csf(const TI* det, TO* csf, hipStream_t st) const {
     for (const auto& C : cls_) {
         if (C.nconf == 0 || C.ncsf == 0) continue;
-        k_csf_gather<TI, TO><<<grid_of(C.nconf * C.ncsf), 256, 0, st>>>(
-            det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ncsf, C.rp.p, C.rj.p, C.rv.p, C.occ.p, n_, nbs_, binom_);
+        const size_t row_bytes = size_t(C.ndp) * sizeof(double);
+        const size_t tbytes = size_t(C.ncsf) * row_bytes;
+        if (C.td.p && tbytes + 8 * row_bytes <= kCsfDenseTotal) {
+            const int cpb = int(std::min<size_t>(64, (kCsfDenseTotal - tbytes) / row_bytes));
+            k_csf_gather_dense<TI, TO><<<unsigned((C.nconf + cpb - 1) / cpb), 256, tbytes + size_t(cpb) * row_bytes, st>>>(
+                det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_);
+        } else if (row_bytes <= kCsfLdsBytes) {
+            // as many configurations per block as fit in 96 KB of LDS (at least one); rows split
+            // over blocks so that classes with few configurations still fill the chip
+            const int cpb = int(std::max<size_t>(1, std::min<size_t>(64, (96u << 10) / row_bytes)));
+            const int64_t nxb = (C.nconf + cpb - 1) / cpb;
+            const double per_row = double(C.h_rj.size()) / C.ncsf;
+            const int mode = per_row >= 32.0 ? 2 : (C.ncsf < 64 ? 0 : 1);
+            const int rpb = mode == 0 ? C.ncsf : (mode == 2 ? 16 : 256);
+            const unsigned nyb = unsigned((C.ncsf + rpb - 1) / rpb);
+            k_csf_gather_lds<TI, TO><<<dim3(unsigned(nxb), nyb), 256, size_t(cpb) * row_bytes, st>>>(
+                det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, rpb, mode, C.ncsf, C.ndp, C.rp.p, C.rj.p, C.rv.p,
+                C.occ.p, n_, nbs_, binom_);
+        } else {
+            k_csf_gather<TI, TO><<<grid_of(C.nconf * C.ncsf), 256, 0, st>>>(
+                det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ncsf, C.rp.p, C.rj.p, C.rv.p, C.occ.p, n_, nbs_,
+                binom_);
+        }
     }
     HIPCHK(hipGetLastError());
 }
@@ -327,8 +604,27 @@ template <class TO> void CsfSpace::to_det(const double* csf, TO* det, int64_t nd
     HIPCHK(hipMemsetAsync(det, 0, size_t(ndet) * sizeof(TO), st));
     for (const auto& C : cls_) {
         if (C.nconf == 0) continue;
-        k_csf_scatter<TO><<<grid_of(C.nconf * C.ndp), 256, 0, st>>>(
-            csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ndp, C.cp.p, C.ci.p, C.cv.p, C.occ.p, n_, nbs_, binom_);
+        const size_t tbytes = size_t(C.ncsf) * C.ndp * sizeof(double);
+        const size_t ybytes = size_t(std::max(1, C.ncsf)) * sizeof(double);
+        // configurations per block: enough (config group, pattern) items for 256 threads, their
+        // CSF coefficients within 32 KB of LDS; patterns split over blocks when there are few
+        const int want = int(std::max<int64_t>(1, (256 * kCsfCT + C.ndp - 1) / C.ndp));
+        const int cpb = std::max(1, std::min<int>(want, int((32u << 10) / ybytes)));
+        if (C.td.p && tbytes + 8 * ybytes <= kCsfDenseTotal) {
+            const int cpd = int(std::min<size_t>(64, (kCsfDenseTotal - tbytes) / ybytes));
+            k_csf_scatter_dense<TO><<<unsigned((C.nconf + cpd - 1) / cpd), 256, tbytes + size_t(cpd) * ybytes, st>>>(
+                csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpd, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_);
+        } else if (C.ncsf > 0 && ybytes <= (32u << 10)) {
+            const int64_t nxb = (C.nconf + cpb - 1) / cpb;
+            const int ppb = nxb >= 1024 ? C.ndp : 256;
+            k_csf_scatter_lds<TO><<<dim3(unsigned(nxb), unsigned((C.ndp + ppb - 1) / ppb)), 256, size_t(cpb) * ybytes,
+                                    st>>>(csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, ppb, C.ncsf, C.ndp,
+                                          C.cp.p, C.ci.p, C.cv.p, C.occ.p, n_, nbs_, binom_);
+        } else {
+            k_csf_scatter<TO><<<grid_of(C.nconf * C.ndp), 256, 0, st>>>(
+                csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ndp, C.cp.p, C.ci.p, C.cv.p, C.occ.p, n_, nbs_,
+                binom_);
+        }
     }
     HIPCHK(hipGetLastError());
 }

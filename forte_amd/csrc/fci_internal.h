@@ -255,6 +255,7 @@ struct CsfClass {
     DBuf<int64_t> off;
     DBuf<int> rp, cp;
     DBuf<double> rv, cv;
+    DBuf<double> td;  // T_N dense [ncsf][ndp], for classes small enough to stage it in LDS
 };
 
 // Forte's SpinAdapter (sparse_ci/ci_spin_adaptation.cc) over the C1 determinant layout
