@@ -392,8 +392,8 @@ double cg_step(int twoS, int twoM, int dS, int dM) {
 
 }  // namespace
 
-CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbsym, int irrep, const uint32_t* d_binom,
-                   int64_t nbs)
+CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbsym, int irrep, const GasSpec* gas,
+                   const uint32_t* d_binom, int64_t nbs)
     : n_(n), na_(na), nb_(nb), twoS_(twoS), nbs_(nbs), binom_(d_binom) {
     const int twoMs = na - nb;
     if (twoS < 0 || twoS < std::abs(twoMs) || (twoS - twoMs) % 2)
@@ -436,6 +436,14 @@ CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbs
                 uint32_t dm = 0;
                 for (size_t q = 0; q < rest.size(); ++q)
                     if ((sub >> q) & 1u) dm |= 1u << rest[q];
+                if (gas) {  // electrons per GAS space: 2 per closed shell, 1 per open shell
+                    bool ok = true;
+                    for (size_t g = 0; g < gas->orbs.size() && ok; ++g) {
+                        const int e = 2 * __builtin_popcount(dm & gas->orbs[g]) + __builtin_popcount(so & gas->orbs[g]);
+                        ok = e >= gas->gmin[g] && e <= gas->gmax[g];
+                    }
+                    if (!ok) continue;
+                }
                 keys.push_back((uint64_t(so) << 32) | dm);
             }
         }

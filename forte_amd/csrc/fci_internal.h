@@ -258,13 +258,19 @@ struct CsfClass {
     DBuf<double> td;  // T_N dense [ncsf][ndp], for classes small enough to stage it in LDS
 };
 
+// GAS restriction: orbital mask of each space and its electron bounds (both spins)
+struct GasSpec {
+    std::vector<uint32_t> orbs;
+    std::vector<int> gmin, gmax;
+};
+
 // Forte's SpinAdapter (sparse_ci/ci_spin_adaptation.cc) over the C1 determinant layout
 // C[Ia * nbs + Ib]: CSFs of spin S (twoS = 2S) in the reference's order, optionally restricted to
-// the determinants of one irrep
+// the determinants of one irrep and to the configurations a GAS restriction allows
 class CsfSpace {
   public:
-    CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbsym, int irrep, const uint32_t* d_binom,
-             int64_t nbs);
+    CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbsym, int irrep, const GasSpec* gas,
+             const uint32_t* d_binom, int64_t nbs);
     int64_t ncsf() const { return ncsf_; }
     int64_t ncoupling() const { return ncoupling_; }
     int64_t nconf() const { return nconf_; }

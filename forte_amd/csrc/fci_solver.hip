@@ -319,6 +319,38 @@ __global__ void k_sector_mask(const uint8_t* __restrict__ sa, const uint8_t* __r
     if (i < na * nb) out[i] = uint8_t((sa[i / nb] ^ sb[i % nb]) == irrep);
 }
 
+// GAS restriction (GenCI, genci/ci_occupation.cc:64-123): a determinant belongs to the space when
+// every GAS space g holds between gmin[g] and gmax[g] electrons of both spins; ANDed into `space`
+// (1 = allowed), which holds the point-group sector or all ones before
+__global__ void k_gas_mask(const uint32_t* __restrict__ sa, const uint32_t* __restrict__ sb, int64_t na, int64_t nb,
+                           int ngas, const uint32_t* __restrict__ orbs, const int* __restrict__ gmin,
+                           const int* __restrict__ gmax, uint8_t* __restrict__ space) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= na * nb) return;
+    const uint32_t a = sa[i / nb], b = sb[i % nb];
+    bool ok = space[i] != 0;
+    for (int g = 0; g < ngas; ++g) {
+        const int e = __popc(a & orbs[g]) + __popc(b & orbs[g]);
+        ok = ok && e >= gmin[g] && e <= gmax[g];
+    }
+    space[i] = uint8_t(ok);
+}
+__global__ void k_fill_u8(uint8_t* __restrict__ v, int64_t n, uint8_t x) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = x;
+}
+__global__ void k_count_u8(const uint8_t* __restrict__ v, int64_t n, unsigned long long* __restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned long long ballot = __ballot(i < n && v[i] != 0);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)__popcll(ballot));
+}
+// sigma restricted to the space: components outside it are dropped (P H P c for c inside)
+template <class T>
+__global__ void k_space_mask(T* __restrict__ v, const uint8_t* __restrict__ space, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && !space[i]) v[i] = T(0);
+}
+
 class Solver {
   public:
     Solver(int n, int na, int nb) : n_(n), na_(na), nb_(nb) {
@@ -332,6 +364,7 @@ class Solver {
         build_spin_tables(sb_, n, nb, binom_.p, 0);
         HIPCHK(hipStreamSynchronize(0));
         ndet_ = sa_.nstr * sb_.nstr;
+        nspace_ = ndet_;
         eng_.reset(new SigmaEngine(&sa_, &sb_, &di_));
         fci_default_options(&opt_);
     }
@@ -405,8 +438,63 @@ class Solver {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(0));
         sym_ = true;
+        update_space();
         dl_.reset();  // the next compute_energy builds a guess in the new sector
         csf_.reset();
+    }
+    // GenCI's GAS spaces (genci/genci_string_lists.cc:61-110, ci_occupation.cc): gas_of[p] in
+    // [0, ngas) for every active orbital, gmin / gmax electrons of both spins per space; ngas = 0
+    // removes the restriction
+    void set_gas(int ngas, const int* gas_of, const int* gmin, const int* gmax) {
+        if (ngas < 0 || ngas > kMaxGas) throw Error(-1, "GAS: at most 6 spaces");
+        gas_orbs_.assign(ngas, 0u);
+        gas_min_.assign(gmin, gmin + ngas);
+        gas_max_.assign(gmax, gmax + ngas);
+        for (int p = 0; p < n_ && ngas > 0; ++p) {
+            if (gas_of[p] < 0 || gas_of[p] >= ngas) throw Error(-1, "GAS: every orbital needs a space in [0, ngas)");
+            gas_orbs_[gas_of[p]] |= 1u << p;
+        }
+        gas_ = ngas > 0;
+        update_space();
+        dl_.reset();
+        csf_.reset();
+    }
+    // the allowed determinants: point-group sector AND GAS restrictions
+    void update_space() {
+        if (!sym_ && !gas_) {
+            spacedet_.release();
+            nspace_ = ndet_;
+            return;
+        }
+        spacedet_.alloc(ndet_);
+        const unsigned g = unsigned((ndet_ + 255) / 256);
+        if (sym_) HIPCHK(hipMemcpy(spacedet_.p, symdet_.p, ndet_, hipMemcpyDeviceToDevice));
+        else k_fill_u8<<<g, 256>>>(spacedet_.p, ndet_, 1);
+        if (gas_) {
+            const int ng = int(gas_orbs_.size());
+            DBuf<uint32_t> orbs(ng);
+            DBuf<int> mn(ng), mx(ng);
+            orbs.upload(gas_orbs_.data(), ng);
+            mn.upload(gas_min_.data(), ng);
+            mx.upload(gas_max_.data(), ng);
+            k_gas_mask<<<g, 256>>>(sa_.str.p, sb_.str.p, sa_.nstr, sb_.nstr, ng, orbs.p, mn.p, mx.p, spacedet_.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(0));
+        }
+        DBuf<unsigned long long> cnt(1);
+        HIPCHK(hipMemset(cnt.p, 0, sizeof(unsigned long long)));
+        k_count_u8<<<g, 256>>>(spacedet_.p, ndet_, cnt.p);
+        HIPCHK(hipGetLastError());
+        unsigned long long c = 0;
+        cnt.download(&c, 1);
+        nspace_ = int64_t(c);
+        if (nspace_ == 0) throw Error(-1, "no determinant satisfies the symmetry and GAS restrictions");
+    }
+    bool masked() const { return sym_ || gas_; }
+    template <class T> void mask_sigma(T* v) {
+        if (!gas_) return;  // a point-group sector is closed under H by itself
+        k_space_mask<T><<<unsigned((ndet_ + 255) / 256), 256>>>(v, spacedet_.p, ndet_);
+        HIPCHK(hipGetLastError());
     }
     // eigenvector `root` restricted to the target irrep, in Forte's block order: alpha irrep ha
     // outer, then alpha address within ha, then beta address within ha ^ irrep (fci_vector.cc:92-115)
@@ -431,7 +519,12 @@ class Solver {
     fci_options opt_;
     int n_, na_, nb_;
     int64_t ndet_ = 0;
-    bool sym_ = false;
+    static constexpr int kMaxGas = 6;
+    bool sym_ = false, gas_ = false;
+    std::vector<uint32_t> gas_orbs_;
+    std::vector<int> gas_min_, gas_max_;
+    DBuf<uint8_t> spacedet_;
+    int64_t nspace_ = 0;
     int irrep_ = 0;
     int64_t nsector_ = 0;
     std::vector<int> orbsym_;
@@ -465,7 +558,9 @@ class Solver {
         if (!csf_ || csf_->twoS() != twoS) {
             csf_.reset();
             dl_.reset();
-            csf_.reset(new CsfSpace(n_, na_, nb_, twoS, sym_ ? &orbsym_ : nullptr, irrep_, binom_.p, sb_.nstr));
+            GasSpec gs{gas_orbs_, gas_min_, gas_max_};
+            csf_.reset(new CsfSpace(n_, na_, nb_, twoS, sym_ ? &orbsym_ : nullptr, irrep_, gas_ ? &gs : nullptr,
+                                    binom_.p, sb_.nstr));
             HIPCHK(hipStreamSynchronize(0));
         }
         return *csf_;
@@ -521,15 +616,16 @@ void Solver::sigma_mixed32(const double* C, float* S) {
 }
 
 std::pair<std::vector<SparseVec>, std::vector<SparseVec>> Solver::initial_guess(int num_guess_states) {
-    if (!sym_)
+    if (!masked())
         return initial_guess_dets(hdiag_.p, ndet_, ndet_, sb_.nstr, n_, na_, nb_, hi_, opt_.multiplicity,
                                   opt_.ndets_per_guess, num_guess_states, 0);
-    // with point-group symmetry only the determinants of the target irrep are candidates (the
-    // reference's FCIVector holds that block only): the others get an infinite diagonal here
+    // with point-group symmetry or GAS only the determinants of the space are candidates (the
+    // reference's vectors hold those blocks only): the others get an infinite diagonal here. Spin
+    // completion keeps the spatial occupations, so it stays inside the space.
     DBuf<double> hsel(ndet_);
-    k_sector_select<<<unsigned((ndet_ + 255) / 256), 256>>>(hdiag_.p, symdet_.p, ndet_, hsel.p);
+    k_sector_select<<<unsigned((ndet_ + 255) / 256), 256>>>(hdiag_.p, spacedet_.p, ndet_, hsel.p);
     HIPCHK(hipGetLastError());
-    return initial_guess_dets(hsel.p, ndet_, nsector_, sb_.nstr, n_, na_, nb_, hi_, opt_.multiplicity,
+    return initial_guess_dets(hsel.p, ndet_, nspace_, sb_.nstr, n_, na_, nb_, hi_, opt_.multiplicity,
                               opt_.ndets_per_guess, num_guess_states, 0);
 }
 
@@ -710,8 +806,8 @@ double Solver::compute_energy() {
         }
     } else {
         dl_->add_h_diag_device(hdiag_.p, ndet_);
-        bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= (sym_ ? nsector_ : ndet_);
-        dl_->set_mask(sym_ ? symdet_.p : nullptr);
+        bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= nspace_;
+        dl_->set_mask(masked() ? spacedet_.p : nullptr);
         if (first_run || use_initial_guess) {
             dl_->reset();
             auto [g, bad] = initial_guess(num_guess_states);
@@ -751,7 +847,10 @@ double Solver::compute_energy() {
                 csf_->to_csf<float, float>(s32_.p, s, 0);
             });
         } else {
-            dl_->add_sigma_builder32([this](const double* b, float* s) { sigma_mixed32(b, s); });
+            dl_->add_sigma_builder32([this](const double* b, float* s) {
+                sigma_mixed32(b, s);
+                mask_sigma(s);
+            });
         }
         dl_->set_e_convergence(std::max(opt_.e_convergence, 1e-7));
         dl_->set_r_convergence(std::max(opt_.r_convergence, opt_.mixed_switch));
@@ -786,7 +885,10 @@ double Solver::compute_energy() {
             csf_->to_csf<double, double>(csf_ws_.p, s, 0);
         });
     } else {
-        dl_->add_sigma_builder([this](const double* b, double* s) { sigma64(b, s); });
+        dl_->add_sigma_builder([this](const double* b, double* s) {
+            sigma64(b, s);
+            mask_sigma(s);
+        });
     }
     dl_->set_e_convergence(opt_.e_convergence);
     dl_->set_r_convergence(opt_.r_convergence);
@@ -1134,6 +1236,13 @@ int fci_set_symmetry(fci_solver* s, const int* orbsym, int irrep) {
     if (!s || !orbsym) return FCI_ERR_ARG;
     return guarded(&s->err, [&] { s->s->set_symmetry(orbsym, irrep); });
 }
+
+int fci_set_gas(fci_solver* s, int ngas, const int* gas_of_orbital, const int* gas_min, const int* gas_max) {
+    if (!s || (ngas > 0 && (!gas_of_orbital || !gas_min || !gas_max))) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] { s->s->set_gas(ngas, gas_of_orbital, gas_min, gas_max); });
+}
+
+int64_t fci_space_size(const fci_solver* s) { return s ? s->s->nspace_ : -1; }
 
 int64_t fci_sector_size(const fci_solver* s) {
     if (!s) return -1;

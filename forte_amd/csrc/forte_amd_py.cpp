@@ -309,6 +309,17 @@ class FCISolver {
     int64_t ndet() const { return fci_ndet(h_); }
     int64_t nstr(int spin) const { return fci_nstr(h_, spin); }
 
+    // GenCI GAS spaces: gas_of_orbital per active orbital, electron bounds per space
+    void set_gas(const std::vector<int>& gas_of_orbital, const std::vector<int>& gas_min,
+                 const std::vector<int>& gas_max) {
+        if (gas_min.size() != gas_max.size()) throw std::runtime_error("set_gas: gas_min and gas_max differ in length");
+        if (!gas_min.empty() && gas_of_orbital.size() != size_t(ints_->n))
+            throw std::runtime_error("set_gas: one GAS index per active orbital is needed");
+        check(fci_set_gas(h_, int(gas_min.size()), gas_of_orbital.data(), gas_min.data(), gas_max.data()),
+              fci_last_error(h_));
+    }
+    int64_t space_size() const { return fci_space_size(h_); }
+
     // CSF spin adaptation (CI_SPIN_ADAPT; FCISolver::set_spin_adapt, fci_solver.cc:75-79)
     void set_spin_adapt(bool v) { o_.spin_adapt = v ? 1 : 0; push(); }
     void set_spin_adapt_full_preconditioner(bool v) { o_.spin_adapt_full_preconditioner = v ? 1 : 0; push(); }
@@ -673,6 +684,8 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("spin2", &FCISolver::spin2)
         .def("stats", &FCISolver::stats)
         .def("evec", &FCISolver::evec)
+        .def("set_gas", &FCISolver::set_gas, py::arg("gas_of_orbital"), py::arg("gas_min"), py::arg("gas_max"))
+        .def("space_size", &FCISolver::space_size)
         .def("set_spin_adapt", &FCISolver::set_spin_adapt)
         .def("set_spin_adapt_full_preconditioner", &FCISolver::set_spin_adapt_full_preconditioner)
         .def("csf_size", &FCISolver::csf_size)

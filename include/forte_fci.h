@@ -101,6 +101,16 @@ int fci_string_classes(fci_solver* s, int spin, int* irrep, int64_t* address);
  * (FCIVector::copy_to, fci_vector.cc:108-169) */
 int fci_evec_sector(fci_solver* s, int root, double* out);
 
+/* GenCI's GAS restriction (genci/genci_solver.cc, genci/ci_occupation.cc:64-123, the GASn /
+ * GASnMIN / GASnMAX options): ngas <= 6 spaces, gas_of_orbital[n] in [0, ngas) for every active
+ * orbital, gas_min / gas_max electrons (both spins) per space. The solver then works in the
+ * determinants that satisfy every bound (and the irrep of fci_set_symmetry); vectors keep the C1
+ * layout and are zero outside. ngas = 0 removes the restriction. sigma here runs over all
+ * determinants and is restricted afterwards (the FCI cost). */
+int fci_set_gas(fci_solver* s, int ngas, const int* gas_of_orbital, const int* gas_min, const int* gas_max);
+/* determinants of the space (sector and GAS restrictions; fci_ndet without either) */
+int64_t fci_space_size(const fci_solver* s);
+
 /* FCISolver::set_options, fci_solver.cc:156-172 */
 int fci_set_options(fci_solver* s, const fci_options* opts);
 /* ActiveSpaceMethod::set_active_space_integrals, active_space_method.h:255 (keeps the

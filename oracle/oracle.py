@@ -127,6 +127,20 @@ def sector_mask(o, orbsym, irrep):
     return ((sa[:, None] ^ sb[None, :]) == irrep).ravel()
 
 
+def gas_mask(n, na, nb, gas_of_orbital, gas_min, gas_max):
+    """Determinants (C1 order) of GenCI's GAS space: every GAS space g holds between gas_min[g] and
+    gas_max[g] electrons of both spins (the occupation patterns of genci/ci_occupation.cc:64-123
+    are exactly those with these per-space totals)."""
+    ng = len(gas_min)
+    orbs = [sum(1 << p for p in range(n) if gas_of_orbital[p] == g) for g in range(ng)]
+    def counts(strs):
+        return np.array([[bin(int(x) & m).count("1") for m in orbs] for x in strs], dtype=np.int64).reshape(-1, ng)
+    ca, cb = counts(make_strings(n, na)), counts(make_strings(n, nb))
+    tot = ca[:, None, :] + cb[None, :, :]
+    ok = np.all((tot >= np.array(gas_min)) & (tot <= np.array(gas_max)), axis=2)
+    return ok.ravel()
+
+
 class DLOpts(ctypes.Structure):
     _fields_ = [("collapse_per_root", ctypes.c_int), ("subspace_per_root", ctypes.c_int),
                 ("maxiter", ctypes.c_int), ("e_convergence", ctypes.c_double),
