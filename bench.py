@@ -355,7 +355,7 @@ def main():
 
     # ---- time to solution ------------------------------------------------------------------------
     if args.solve:
-        legs = [("fp64", "diagonal"), ("mixed", "learned"), ("mixed", "diagonal")]
+        legs = [("fp64", "diagonal"), ("mixed", "learned"), ("mixed", "diagonal"), ("mixed", "olsen")]
         lp_w = None
         if os.path.exists(forte_amd.learned.WEIGHTS):
             lp_w = forte_amd.learned.load_weights()
@@ -373,6 +373,8 @@ def main():
             if pc == "learned":
                 s.set_lp_weights(lp_w)
                 s.set_preconditioner("learned")
+            elif pc == "olsen":
+                s.set_preconditioner("olsen")
             forte_amd.device_synchronize()
             t0 = time.perf_counter()
             e = s.compute_energy()
@@ -397,6 +399,7 @@ def main():
                 "runs": tts,
                 "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
                 "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
+                "speedup_mixed_olsen_vs_fp64": round(ref["seconds"] / tts["mixed+olsen"]["seconds"], 3),
                 "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
                 "thresholds": {"e_conv": 1e-9, "r_conv": r_conv, "maxiter": args.maxiter},
                 "learned_weights": os.path.relpath(forte_amd.learned.WEIGHTS, ROOT) if lp_w is not None else None}

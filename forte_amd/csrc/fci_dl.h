@@ -45,6 +45,8 @@ class DavidsonLiu {
     using Sigma32Fn = std::function<void(const double* b, float* sigma)>;
     void add_sigma_builder32(Sigma32Fn f);
     void set_preconditioner(PrecondFn f) { precond_ = std::move(f); }
+    // Olsen's correction on top of the diagonal one (not in the reference; off by default)
+    void set_olsen(bool v) { olsen_ = v; }
     // determinants allowed to be nonzero (point-group sector); random vectors are zeroed elsewhere
     void set_mask(const uint8_t* d_mask) { mask_ = d_mask; }
     // called after the residual norms of an iteration are known; returning true ends the solve
@@ -84,6 +86,9 @@ class DavidsonLiu {
     DBuf<float> sigma32_v_;
     PrecondFn precond_;
     IterFn observe_;
+    bool olsen_ = false;
+    DBuf<double> olsen_x_;
+    void olsen_correct(int k);
     const uint8_t* mask_ = nullptr;
     bool have_hdiag_ = false;
     DBuf<double> hdiag_, b_, sigma_v_, r_, temp_;

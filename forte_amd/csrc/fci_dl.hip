@@ -383,6 +383,45 @@ __global__ void k_diag_precond(double* __restrict__ r, const double* __restrict_
     r[e] = (fabs(den) > 1.0e-6) ? r[e] / den : 0.0;
 }
 
+// Olsen's correction (J. Olsen, P. Jorgensen, J. Simons, Chem. Phys. Lett. 169, 463 (1990)):
+// delta = (lambda - D)^-1 r - eps (lambda - D)^-1 x with eps = x.(lambda - D)^-1 r / x.(lambda - D)^-1 x,
+// the diagonal correction made orthogonal to the Ritz vector x in the metric of (lambda - D)^-1.
+// Pass 1: partial sums of x.u (u = the already-preconditioned residual) and x.(x / den).
+__global__ __launch_bounds__(256) void k_olsen_dots(const double* __restrict__ u, const double* __restrict__ x,
+                                                    const double* __restrict__ h, double lambda, int64_t n,
+                                                    double* __restrict__ part) {
+    __shared__ double red[2][4];
+    double a = 0.0, b = 0.0;
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x) {
+        const double den = lambda - h[e];
+        const double w = (fabs(den) > 1.0e-6) ? x[e] / den : 0.0;
+        a += x[e] * u[e];
+        b += x[e] * w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        part[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    }
+}
+// pass 2: u -= eps x / den
+__global__ void k_olsen_update(double* __restrict__ u, const double* __restrict__ x, const double* __restrict__ h,
+                               double lambda, double eps, int64_t n) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double den = lambda - h[e];
+    if (fabs(den) > 1.0e-6) u[e] -= eps * x[e] / den;
+}
+
 // project_out_roots (davidson_liu_solver.cc:590-603): sequential over bad roots, one block
 __global__ void k_project_out(double* __restrict__ v, const int64_t* __restrict__ off, int nbad,
                               const int64_t* __restrict__ idx, const double* __restrict__ val) {
@@ -622,6 +661,27 @@ void DavidsonLiu::rotate_t(const T* X, T* out, int kout) {
     lincomb_t(X, basis_size_, coef.data(), kout, out);
 }
 void DavidsonLiu::rotate(const double* X, double* out, int kout) { rotate_t(X, out, kout); }
+
+// Olsen's correction of root k's preconditioned residual (r_ row k), with its Ritz vector formed
+// in a scratch row
+void DavidsonLiu::olsen_correct(int k) {
+    if (olsen_x_.n < size_t(size_)) olsen_x_.alloc(size_);
+    const int m = subspace_size_;
+    std::vector<double> coef(basis_size_);
+    for (int j = 0; j < basis_size_; ++j) coef[j] = alpha_[size_t(j) * m + k];
+    lincomb(b_.p, basis_size_, coef.data(), 1, olsen_x_.p);
+    double* u = r_.p + size_t(k) * size_;
+    const int nb = int(std::min<int64_t>(DL_NBLK, (size_ + 255) / 256));
+    k_olsen_dots<<<nb, 256>>>(u, olsen_x_.p, hdiag_.p, lambda_[k], size_, part_.p);
+    double* res = part_.p + size_t(DL_NBLK) * DL_MAXM;
+    k_reduce_parts<<<2, 256>>>(part_.p, nb, 2, res);
+    HIPCHK(hipGetLastError());
+    double ab[2];
+    HIPCHK(hipMemcpy(ab, res, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    if (std::fabs(ab[1]) < 1.0e-300) return;
+    k_olsen_update<<<nblk(size_, 256), 256>>>(u, olsen_x_.p, hdiag_.p, lambda_[k], ab[0] / ab[1], size_);
+    HIPCHK(hipGetLastError());
+}
 
 // Gram-Schmidt of vsrc against the rows of A into row rowsA, in fused passes: each cycle
 // subtracts the projections found by the previous pass and, in the same sweep, measures the norm
@@ -888,11 +948,13 @@ bool DavidsonLiu::solve() {
         // One root, the diagonal preconditioner, nothing to project out and no collapse ahead:
         // the residual sweep also measures the correction's norm and its overlaps with the basis,
         // which replaces the normalization pass and the first overlap pass of Gram-Schmidt.
-        const bool fuse = nroot_ == 1 && !precond_ && poff_.size() <= 1 && basis_size_ + nroot_ <= m &&
+        const bool fuse = nroot_ == 1 && !precond_ && !olsen_ && poff_.size() <= 1 && basis_size_ + nroot_ <= m &&
                           basis_size_ <= 16;  // the overlaps stay in registers up to 16 vectors
         std::vector<double> fov;
         for (int k = 0; k < nroot_; ++k)
             resid_[k] = std::sqrt(residual(k, !precond_, fuse ? &fov : nullptr));
+        if (olsen_ && !precond_)
+            for (int k = 0; k < nroot_; ++k) olsen_correct(k);
         if (precond_) {
             if (!precond_(r_.p, lambda_.data(), nroot_, iter)) {
                 // the custom preconditioner declined: the diagonal one, in place on the residuals

@@ -785,6 +785,7 @@ double Solver::compute_energy() {
     }
     csf_solved_ = false;
     dl_->set_print_level(opt_.print);
+    dl_->set_olsen(opt_.preconditioner == FCI_PRECOND_OLSEN);
     const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, basis));
     compute_hdiag();
     if (csf) {
@@ -1121,7 +1122,8 @@ int fci_set_options(fci_solver* s, const fci_options* o) {
         if (o->nroot < 1) throw Error(FCI_ERR_ARG, "nroot must be >= 1");
         if (o->guess_per_root < 1 || o->ndets_per_guess < 1) throw Error(FCI_ERR_ARG, "invalid guess options");
         if (o->precision != FCI_PREC_FP64 && o->precision != FCI_PREC_MIXED) throw Error(FCI_ERR_ARG, "invalid precision");
-        if (o->preconditioner != FCI_PRECOND_DIAGONAL && o->preconditioner != FCI_PRECOND_LEARNED)
+        if (o->preconditioner != FCI_PRECOND_DIAGONAL && o->preconditioner != FCI_PRECOND_LEARNED &&
+            o->preconditioner != FCI_PRECOND_OLSEN)
             throw Error(FCI_ERR_ARG, "invalid preconditioner");
         s->s->opt_ = *o;
     });

@@ -217,7 +217,8 @@ class FCISolver {
     void set_preconditioner(const std::string& p) {
         if (p == "diagonal") o_.preconditioner = FCI_PRECOND_DIAGONAL;
         else if (p == "learned") o_.preconditioner = FCI_PRECOND_LEARNED;
-        else throw std::runtime_error("preconditioner must be 'diagonal' or 'learned'");
+        else if (p == "olsen") o_.preconditioner = FCI_PRECOND_OLSEN;
+        else throw std::runtime_error("preconditioner must be 'diagonal', 'learned' or 'olsen'");
         push();
     }
     // FCISolver::set_options (fci_solver.cc:156-172) from a dict of Forte option names
@@ -234,7 +235,8 @@ class FCISolver {
         auto set_preconditioner = [&](const std::string& p) {
             if (p == "diagonal") o_.preconditioner = FCI_PRECOND_DIAGONAL;
             else if (p == "learned") o_.preconditioner = FCI_PRECOND_LEARNED;
-            else throw std::runtime_error("preconditioner must be 'diagonal' or 'learned'");
+            else if (p == "olsen") o_.preconditioner = FCI_PRECOND_OLSEN;
+            else throw std::runtime_error("preconditioner must be 'diagonal', 'learned' or 'olsen'");
         };
         for (auto item : d) {
             std::string k = py::str(item.first);

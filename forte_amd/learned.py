@@ -108,12 +108,14 @@ def group_losses(params, X, d0, e, group):
     return np.array(out)
 
 
-def compare_solves(params, seeds, n, e_conv=1e-9, r_conv=1e-5, maxiter=400, precision="fp64"):
-    """Davidson-Liu iterations and wall time with the diagonal and the learned preconditioner."""
+def compare_solves(params, seeds, n, e_conv=1e-9, r_conv=1e-5, maxiter=400, precision="fp64",
+                   variants=("diagonal", "learned")):
+    """Davidson-Liu iterations and wall time per preconditioner variant: "diagonal" (Forte's),
+    "learned", "olsen" (the diagonal correction made orthogonal to the Ritz vector)."""
     out = []
     for sd in seeds:
         row = {"seed": sd}
-        for pc in ("diagonal", "learned"):
+        for pc in variants:
             s = seeded_solver(n, sd)
             s.set_e_convergence(e_conv)
             s.set_r_convergence(r_conv)
@@ -123,6 +125,8 @@ def compare_solves(params, seeds, n, e_conv=1e-9, r_conv=1e-5, maxiter=400, prec
             if pc == "learned":
                 s.set_lp_weights(params)
                 s.set_preconditioner("learned")
+            elif pc == "olsen":
+                s.set_preconditioner("olsen")
             t0 = time.time()
             e = s.compute_energy()
             st = s.stats()
@@ -144,10 +148,21 @@ def main(argv=None):
     ap.add_argument("--reg", type=float, default=1e-4)
     ap.add_argument("--holdout", type=int, default=8, help="problems kept out of training")
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--compare", action="store_true",
+                    help="only solve the held-out problems with the diagonal, learned (shipped weights) and "
+                         "Olsen preconditioners and print the iteration counts")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     seeds = list(range(a.first_seed, a.first_seed + a.problems))
     ntrain = a.problems - a.holdout
+    if a.compare:
+        cmp = compare_solves(load_weights(), seeds[ntrain:], a.n, variants=("diagonal", "learned", "olsen"))
+        tot = {v: sum(r[v]["iterations"] for r in cmp) for v in ("diagonal", "learned", "olsen")}
+        for row in cmp:
+            log(f"seed {row['seed']}: " + ", ".join(f"{v} {row[v]['iterations']}" for v in tot))
+        log("total iterations: " + ", ".join(f"{v} {t}" for v, t in tot.items()))
+        print(json.dumps(dict(n=a.n, seeds=seeds[ntrain:], totals=tot, solves=cmp)))
+        return
     t0 = time.time()
     X, d0, e = harvest((seeded_solver(a.n, s) for s in seeds[:ntrain]), a.per_iter, a.max_groups, log=log)
     Xh, dh, eh = harvest((seeded_solver(a.n, s) for s in seeds[ntrain:]), a.per_iter, a.max_groups, seed=10**6,

@@ -47,7 +47,10 @@ enum fci_precision {
 
 enum fci_preconditioner {
     FCI_PRECOND_DIAGONAL = 0, /* r / (lambda - H_II), davidson_liu_solver.cc:432-448 */
-    FCI_PRECOND_LEARNED = 1   /* MLP correction with diagonal fallback */
+    FCI_PRECOND_LEARNED = 1,  /* MLP correction with diagonal fallback */
+    FCI_PRECOND_OLSEN = 2     /* diagonal correction made orthogonal to the Ritz vector in the
+                                 (lambda - D)^-1 metric (Olsen, Jorgensen, Simons 1990); not in the
+                                 reference */
 };
 
 /* Options of FCISolver::set_options (fci_solver.cc:156-172) and of the ActiveSpaceMethod base

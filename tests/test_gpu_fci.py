@@ -379,3 +379,24 @@ def test_fp64_sigma_run_to_run():
                                                  [r2.g1a(), r2.g2aa(), r2.g2ab(), r2.g2bb()]))
     say(f"  RDM run-to-run max difference {dr:.2e}")
     assert dr <= 1e-13
+
+
+@pytest.mark.parametrize("n,na,nb,nroot,mult", [(8, 4, 4, 2, 1), (7, 4, 3, 2, 2), (10, 5, 5, 1, 1)])
+def test_olsen_preconditioner_same_answer(oracle, n, na, nb, nroot, mult):
+    """Olsen's correction changes the Davidson-Liu path, not the eigenpairs."""
+    p = seeded_problem(n, na, nb, 31)
+    res = {}
+    for pc in ("diagonal", "olsen"):
+        g = gpu_solver(p, nroot, mult)
+        g.set_preconditioner(pc)
+        g.set_e_convergence(1e-12)
+        g.set_r_convergence(1e-8)
+        g.set_maxiter(500)
+        g.compute_energy()
+        assert g.stats()["converged"]
+        res[pc] = (np.array(g.energies()), g.stats()["iterations"])
+    say(f"  diagonal {res['diagonal'][1]} iterations, olsen {res['olsen'][1]}")
+    assert np.abs(res["diagonal"][0] - res["olsen"][0]).max() < 1e-10
+    if n <= 8:
+        ref = cpu_oracle(oracle, p).solve(nroot=nroot, multiplicity=mult, e_conv=1e-12, r_conv=1e-8)
+        assert np.abs(res["olsen"][0] - np.array(ref["energies"])).max() < 1e-9
