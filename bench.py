@@ -23,9 +23,11 @@ written and re-read), inputs resident in HBM. The line also carries:
   time_to_solution  full Davidson-Liu solves to e_conv 1e-9 at r_conv 1e-4 and at Forte's
                     default 1e-6: FP64 with the diagonal preconditioner, mixed precision
                     (FP32/BF16x3 sigma + FP64 refinement) with the learned preconditioner (shipped
-                    weights) and with the diagonal one;
+                    weights), the diagonal one and Olsen's correction;
   configs       CAS(8,8) FP64 and CAS(14,14) FP64 3-root solves (with root-0 RDM time) and their
                 CPU baselines;
+  spin_adaptation  CAS(14,14) solved in determinants and in CSFs (CI_SPIN_ADAPT);
+  batched       512 small problems in one batched call against 16 concurrent solvers;
   cpu_baseline  the CPU restatement of Forte's sigma (oracle/fci_oracle.c, one core) on a
                 strided sample of its loops, extrapolated to one full sigma.
 """
@@ -184,6 +186,33 @@ def other_configs(forte_amd, with_cpu):
             print(f"# {key} cpu: {d['cpu_baseline']}", file=sys.stderr, flush=True)
         out[key] = d
         del s
+    return out
+
+
+def spin_adapt_leg(forte_amd):
+    """CAS(14,14) singlet, seed 3, FP64, e_conv 1e-9 / r_conv 1e-6: the determinant-basis solve and
+    the CSF-basis one (CI_SPIN_ADAPT), with the CSF and coupling counts."""
+    h, eri = forte_amd.seeded_integrals(14, 3)
+    ints = forte_amd.make_active_space_integrals(h, eri)
+    out = {"workload": "CAS(14,14) seed 3 singlet, FP64, e_conv 1e-9, r_conv 1e-6"}
+    for label, sa in (("determinants", False), ("csf", True)):
+        s = forte_amd.FCISolver(7, 7, 1, 1, ints)
+        s.set_spin_adapt(sa)
+        s.set_e_convergence(1e-9)
+        s.set_r_convergence(1e-6)
+        s.set_maxiter(400)
+        if sa:
+            out["ncsf"], out["ncoupling"] = s.csf_size()
+        forte_amd.device_synchronize()
+        t0 = time.perf_counter()
+        e = s.compute_energy()
+        forte_amd.device_synchronize()
+        st = s.stats()
+        out[label] = dict(seconds=round(time.perf_counter() - t0, 3), energy=e, iterations=st["iterations"],
+                          converged=st["converged"])
+        print(f"# spin adaptation {label}: {out[label]}", file=sys.stderr, flush=True)
+        del s
+    out["ndet"] = int(forte_amd.FCISolver(7, 7, 1, 1, ints).ndet())
     return out
 
 
@@ -407,6 +436,7 @@ def main():
     # ---- the other configurations: CAS(8,8) and CAS(14,14) FP64 solves ---------------------------
     if args.configs:
         result["configs"] = other_configs(forte_amd, args.cpu)
+        result["spin_adaptation"] = spin_adapt_leg(forte_amd)
 
     # ---- batched small problems -----------------------------------------------------------------
     if args.batch > 0:
