@@ -477,6 +477,13 @@ def main():
 
     if args.cpu:
         result["cpu_baseline"] = cpu_baseline(n, h, eri, args.cpu_stride)
+        # the same FP64 solves on the CPU, extrapolated: one sigma times the GPU FP64 solve's sigma
+        # count (Davidson-Liu vector work not included)
+        if "time_to_solution" in result:
+            sps = result["cpu_baseline"]["seconds_per_sigma_extrapolated"]
+            result["cpu_baseline"]["time_to_solution_extrapolated_seconds"] = {
+                k: round(sps * v["runs"]["fp64+diagonal"]["sigma_builds"], 1)
+                for k, v in result["time_to_solution"].items()}
         print(f"# cpu baseline {result['cpu_baseline']}", file=sys.stderr, flush=True)
     else:
         result["cpu_baseline"] = None
