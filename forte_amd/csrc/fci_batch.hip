@@ -1120,26 +1120,32 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
         constexpr int kPoll = 4;
         int max_it = 0;
         for (auto& c : classes) max_it = std::max(max_it, c->maxiter);
-        // autotune: each candidate runs two of the class's first iterations, timed with events
+        // autotune: after one untimed iteration (the guess vectors' sigma builds make the first one
+        // heavier than the rest), the candidates run one iteration each in turn, twice over, timed
+        // with events; a candidate's time is its faster round (iterations differ in basis size
+        // and collapses, so interleaving keeps the comparison fair)
         for (auto& c : classes) {
             BatchClass& C = *c;
             if (C.var.size() < 2) continue;
             hipEvent_t e0, e1;
             HIPCHK(hipEventCreate(&e0));
             HIPCHK(hipEventCreate(&e1));
-            for (auto& v : C.var) {
-                HIPCHK(hipEventRecord(e0, C.stream));
-                for (int r = 0; r < 2; ++r) C.launch(int(&v - C.var.data()));
-                HIPCHK(hipEventRecord(e1, C.stream));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                v.ms = ms / 2.0;
-                C.iters_run += 2;
-            }
+            C.launch(0);
+            C.iters_run += 1;
+            for (auto& v : C.var) v.ms = 1e30;
+            for (int round = 0; round < 2; ++round)
+                for (auto& v : C.var) {
+                    HIPCHK(hipEventRecord(e0, C.stream));
+                    C.launch(int(&v - C.var.data()));
+                    HIPCHK(hipEventRecord(e1, C.stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    v.ms = std::min(v.ms, double(ms));
+                    C.iters_run += 1;
+                }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            // the first candidate ran the very first iterations (guess sigma builds): compare the rest
             int best = 0;
             for (int i = 1; i < int(C.var.size()); ++i)
                 if (C.var[i].ms < C.var[best].ms) best = i;
