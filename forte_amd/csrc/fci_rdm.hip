@@ -84,30 +84,40 @@ __global__ __launch_bounds__(RD_BS) void k_rdm1_rows(const double* __restrict__ 
 // same-spin 2-RDM through pair cliques: acc[m_pair][m'_pair] += s_m s_m' sum_c L[I_m,c] R[I_m',c]
 constexpr int RC_TW = 32;
 constexpr int RC_MAXM = 96;
+// Same-spin 2-RDM through the pair cliques: for clique K, the member-by-member products
+// sum_c DL[m][c] DR[m'][c] of the signed member rows. Cliques of more than RC_MAXM members are split
+// into RC_MAXM x RC_MAXM member tiles, one block per (K, tile pair): blockIdx.y = ti * nt + tj.
 __global__ __launch_bounds__(RD_BS) void k_rdm2_clique(const double* __restrict__ L, const double* __restrict__ R,
                                                        int64_t ncol, const uint32_t* __restrict__ clq, int M,
-                                                       double* __restrict__ acc_out, int gdim) {
-    __shared__ uint32_t mem[RC_MAXM];
+                                                       double* __restrict__ acc_out, int gdim, int nt) {
+    __shared__ uint32_t memI[RC_MAXM], memJ[RC_MAXM];
     __shared__ double DL_[RC_MAXM][RC_TW + 1], DR_[RC_MAXM][RC_TW + 1];
     const int64_t K = blockIdx.x;
+    const int ti = blockIdx.y / nt, tj = blockIdx.y % nt;
+    const int i0 = ti * RC_MAXM, j0 = tj * RC_MAXM;
+    const int Mi = min(RC_MAXM, M - i0), Mj = min(RC_MAXM, M - j0);
     const int tid = threadIdx.x;
-    for (int i = tid; i < M; i += RD_BS) mem[i] = clq[K * M + i];
+    for (int i = tid; i < Mi; i += RD_BS) memI[i] = clq[K * M + i0 + i];
+    for (int i = tid; i < Mj; i += RD_BS) memJ[i] = clq[K * M + j0 + i];
     __syncthreads();
     constexpr int PER = (RC_MAXM * RC_MAXM + RD_BS - 1) / RD_BS;
     double acc[PER];
 #pragma unroll
     for (int x = 0; x < PER; ++x) acc[x] = 0.0;
     for (int64_t c0 = 0; c0 < ncol; c0 += RC_TW) {
-        for (int i = tid; i < M * RC_TW; i += RD_BS) {
-            int m = i / RC_TW, c = i % RC_TW;
+        for (int i = tid; i < max(Mi, Mj) * RC_TW; i += RD_BS) {
+            const int m = i / RC_TW, c = i % RC_TW;
             double l = 0.0, r = 0.0;
             if (c0 + c < ncol) {
-                int64_t row = int64_t(mem[m] & kAddrMask);
-                l = L[row * ncol + c0 + c];
-                r = R[row * ncol + c0 + c];
-                if (mem[m] >> 31) {
-                    l = -l;
-                    r = -r;
+                if (m < Mi) {
+                    const int64_t row = int64_t(memI[m] & kAddrMask);
+                    l = L[row * ncol + c0 + c];
+                    if (memI[m] >> 31) l = -l;
+                }
+                if (m < Mj) {
+                    const int64_t row = int64_t(memJ[m] & kAddrMask);
+                    r = R[row * ncol + c0 + c];
+                    if (memJ[m] >> 31) r = -r;
                 }
             }
             DL_[m][c] = l;
@@ -116,22 +126,22 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_clique(const double* __restrict_
         __syncthreads();
 #pragma unroll
         for (int x = 0; x < PER; ++x) {
-            int idx = tid + x * RD_BS;
-            if (idx < M * M) {
-                int m = idx / M, mp = idx % M;
-                double s = 0.0;
-                for (int c = 0; c < RC_TW; ++c) s += DL_[m][c] * DR_[mp][c];
-                acc[x] += s;
+            const int idx = tid + x * RD_BS;
+            if (idx < Mi * Mj) {
+                const int m = idx / Mj, mp = idx % Mj;
+                double sum = 0.0;
+                for (int c = 0; c < RC_TW; ++c) sum += DL_[m][c] * DR_[mp][c];
+                acc[x] += sum;
             }
         }
         __syncthreads();
     }
 #pragma unroll
     for (int x = 0; x < PER; ++x) {
-        int idx = tid + x * RD_BS;
-        if (idx < M * M) {
-            int m = idx / M, mp = idx % M;
-            int pm = int((mem[m] >> 21) & 0x3FFu), pmp = int((mem[mp] >> 21) & 0x3FFu);
+        const int idx = tid + x * RD_BS;
+        if (idx < Mi * Mj) {
+            const int m = idx / Mj, mp = idx % Mj;
+            const int pm = int((memI[m] >> 21) & 0x3FFu), pmp = int((memJ[mp] >> 21) & 0x3FFu);
             atomicAdd(&acc_out[size_t(pm) * gdim + pmp], acc[x]);
         }
     }
@@ -491,15 +501,17 @@ void rdm2_same_device(const SpinTables& a, const SpinTables& b, const double* L,
     HIPCHK(hipMemsetAsync(acc_a, 0, size_t(npair) * npair * sizeof(double), st));
     HIPCHK(hipMemsetAsync(acc_b, 0, size_t(npair) * npair * sizeof(double), st));
     if (a.d == 2) {
-        if (a.M > RC_MAXM) throw Error(-1, "rdm2: clique too large");
-        k_rdm2_clique<<<unsigned(a.nK), RD_BS, 0, st>>>(L, R, Nb, a.clq.p, a.M, acc_a, npair);
+        const int nta = (a.M + RC_MAXM - 1) / RC_MAXM;
+        k_rdm2_clique<<<dim3(unsigned(a.nK), unsigned(nta * nta)), RD_BS, 0, st>>>(L, R, Nb, a.clq.p, a.M, acc_a, npair,
+                                                                                  nta);
     }
     if (b.d == 2) {
-        if (b.M > RC_MAXM) throw Error(-1, "rdm2: clique too large");
+        const int ntb = (b.M + RC_MAXM - 1) / RC_MAXM;
         DBuf<double> Lt(size_t(Na) * Nb), Rt(size_t(Na) * Nb);
         transpose_device<double>(L, Lt.p, Na, Nb, st);
         transpose_device<double>(R, Rt.p, Na, Nb, st);
-        k_rdm2_clique<<<unsigned(b.nK), RD_BS, 0, st>>>(Lt.p, Rt.p, Na, b.clq.p, b.M, acc_b, npair);
+        k_rdm2_clique<<<dim3(unsigned(b.nK), unsigned(ntb * ntb)), RD_BS, 0, st>>>(Lt.p, Rt.p, Na, b.clq.p, b.M, acc_b,
+                                                                                   npair, ntb);
         HIPCHK(hipStreamSynchronize(st));
     }
     HIPCHK(hipGetLastError());

@@ -11,8 +11,7 @@
  * Scope: all orbitals active (doubly occupied ones are folded into the integrals by the caller),
  * any na / nb, C1 or one irrep of a point group (fci_set_symmetry); n <= 22 orbitals for the string
  * tables and n <= 19 for sigma (the alpha-beta kernels hold at most 12 row tiles of 16 packed pairs).
- * The same-spin 2-RDM needs cliques of at most 96 members, C(n - k + 2, 2) <= 96 for k electrons
- * of a spin (so CAS(16,16) works, but 2 electrons in 16 orbitals do not). Integrals use Forte's
+ * Integrals use Forte's
  * layout (forte/integrals/active_space_integrals.h:103-127): oei n*n row-major, tei n^4
  * row-major physicist, tei_aa / tei_bb antisymmetrized <pq||rs>, tei_ab = <pq|rs>.
  * Determinants are ordered alpha-major, C[Ia*Nb + Ib] (forte/fci/fci_vector.cc:108-114).

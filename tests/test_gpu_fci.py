@@ -400,3 +400,21 @@ def test_olsen_preconditioner_same_answer(oracle, n, na, nb, nroot, mult):
     if n <= 8:
         ref = cpu_oracle(oracle, p).solve(nroot=nroot, multiplicity=mult, e_conv=1e-12, r_conv=1e-8)
         assert np.abs(res["olsen"][0] - np.array(ref["energies"])).max() < 1e-9
+
+
+@pytest.mark.parametrize("n,na,nb", [(16, 2, 2), (15, 2, 3), (14, 1, 2)])
+def test_rdms_large_cliques(oracle, n, na, nb):
+    """Same-spin 2-RDM cliques of more than 96 members (C(n - k + 2, 2) = 120, 105, 91/78 here):
+    the member products are split into tiles."""
+    say("rdms large cliques", n, na, nb)
+    p = seeded_problem(n, na, nb, 41)
+    g, o = gpu_solver(p, 1, abs(na - nb) + 1), cpu_oracle(oracle, p)
+    g.set_e_convergence(1e-12)
+    g.set_r_convergence(1e-8)
+    g.set_maxiter(300)
+    g.compute_energy()
+    rd = g.rdms([(0, 0)], 2)[0]
+    ref = o.rdms(g.evec(0))
+    for name, a, b in zip(("g1a", "g1b", "g2aa", "g2ab", "g2bb"), [rd.g1a(), rd.g1b(), rd.g2aa(), rd.g2ab(), rd.g2bb()],
+                          ref):
+        assert np.abs(a - b).max() < 1e-12, name
