@@ -43,13 +43,15 @@ def test_default_options_without_gpu():
                     ("maxiter", ctypes.c_int), ("e_convergence", ctypes.c_double),
                     ("r_convergence", ctypes.c_double), ("die_if_not_converged", ctypes.c_int),
                     ("print", ctypes.c_int), ("precision", ctypes.c_int), ("preconditioner", ctypes.c_int),
-                    ("mixed_switch", ctypes.c_double)]
+                    ("mixed_switch", ctypes.c_double), ("spin_adapt", ctypes.c_int),
+                    ("spin_adapt_full_preconditioner", ctypes.c_int)]
     o = Opts()
     lib.fci_default_options(ctypes.byref(o))
     # forte/options.yaml DL_* defaults and E/R convergence of ActiveSpaceSolver
     assert (o.nroot, o.guess_per_root, o.ndets_per_guess, o.collapse_per_root, o.subspace_per_root,
             o.maxiter) == (1, 1, 50, 2, 10, 100)
     assert o.e_convergence == 1e-10 and o.r_convergence == 1e-6
+    assert (o.spin_adapt, o.spin_adapt_full_preconditioner) == (0, 0)  # CI_SPIN_ADAPT defaults
 
 
 def test_seeded_integrals_match_oracle(oracle):
