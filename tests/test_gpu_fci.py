@@ -418,3 +418,31 @@ def test_rdms_large_cliques(oracle, n, na, nb):
     for name, a, b in zip(("g1a", "g1b", "g2aa", "g2ab", "g2bb"), [rd.g1a(), rd.g1b(), rd.g2aa(), rd.g2ab(), rd.g2bb()],
                           ref):
         assert np.abs(a - b).max() < 1e-12, name
+
+
+@pytest.mark.parametrize("n,na,nb", [(19, 2, 2), (19, 3, 2)])
+def test_sigma_nineteen_orbitals(oracle, n, na, nb):
+    """The largest orbital count of the alpha-beta kernels (190 packed pairs, 12 row tiles)."""
+    import forte_amd
+    say("sigma n=19", n, na, nb)
+    p = seeded_problem(n, na, nb, 5 + na)
+    g, o = gpu_solver(p), cpu_oracle(oracle, p)
+    C = np.random.default_rng(n + na).standard_normal(o.ndet)
+    C /= np.linalg.norm(C)
+    ref = o.sigma(C)
+    assert np.abs(g.sigma(C) - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    c32 = forte_amd.DeviceArray(o.ndet, "float32")
+    s32 = forte_amd.DeviceArray(o.ndet, "float32")
+    c32.upload(C.astype(np.float32))
+    g.sigma_arrays(c32, s32)
+    err = np.linalg.norm(s32.download().astype(np.float64) - ref) / np.linalg.norm(ref)
+    say("  mixed relative error", err)
+    assert err < 3e-6
+
+
+def test_twenty_orbitals_sigma_is_refused():
+    """Beyond 19 orbitals the alpha-beta kernels raise before any launch."""
+    p = seeded_problem(20, 2, 2, 1)
+    g = gpu_solver(p)
+    with pytest.raises(RuntimeError, match="too many orbitals"):
+        g.sigma(np.ones(g.ndet()))
