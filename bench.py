@@ -20,10 +20,14 @@ written and re-read), inputs resident in HBM. The line also carries:
                     2 N_det (s + oo + vvoo) flops on the FP32 / FP64 VALU; alpha-beta: executed
                     MFMA flops);
   fp64_sigma    the FP64 sigma of the plain solver on the same vector;
+  ms0_symmetric_sigma  the sigma of a spin-flip-symmetric vector (C^T = C) without its beta
+                same-spin term (fci_sigma_ms0_device), mixed and FP64;
   time_to_solution  full Davidson-Liu solves to e_conv 1e-9 at r_conv 1e-4 and at Forte's
                     default 1e-6: FP64 with the diagonal preconditioner, mixed precision
                     (FP32/BF16x3 sigma + FP64 refinement) with the learned preconditioner (shipped
-                    weights), the diagonal one and Olsen's correction;
+                    weights), the diagonal one and Olsen's correction, all in the Ms = 0
+                    spin-flip-symmetric subspace (the default); FP64 and mixed+diagonal also with
+                    the symmetry off ("/full_sigma", every sigma built in full as in Forte);
   configs       CAS(8,8) FP64 and CAS(14,14) FP64 3-root solves (with root-0 RDM time) and their
                 CPU baselines;
   spin_adaptation  CAS(14,14) solved in determinants and in CSFs (CI_SPIN_ADAPT);
@@ -325,6 +329,30 @@ def main():
           f"alpha-beta {t3_64[2]:.2f} ms", file=sys.stderr, flush=True)
     del C64, S64
 
+    # ---- the spin-flip-symmetric sigma on an Ms = 0 singlet-symmetric vector (C^T = C) -----------
+    # the solves below keep their vectors in that subspace, where the beta same-spin term is the
+    # transposed alpha one (fci_sigma_ms0_device)
+    N = int(round(np.sqrt(nd)))
+    xs = x.reshape(N, N)
+    xs = xs + xs.T
+    xs = (xs / np.linalg.norm(xs)).reshape(-1)
+    ms0 = {}
+    for dt_name, dt in (("mixed", "float32"), ("fp64", "float64")):
+        Cs = forte_amd.DeviceArray(nd, dt)
+        Ss = forte_amd.DeviceArray(nd, dt)
+        Cs.upload(xs.astype(dt))
+        solver.sigma_ms0_arrays(Cs, Ss, 1)
+        reps = max(1, args.steps // (2 if dt == "float32" else 4))
+        forte_amd.device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            solver.sigma_ms0_arrays(Cs, Ss, 1)
+        ms0[dt_name] = 1e3 * (time.perf_counter() - t0) / reps
+        del Cs, Ss
+    del xs
+    print(f"# spin-flip-symmetric sigma: mixed {ms0['mixed']:.2f} ms, fp64 {ms0['fp64']:.2f} ms", file=sys.stderr,
+          flush=True)
+
     result = {
         "metric": "CAS(16,16) FCI time to |dE|<1e-8 Eh: mixed+learned vs FP64 plain; sigma GB/s",
         "value": round(value, 3),
@@ -358,6 +386,10 @@ def main():
         "fp64_sigma": {"ms": round(ms64, 3), "GBps": round(9 * nd * 8 / (ms64 * 1e-3) / 1e9, 3),
                        "split_ms": [round(t, 3) for t in t3_64],
                        "host_buffers_pcie_inclusive_ms": None if pcie_ms is None else round(pcie_ms, 1)},
+        "ms0_symmetric_sigma": {"vector": "C^T = C (Ms = 0, even S), the subspace the singlet solves keep",
+                                "mixed_ms": round(ms0["mixed"], 3), "fp64_ms": round(ms0["fp64"], 3),
+                                "note": "beta same-spin term = transposed alpha term, not built; "
+                                        "wall time per call incl. one host synchronization"},
     }
 
     # every sigma kernel group against the roofline that bounds it
@@ -384,21 +416,27 @@ def main():
 
     # ---- time to solution ------------------------------------------------------------------------
     if args.solve:
-        legs = [("fp64", "diagonal"), ("mixed", "learned"), ("mixed", "diagonal"), ("mixed", "olsen")]
+        # (precision, preconditioner, Ms = 0 spin-flip symmetry); "full_sigma": the symmetry off, every
+        # sigma with its beta same-spin term, as the reference builds it
+        legs = [("fp64", "diagonal", True), ("fp64", "diagonal", False), ("mixed", "learned", True),
+                ("mixed", "diagonal", True), ("mixed", "olsen", True), ("mixed", "diagonal", False)]
         lp_w = None
         if os.path.exists(forte_amd.learned.WEIGHTS):
             lp_w = forte_amd.learned.load_weights()
         else:
             legs = [leg for leg in legs if leg[1] != "learned"]
-        ms_mixed, ms_fp64 = ms_per_step, ms64
 
-        def solve_leg(prec, pc, r_conv):
+        def leg_name(prec, pc, sym):
+            return f"{prec}+{pc}" + ("" if sym else "/full_sigma")
+
+        def solve_leg(prec, pc, sym, r_conv):
             s = forte_amd.FCISolver(na, nb, 1, 1, ints)
             s.set_e_convergence(1e-9)
             s.set_r_convergence(r_conv)
             s.set_maxiter(args.maxiter)
             s.set_die_if_not_converged(False)
             s.set_precision(prec)
+            s.set_ms0_symmetry(sym)
             if pc == "learned":
                 s.set_lp_weights(lp_w)
                 s.set_preconditioner("learned")
@@ -410,25 +448,33 @@ def main():
             forte_amd.device_synchronize()
             dt = time.perf_counter() - t0
             st = s.stats()
-            print(f"# solve {prec}+{pc} r_conv {r_conv:g}: E={e:.12f} {dt:.2f}s {st}", file=sys.stderr, flush=True)
+            print(f"# solve {leg_name(prec, pc, sym)} r_conv {r_conv:g}: E={e:.12f} {dt:.2f}s {st}", file=sys.stderr,
+                  flush=True)
             # time outside the sigma builds (Davidson-Liu vector passes, guess, host steps) per iteration
             n32 = st["sigma_builds"] - st["sigma_builds_fp64"]
-            rest = dt - 1e-3 * (n32 * ms_mixed + st["sigma_builds_fp64"] * ms_fp64)
+            m32, m64 = (ms0["mixed"], ms0["fp64"]) if st["ms0_symmetry"] else (ms_per_step, ms64)
+            rest = dt - 1e-3 * (n32 * m32 + st["sigma_builds_fp64"] * m64)
             return dict(seconds=round(dt, 3), energy=e,
                         non_sigma_ms_per_iteration=round(1e3 * rest / max(1, st["iterations"]), 2),
                         **{k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
-                                              "converged", "fell_back")})
+                                              "converged", "fell_back", "ms0_symmetry")})
 
         result["time_to_solution"] = {}
         for r_conv in (1e-4, 1e-6):
-            tts = {f"{prec}+{pc}": solve_leg(prec, pc, r_conv) for prec, pc in legs}
+            tts = {leg_name(*leg): solve_leg(*leg, r_conv) for leg in legs}
             ref = tts["fp64+diagonal"]
+            plain = tts["fp64+diagonal/full_sigma"]
             best = tts.get("mixed+learned", tts["mixed+diagonal"])
             result["time_to_solution"][f"r_conv_{r_conv:g}"] = {
                 "runs": tts,
                 "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
                 "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
                 "speedup_mixed_olsen_vs_fp64": round(ref["seconds"] / tts["mixed+olsen"]["seconds"], 3),
+                "speedup_mixed_diagonal_vs_fp64_full_sigma": round(plain["seconds"] / tts["mixed+diagonal"]["seconds"],
+                                                                   3),
+                "speedup_ms0_symmetry_fp64": round(plain["seconds"] / ref["seconds"], 3),
+                "speedup_ms0_symmetry_mixed": round(tts["mixed+diagonal/full_sigma"]["seconds"]
+                                                    / tts["mixed+diagonal"]["seconds"], 3),
                 "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
                 "thresholds": {"e_conv": 1e-9, "r_conv": r_conv, "maxiter": args.maxiter},
                 "learned_weights": os.path.relpath(forte_amd.learned.WEIGHTS, ROOT) if lp_w is not None else None}
@@ -482,7 +528,7 @@ def main():
         if "time_to_solution" in result:
             sps = result["cpu_baseline"]["seconds_per_sigma_extrapolated"]
             result["cpu_baseline"]["time_to_solution_extrapolated_seconds"] = {
-                k: round(sps * v["runs"]["fp64+diagonal"]["sigma_builds"], 1)
+                k: round(sps * v["runs"]["fp64+diagonal/full_sigma"]["sigma_builds"], 1)
                 for k, v in result["time_to_solution"].items()}
         print(f"# cpu baseline {result['cpu_baseline']}", file=sys.stderr, flush=True)
     else:

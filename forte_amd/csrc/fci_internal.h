@@ -136,6 +136,9 @@ struct DevInts {
     // packed chemist ERIs V[P][Q] = (pq|rs), P,Q packed unordered pairs (np x np)
     DBuf<double> vpk;
     DBuf<float> vpk32;
+    // the same scaled by 1/2 (exact): the alpha-beta term of the Ms = 0 spin-flip-symmetric sigma
+    DBuf<double> vpk_h;
+    DBuf<float> vpk32_h;
     // same-spin clique operator G'[m][m'] over members (pairs or orbitals), per spin
     DBuf<double> g2[2];
     DBuf<float> g2f[2];
@@ -183,6 +186,12 @@ class SigmaEngine {
     // waited for. Throws before launching anything if ext_bytes < scratch_min_bytes(sizeof(T)).
     void apply_ext(const double* C, double* S, hipStream_t st, void* ext, size_t ext_bytes);
     void apply_ext(const float* C, float* S, hipStream_t st, void* ext, size_t ext_bytes);
+    // sigma = H C for Ms = 0 vectors with C^T = eta C (eta = +-1; same alpha and beta strings and
+    // integrals): the beta same-spin term is eta (alpha same-spin term)^T, so it is not built.
+    // S = 1/2 e_core C + H_aa C + 1/2 H_ab C, then S <- S + eta S^T in place; the result is
+    // exactly (bitwise) eta-symmetric.
+    void apply_sym(const double* C, double* S, int eta, hipStream_t st);
+    void apply_sym(const float* C, float* S, int eta, hipStream_t st);
     // smallest external scratch one sigma needs with w-byte elements
     size_t scratch_min_bytes(size_t w) const;
     void timings(double* ms3) const;
@@ -190,6 +199,7 @@ class SigmaEngine {
 
   private:
     template <class T> void apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, size_t ext_bytes);
+    template <class T> void apply_sym_t(const T* C, T* S, int eta, hipStream_t st);
     const SpinTables* a_;
     const SpinTables* b_;
     const DevInts* ints_;
@@ -233,6 +243,10 @@ void transpose_device(const T* in, T* out, int64_t rows, int64_t cols, hipStream
 template <class T>
 void transpose_add_device(const T* in, T* out, int64_t rows, int64_t cols, const T* base,
                           T base_scale, hipStream_t st);
+// square N x N matrix in place: S <- S + eta S^T (eta = +-1; each mirrored pair is computed once,
+// so the result is exactly eta-symmetric); with avg, S <- (S + eta S^T) / 2
+template <class T>
+void sym_add_device(T* S, int64_t N, int eta, bool avg, hipStream_t st);
 
 // ---- CSF spin adaptation (fci_csf.hip) -----------------------------------------------------------
 // a determinant as its alpha / beta occupation bits

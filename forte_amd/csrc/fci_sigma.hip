@@ -144,6 +144,48 @@ template void transpose_device<float>(const float*, float*, int64_t, int64_t, hi
 template void transpose_add_device<double>(const double*, double*, int64_t, int64_t, const double*, double, hipStream_t);
 template void transpose_add_device<float>(const float*, float*, int64_t, int64_t, const float*, float, hipStream_t);
 
+// S <- S + eta S^T in place (optionally halved), tile pair (I, J), I <= J, per block: both 64 x 64
+// tiles are read into LDS before either is written. Element (i, j) becomes a + eta b and its mirror
+// b + eta a with a = S[i][j], b = S[j][i]; IEEE addition is commutative and x - y = -(y - x), so
+// the mirror is exactly eta times the element.
+template <class T>
+__global__ __launch_bounds__(256) void k_sym_add(T* __restrict__ S, int64_t N, int eta, int avg) {
+    const int I = blockIdx.y, J = blockIdx.x;
+    if (J < I) return;
+    __shared__ T ta[64][65], tb[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t r0 = int64_t(I) * 64, c0 = int64_t(J) * 64;
+    const T e = T(eta), h = avg ? T(0.5) : T(1);
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t r = r0 + j, c = c0 + tx;
+        if (r < N && c < N) ta[j][tx] = S[r * N + c];  // tile (I, J)
+        const int64_t r2 = c0 + j, c2 = r0 + tx;
+        if (I != J && r2 < N && c2 < N) tb[j][tx] = S[r2 * N + c2];  // tile (J, I)
+    }
+    __syncthreads();
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t r = r0 + j, c = c0 + tx;
+        if (r < N && c < N) {
+            const T a = ta[j][tx], b = I != J ? tb[tx][j] : ta[tx][j];
+            S[r * N + c] = h * (a + e * b);
+        }
+        const int64_t r2 = c0 + j, c2 = r0 + tx;
+        if (I != J && r2 < N && c2 < N) {
+            const T a = tb[j][tx], b = ta[tx][j];
+            S[r2 * N + c2] = h * (a + e * b);
+        }
+    }
+}
+
+template <class T> void sym_add_device(T* S, int64_t N, int eta, bool avg, hipStream_t st) {
+    if (N <= 0) return;
+    const unsigned t = unsigned((N + 63) / 64);
+    k_sym_add<T><<<dim3(t, t), 256, 0, st>>>(S, N, eta, avg ? 1 : 0);
+    HIPCHK(hipGetLastError());
+}
+template void sym_add_device<double>(double*, int64_t, int, bool, hipStream_t);
+template void sym_add_device<float>(float*, int64_t, int, bool, hipStream_t);
+
 // ---- same-spin pair-clique GEMM ----------------------------------------------------------------
 constexpr int CQ_TW = 64;   // columns per block (4 waves x 16)
 constexpr int CQ_MT = 48;   // output members per block (3 MFMA row tiles)
@@ -1916,7 +1958,7 @@ static void launch_clique(const T* X, T* Y, int64_t L, const SpinTables& sp, con
 
 template <int KS>
 static void ab_f64_launch(const double* C, double* S, const SpinTables& a, const SpinTables& b,
-                          const DevInts& ints, hipStream_t st) {
+                          const DevInts& ints, hipStream_t st, bool half) {
     int nmt = (b.np + 15) / 16;
     size_t xs = size_t(KS * 4) * AB_TW * sizeof(double) + AB_TW * sizeof(double);
     size_t row = size_t(b.nstr) * sizeof(double);
@@ -1925,12 +1967,12 @@ static void ab_f64_launch(const double* C, double* S, const SpinTables& a, const
     auto kern = k_ab_f64<KS>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 256, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.out.p, b.np,
-                                            ints.vpk.p, nmt, use_row);
+                                            (half ? ints.vpk_h.p : ints.vpk.p), nmt, use_row);
     HIPCHK(hipGetLastError());
 }
 template <int KS>
 static void ab_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
-                            const DevInts& ints, hipStream_t st) {
+                            const DevInts& ints, hipStream_t st, bool half) {
     int nmt = (b.np + 15) / 16;
     size_t xs = size_t(AB_TW) * (KS * 32 + 8) * 2 * sizeof(__bf16) + AB_TW * sizeof(float);
     size_t row = size_t(b.nstr) * sizeof(float);
@@ -1939,23 +1981,23 @@ static void ab_mixed_launch(const float* C, float* S, const SpinTables& a, const
     auto kern = k_ab_mixed<KS>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 256, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.out.p, b.np,
-                                            ints.vpk32.p, nmt, use_row);
+                                            (half ? ints.vpk32_h.p : ints.vpk32.p), nmt, use_row);
     HIPCHK(hipGetLastError());
 }
 
 template <int KS, int TPW, bool SPLIT>
 static void ab2_f64_launch(const double* C, double* S, const SpinTables& a, const SpinTables& b,
-                           const DevInts& ints, hipStream_t st) {
+                           const DevInts& ints, hipStream_t st, bool half) {
     const size_t sh = size_t(b.nstr) * sizeof(double);
     auto kern = k_ab2_f64<KS, TPW, SPLIT>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 512, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
-                                            ints.vpk.p, (b.np + 15) / 16, nullptr, nullptr, nullptr, nullptr);
+                                            (half ? ints.vpk_h.p : ints.vpk.p), (b.np + 15) / 16, nullptr, nullptr, nullptr, nullptr);
     HIPCHK(hipGetLastError());
 }
 template <int KS, int TPW, int WM, int WC>
 static void ab3_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
-                             const DevInts& ints, hipStream_t st) {
+                             const DevInts& ints, hipStream_t st, bool half) {
     const int nmt = (b.np + 15) / 16;
     const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * sizeof(double) + size_t(nmt) * KS * 2 * 64 * 16;
     if (nmt > WM * TPW || sh3 + KS * 32 * 4 + 12 * 16 * 4 > 160 * 1024)
@@ -1963,18 +2005,18 @@ static void ab3_mixed_launch(const float* C, float* S, const SpinTables& a, cons
     auto kern3 = k_ab3_mixed<KS, TPW, WM, WC>;
     HIPCHK(hipFuncSetAttribute((const void*)kern3, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh3)));
     kern3<<<xcd_grid(a.nstr), 64 * WM * WC, sh3, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p,
-                                                       b.npp, b.np, ints.vpk32.p, nmt);
+                                                       b.npp, b.np, (half ? ints.vpk32_h.p : ints.vpk32.p), nmt);
     HIPCHK(hipGetLastError());
 }
 
 template <int KS, int TPW, int WM, int WC>
 static void ab2_mixed_launch(const float* C, float* S, const SpinTables& a, const SpinTables& b,
-                             const DevInts& ints, hipStream_t st) {
+                             const DevInts& ints, hipStream_t st, bool half) {
     const size_t sh = size_t(b.nstr) * sizeof(double);
     auto kern = k_ab2_mixed<KS, TPW, WM, WC>;
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh)));
     kern<<<xcd_grid(a.nstr), 64 * WM * WC, sh, st>>>(C, S, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.outT.p, b.npp, b.np,
-                                            ints.vpk32.p, (b.np + 15) / 16);
+                                            (half ? ints.vpk32_h.p : ints.vpk32.p), (b.np + 15) / 16);
     HIPCHK(hipGetLastError());
 }
 
@@ -1985,7 +2027,7 @@ static bool ab2_ok(const SpinTables& a, const SpinTables& b, size_t w) {
 }
 
 static void launch_ab(const double* C, double* S, const SpinTables& a, const SpinTables& b,
-                      const DevInts& ints, hipStream_t st) {
+                      const DevInts& ints, hipStream_t st, bool half) {
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 15) / 16) * 4;  // k-steps of 4, in buckets of 16 excitations
         const int tpw = ((b.np + 15) / 16 + 3) / 4;
@@ -1993,8 +2035,8 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
         const bool split = tpw > 1 && (b.np + 15) / 16 == 4 * (tpw - 1) + 1;
 #define F64CASE(K, T)                                                                      \
     if (ks == K && tpw == T)                                                                \
-        return split ? ab2_f64_launch<K, T, (T > 1)>(C, S, a, b, ints, st)                  \
-                     : ab2_f64_launch<K, T, false>(C, S, a, b, ints, st);
+        return split ? ab2_f64_launch<K, T, (T > 1)>(C, S, a, b, ints, st, half)                  \
+                     : ab2_f64_launch<K, T, false>(C, S, a, b, ints, st, half);
 #define F64ROW(K) F64CASE(K, 1) F64CASE(K, 2) F64CASE(K, 3)
         F64ROW(4) F64ROW(8) F64ROW(12) F64ROW(16) F64ROW(20) F64ROW(24)
 #undef F64ROW
@@ -2005,7 +2047,7 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
     if ((b.np + 15) / 16 > 4 * AB_MTW) throw Error(-1, "too many orbitals for the alpha-beta kernel");
     switch (ks) {
 #define CASE(K) \
-    case K: ab_f64_launch<K>(C, S, a, b, ints, st); break;
+    case K: ab_f64_launch<K>(C, S, a, b, ints, st, half); break;
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
         CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19) CASE(20)
         CASE(21) CASE(22) CASE(23) CASE(24) CASE(25) CASE(26) CASE(27) CASE(28) CASE(29) CASE(30)
@@ -2015,7 +2057,7 @@ static void launch_ab(const double* C, double* S, const SpinTables& a, const Spi
     }
 }
 static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinTables& b,
-                      const DevInts& ints, hipStream_t st) {
+                      const DevInts& ints, hipStream_t st, bool half) {
     if (ab2_ok(a, b, sizeof(double))) {
         const int ks = std::max(1, (a.ns + 31) / 32);
         const int nmt = (b.np + 15) / 16;
@@ -2031,7 +2073,7 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
         if (layout == 3 && (nmt + 2) / 3 * ks <= 6) {
             const int tpw = (nmt + 2) / 3;
 #define MXCASE(K, T) \
-    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 3, 4>(C, S, a, b, ints, st);
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 3, 4>(C, S, a, b, ints, st, half);
             MXCASE(1, 1) MXCASE(1, 2) MXCASE(1, 3) MXCASE(1, 4) MXCASE(2, 1) MXCASE(2, 2) MXCASE(2, 3)
             MXCASE(3, 1) MXCASE(3, 2)
 #undef MXCASE
@@ -2039,7 +2081,7 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
         if (layout == 12 && (nmt + 3) / 4 * ks <= 6) {
             const int tpw = (nmt + 3) / 4;
 #define MXCASE(K, T) \
-    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 4, 3>(C, S, a, b, ints, st);
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 4, 3>(C, S, a, b, ints, st, half);
             MXCASE(1, 1) MXCASE(1, 2) MXCASE(1, 3) MXCASE(2, 1) MXCASE(2, 2) MXCASE(2, 3)
             MXCASE(3, 1) MXCASE(3, 2)
 #undef MXCASE
@@ -2047,14 +2089,14 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
         const size_t sh3 = size_t((b.nstr + 1) & ~int64_t(1)) * 8 + size_t(nmt) * ks * 2 * 64 * 16 + 1024 + 768;
         if (layout == 1 && sh3 <= 160 * 1024) {
 #define MXCASE(K, T) \
-    if (ks == K && nmt == T) return ab3_mixed_launch<K, T, 1, 8>(C, S, a, b, ints, st);
+    if (ks == K && nmt == T) return ab3_mixed_launch<K, T, 1, 8>(C, S, a, b, ints, st, half);
             MXCASE(1, 3) MXCASE(1, 4) MXCASE(1, 5) MXCASE(1, 6) MXCASE(1, 7) MXCASE(1, 8) MXCASE(1, 9) MXCASE(1, 10)
             MXCASE(2, 3) MXCASE(2, 4) MXCASE(2, 5) MXCASE(2, 6) MXCASE(2, 7) MXCASE(2, 8) MXCASE(2, 9) MXCASE(2, 10)
 #undef MXCASE
         }
         const int tpw = (nmt + 1) / 2;
 #define MXCASE(K, T) \
-    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 2, 4>(C, S, a, b, ints, st);
+    if (ks == K && tpw == T) return ab2_mixed_launch<K, T, 2, 4>(C, S, a, b, ints, st, half);
 #define MXROW(K) MXCASE(K, 1) MXCASE(K, 2) MXCASE(K, 3) MXCASE(K, 4) MXCASE(K, 5) MXCASE(K, 6)
         MXROW(1) MXROW(2) MXROW(3)
 #undef MXROW
@@ -2064,11 +2106,11 @@ static void launch_ab(const float* C, float* S, const SpinTables& a, const SpinT
     int ks = std::max(1, (a.ns + 31) / 32);
     if ((b.np + 15) / 16 > 4 * AB_MTW) throw Error(-1, "too many orbitals for the alpha-beta kernel");
     switch (ks) {
-        case 1: ab_mixed_launch<1>(C, S, a, b, ints, st); break;
-        case 2: ab_mixed_launch<2>(C, S, a, b, ints, st); break;
-        case 3: ab_mixed_launch<3>(C, S, a, b, ints, st); break;
-        case 4: ab_mixed_launch<4>(C, S, a, b, ints, st); break;
-        case 5: ab_mixed_launch<5>(C, S, a, b, ints, st); break;
+        case 1: ab_mixed_launch<1>(C, S, a, b, ints, st, half); break;
+        case 2: ab_mixed_launch<2>(C, S, a, b, ints, st, half); break;
+        case 3: ab_mixed_launch<3>(C, S, a, b, ints, st, half); break;
+        case 4: ab_mixed_launch<4>(C, S, a, b, ints, st, half); break;
+        case 5: ab_mixed_launch<5>(C, S, a, b, ints, st, half); break;
         default: throw Error(-1, "alpha-beta kernel: too many single excitations");
     }
 }
@@ -2251,7 +2293,7 @@ void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, 
     if (timing_) HIPCHK(hipEventRecord(w_->ev[1], st));
     if (a.d > 0) launch_clique<T>(C, S, Nb, a, *ints_, 0, true, *w_, st, eext, eext_bytes);
     if (timing_) HIPCHK(hipEventRecord(w_->ev[2], st));
-    if (a.k > 0 && b.k > 0) launch_ab(C, S, a, b, *ints_, st);
+    if (a.k > 0 && b.k > 0) launch_ab(C, S, a, b, *ints_, st, false);
     if (timing_) {
         HIPCHK(hipEventRecord(w_->ev[3], st));
         HIPCHK(hipEventSynchronize(w_->ev[3]));
@@ -2262,6 +2304,39 @@ void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, 
         }
     }
 }
+
+// Ms = 0 with C^T = eta C: sigma = e_core C + H_aa C + H_bb C + H_ab C where H_bb C = eta (H_aa C)^T
+// (the beta same-spin operator is the alpha one on the transposed vector, and the two spins share
+// strings and integrals) and H_ab C is eta-symmetric. Built as S = 1/2 e_core C + H_aa C + 1/2 H_ab C
+// (packed integrals scaled by 1/2, exact), then S <- S + eta S^T; the eta-symmetric part of H_ab C
+// replaces H_ab C itself, which differs from it only by rounding.
+template <class T>
+void SigmaEngine::apply_sym_t(const T* C, T* S, int eta, hipStream_t st) {
+    const SpinTables& a = *a_;
+    const SpinTables& b = *b_;
+    if (a.nstr != b.nstr || a.k != b.k) throw Error(-1, "sigma: the spin-flip form needs equal alpha and beta strings");
+    if (eta != 1 && eta != -1) throw Error(-1, "sigma: eta must be +1 or -1");
+    const int64_t N = a.nstr, nd = N * N;
+    if (timing_) HIPCHK(hipEventRecord(w_->ev[0], st));
+    k_scale_copy<T><<<nblk(nd, 256), 256, 0, st>>>(C, S, nd, T(0.5 * ints_->e_core));
+    HIPCHK(hipGetLastError());
+    if (timing_) HIPCHK(hipEventRecord(w_->ev[1], st));
+    if (a.d > 0) launch_clique<T>(C, S, N, a, *ints_, 0, true, *w_, st, nullptr, 0);
+    if (timing_) HIPCHK(hipEventRecord(w_->ev[2], st));
+    if (a.k > 0) launch_ab(C, S, a, b, *ints_, st, true);
+    sym_add_device<T>(S, N, eta, false, st);
+    if (timing_) {
+        HIPCHK(hipEventRecord(w_->ev[3], st));
+        HIPCHK(hipEventSynchronize(w_->ev[3]));
+        for (int i = 0; i < 3; ++i) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, w_->ev[i], w_->ev[i + 1]));
+            ms_[i] = ms;
+        }
+    }
+}
+void SigmaEngine::apply_sym(const double* C, double* S, int eta, hipStream_t st) { apply_sym_t<double>(C, S, eta, st); }
+void SigmaEngine::apply_sym(const float* C, float* S, int eta, hipStream_t st) { apply_sym_t<float>(C, S, eta, st); }
 
 void SigmaEngine::apply(const double* C, double* S, hipStream_t st) { apply_t<double>(C, S, st, nullptr, 0); }
 void SigmaEngine::apply(const float* C, float* S, hipStream_t st) { apply_t<float>(C, S, st, nullptr, 0); }

@@ -391,6 +391,13 @@ class Solver {
         std::vector<float> v32(pk.vpk.begin(), pk.vpk.end());
         di_.vpk.alloc(pk.vpk.size()); di_.vpk.upload(pk.vpk.data(), pk.vpk.size());
         di_.vpk32.alloc(v32.size()); di_.vpk32.upload(v32.data(), v32.size());
+        {  // halved copies (exact) for the spin-flip-symmetric sigma
+            std::vector<double> vh(pk.vpk.begin(), pk.vpk.end());
+            for (double& x : vh) x *= 0.5;
+            std::vector<float> vh32(vh.begin(), vh.end());
+            di_.vpk_h.alloc(vh.size()); di_.vpk_h.upload(vh.data(), vh.size());
+            di_.vpk32_h.alloc(vh32.size()); di_.vpk32_h.upload(vh32.data(), vh32.size());
+        }
         for (int spin = 0; spin < 2; ++spin) {
             const SpinTables& sp = spin ? sb_ : sa_;
             const std::vector<double>& g = pk.g2[spin];
@@ -548,6 +555,10 @@ class Solver {
     DBuf<float> c32_, s32_;
     std::vector<double> energies_, spin2_;
     bool solved_ = false;
+    // Ms = 0 spin-flip symmetry of the current solve: C^T = eta C, eta = (-1)^S (0: not used)
+    int eta_ = 0;
+    DBuf<double> hsym_;  // the H diagonal made exactly symmetric (Davidson-Liu preconditioner)
+    std::vector<SparseVec> bad_roots_;  // lower roots of other multiplicities from the last guess
     int stat_iter_ = 0, stat_nsigma_ = 0, stat_nsigma64_ = 0, stat_conv_ = 0, stat_fellback_ = 0;
     double stat_sec_ = 0.0;
     std::string err_;
@@ -574,6 +585,33 @@ class Solver {
         return dl_->eigenvector_device(root);
     }
     void sigma64(const double* C, double* S) { eng_->apply(C, S, 0); }
+    // the spin-flip symmetry applies: Ms = 0, an odd multiplicity, equal alpha and beta integrals,
+    // determinant basis (CSF vectors are spin-adapted by other means); eta = (-1)^S, the sign of
+    // C^T = eta C for a state of spin S in this string order (C[Ia][Ib] alpha-major)
+    int ms0_eta(bool csf) const {
+        if (!opt_.ms0_symmetry || csf || na_ != nb_ || opt_.multiplicity % 2 == 0) return 0;
+        if (hi_.oei_a != hi_.oei_b || hi_.tei_aa != hi_.tei_bb) return 0;
+        if (sa_.nstr != sb_.nstr || ndet_ < 2) return 0;
+        return ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
+    }
+    // a sparse vector made exactly eta-symmetric: entries of (Ia, Ib) and (Ib, Ia) become
+    // s = (c + eta c') / 2 and eta s
+    SparseVec symmetrize_sparse(const SparseVec& v, int eta) const {
+        const int64_t N = sa_.nstr;
+        std::map<int64_t, double> m;
+        for (const auto& [i, c] : v) m[i] = c;  // later duplicates overwrite, as set_vector
+        std::map<int64_t, double> out;
+        for (const auto& [i, c] : m) {
+            const int64_t j = (i % N) * N + i / N;
+            if (j < i && m.count(j)) continue;  // the pair was handled at j
+            auto it = m.find(j);
+            const double cj = it == m.end() ? 0.0 : it->second;
+            const double sv = 0.5 * (c + eta * cj);
+            out[i] = sv;
+            out[j] = eta * sv;
+        }
+        return SparseVec(out.begin(), out.end());
+    }
     void sigma_mixed(const double* C, double* S);
     void sigma_mixed32(const double* C, float* S);
     void compute_hdiag() {
@@ -788,6 +826,20 @@ double Solver::compute_energy() {
     dl_->set_olsen(opt_.preconditioner == FCI_PRECOND_OLSEN);
     const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, basis));
     compute_hdiag();
+    eta_ = ms0_eta(csf);
+    if (eta_) {
+        // every vector of the solve is kept exactly eta-symmetric: the diagonal (equal for a
+        // determinant and its mirror, up to rounding) made symmetric, the guesses and the roots to
+        // project out symmetrized, custom corrections and random vectors projected
+        if (hsym_.n < size_t(ndet_)) hsym_.alloc(ndet_);
+        HIPCHK(hipMemcpy(hsym_.p, hdiag_.p, size_t(ndet_) * sizeof(double), hipMemcpyDeviceToDevice));
+        sym_add_device<double>(hsym_.p, sa_.nstr, 1, true, 0);
+        const int eta = eta_;
+        const int64_t N = sa_.nstr;
+        dl_->set_symmetrizer([eta, N](double* v) { sym_add_device<double>(v, N, eta, true, 0); });
+    } else {
+        dl_->set_symmetrizer(nullptr);
+    }
     if (csf) {
         compute_hdiag_csf();
         dl_->add_h_diag_device(csf_hd_.p, basis);
@@ -806,14 +858,35 @@ double Solver::compute_energy() {
             dl_->add_project_out_vectors({});
         }
     } else {
-        dl_->add_h_diag_device(hdiag_.p, ndet_);
+        dl_->add_h_diag_device(eta_ ? hsym_.p : hdiag_.p, ndet_);
         bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= nspace_;
         dl_->set_mask(masked() ? spacedet_.p : nullptr);
+        if (!(first_run || use_initial_guess) && eta_ && !dl_->symmetrize_basis()) use_initial_guess = true;
         if (first_run || use_initial_guess) {
             dl_->reset();
             auto [g, bad] = initial_guess(num_guess_states);
+            if (eta_)
+                for (auto& v : g) v = symmetrize_sparse(v, eta_);
             dl_->add_guesses(g);
-            dl_->add_project_out_vectors(bad);
+            bad_roots_ = std::move(bad);
+        }
+        // the roots to project out, kept from the last guess and set for every solve, so that a warm
+        // restart uses them in the form of its own subspace
+        if (eta_) {
+            // roots of the other spin parity vanish under the projection; the rest are renormalized
+            std::vector<SparseVec> keep;
+            for (const auto& v : bad_roots_) {
+                SparseVec w = symmetrize_sparse(v, eta_);
+                double n2 = 0.0;
+                for (const auto& e : w) n2 += e.second * e.second;
+                if (n2 < 1e-12) continue;
+                const double f = 1.0 / std::sqrt(n2);
+                for (auto& e : w) e.second *= f;
+                keep.push_back(std::move(w));
+            }
+            dl_->add_project_out_vectors(keep);
+        } else {
+            dl_->add_project_out_vectors(bad_roots_);
         }
     }
     int nsig64 = 0, nsig = 0, iters = 0;
@@ -846,6 +919,14 @@ double Solver::compute_energy() {
                 csf_->to_det<float>(b, c32_.p, ndet_, 0);
                 eng_->apply(c32_.p, s32_.p, 0);
                 csf_->to_csf<float, float>(s32_.p, s, 0);
+            });
+        } else if (eta_) {
+            dl_->add_sigma_builder32([this](const double* b, float* s) {
+                if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
+                k_f64_to_f32<<<unsigned((ndet_ + 255) / 256), 256>>>(b, c32_.p, ndet_);
+                HIPCHK(hipGetLastError());
+                eng_->apply_sym(c32_.p, s, eta_, 0);
+                mask_sigma(s);
             });
         } else {
             dl_->add_sigma_builder32([this](const double* b, float* s) {
@@ -884,6 +965,11 @@ double Solver::compute_energy() {
             csf_->to_det<double>(b, csf_wb_.p, ndet_, 0);
             eng_->apply(csf_wb_.p, csf_ws_.p, 0);
             csf_->to_csf<double, double>(csf_ws_.p, s, 0);
+        });
+    } else if (eta_) {
+        dl_->add_sigma_builder([this](const double* b, double* s) {
+            eng_->apply_sym(b, s, eta_, 0);
+            mask_sigma(s);
         });
     } else {
         dl_->add_sigma_builder([this](const double* b, double* s) {
@@ -959,6 +1045,7 @@ int64_t Solver::lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, 
         auto [g, bad] = initial_guess(num_guess_states);
         dl_->add_guesses(g);
         dl_->add_project_out_vectors(bad);
+        bad_roots_ = bad;
     }
     uint64_t rng = seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
     auto next = [&rng]() {
@@ -1094,6 +1181,7 @@ void fci_default_options(fci_options* o) {
     o->mixed_switch = 1.0e-4;
     o->spin_adapt = 0;                      // CI_SPIN_ADAPT (options.yaml)
     o->spin_adapt_full_preconditioner = 0;  // CI_SPIN_ADAPT_FULL_PRECONDITIONER
+    o->ms0_symmetry = 1;
 }
 
 int fci_create(int n, int na, int nb, const double* oa, const double* ob, const double* taa, const double* tab,
@@ -1161,6 +1249,12 @@ int fci_solve_stats(const fci_solver* s, int* it, int* ns, int* ns64, double* se
     if (sec) *sec = s->s->stat_sec_;
     if (conv) *conv = s->s->stat_conv_;
     if (fb) *fb = s->s->stat_fellback_;
+    return FCI_OK;
+}
+
+int fci_solve_ms0(const fci_solver* s, int* eta) {
+    if (!s || !eta) return FCI_ERR_ARG;
+    *eta = s->s->eta_;
     return FCI_OK;
 }
 
@@ -1462,6 +1556,26 @@ int fci_sigma(fci_solver* s, const double* C, double* sigma) {
         S.sigma64(dc.p, ds.p);
         HIPCHK(hipStreamSynchronize(0));
         ds.download(sigma, S.ndet_);
+    });
+}
+
+int fci_sigma_ms0_device(fci_solver* s, const void* C, void* sigma, int dtype, int eta, void* stream) {
+    if (!s || !C || !sigma) return FCI_ERR_ARG;
+    return guarded(&s->err, [&] {
+        fci::Solver& S = *s->s;
+        if (eta != 1 && eta != -1) throw Error(FCI_ERR_ARG, "eta must be +1 or -1");
+        if (S.na_ != S.nb_) throw Error(FCI_ERR_ARG, "the spin-flip sigma needs as many alpha as beta electrons");
+        if (S.hi_.oei_a != S.hi_.oei_b || S.hi_.tei_aa != S.hi_.tei_bb)
+            throw Error(FCI_ERR_ARG, "the spin-flip sigma needs equal alpha and beta integrals");
+        S.eng_->set_timing(true);
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        if (dtype == 0)
+            S.eng_->apply_sym(static_cast<const double*>(C), static_cast<double*>(sigma), eta, st);
+        else if (dtype == 1)
+            S.eng_->apply_sym(static_cast<const float*>(C), static_cast<float*>(sigma), eta, st);
+        else
+            throw Error(FCI_ERR_ARG, "dtype must be 0 (FP64) or 1 (FP32)");
+        S.eng_->set_timing(false);
     });
 }
 

@@ -252,6 +252,7 @@ class FCISolver {
             else if (k == "PRINT") o_.print = v.cast<int>();
             else if (k == "CI_SPIN_ADAPT") o_.spin_adapt = v.cast<bool>() ? 1 : 0;
             else if (k == "CI_SPIN_ADAPT_FULL_PRECONDITIONER") o_.spin_adapt_full_preconditioner = v.cast<bool>() ? 1 : 0;
+            else if (k == "FCI_MS0_SYMMETRY") o_.ms0_symmetry = v.cast<bool>() ? 1 : 0;
             else if (k == "FCI_TEST_RDMS") {
                 // the brute-force RDM self test is the test suite's job here
             } else if (k == "FCI_PRECISION") set_precision(v.cast<std::string>());
@@ -300,6 +301,9 @@ class FCISolver {
         d["seconds"] = sec;
         d["converged"] = bool(conv);
         d["fell_back"] = bool(fb);
+        int eta = 0;
+        fci_solve_ms0(h_, &eta);
+        d["ms0_symmetry"] = eta;
         return d;
     }
     py::array_t<double> evec(int root) const {
@@ -325,6 +329,8 @@ class FCISolver {
     // CSF spin adaptation (CI_SPIN_ADAPT; FCISolver::set_spin_adapt, fci_solver.cc:75-79)
     void set_spin_adapt(bool v) { o_.spin_adapt = v ? 1 : 0; push(); }
     void set_spin_adapt_full_preconditioner(bool v) { o_.spin_adapt_full_preconditioner = v ? 1 : 0; push(); }
+    // the Ms = 0 spin-flip symmetry of the determinant vectors (not in the reference; on by default)
+    void set_ms0_symmetry(bool v) { o_.ms0_symmetry = v ? 1 : 0; push(); }
     std::pair<int64_t, int64_t> csf_size() {
         int64_t nc = 0, nz = 0;
         check(fci_csf_size(h_, &nc, &nz), fci_last_error(h_));
@@ -458,6 +464,14 @@ class FCISolver {
             throw std::runtime_error("sigma_arrays: buffers must hold ndet elements of one dtype");
         check(fci_sigma_device(h_, reinterpret_cast<const void*>(C.ptr()), reinterpret_cast<void*>(S.ptr()),
                                C.dtype_code(), nullptr),
+              fci_last_error(h_));
+        check(fci_device_synchronize(), fci_last_error(nullptr));
+    }
+    void sigma_ms0_arrays(const DeviceArray& C, DeviceArray& S, int eta) {
+        if (C.size() != fci_ndet(h_) || S.size() != C.size() || C.dtype_code() != S.dtype_code())
+            throw std::runtime_error("sigma_ms0_arrays: buffers must hold ndet elements of one dtype");
+        check(fci_sigma_ms0_device(h_, reinterpret_cast<const void*>(C.ptr()), reinterpret_cast<void*>(S.ptr()),
+                                   C.dtype_code(), eta, nullptr),
               fci_last_error(h_));
         check(fci_device_synchronize(), fci_last_error(nullptr));
     }
@@ -690,6 +704,8 @@ PYBIND11_MODULE(_forte_amd, m) {
         .def("space_size", &FCISolver::space_size)
         .def("set_spin_adapt", &FCISolver::set_spin_adapt)
         .def("set_spin_adapt_full_preconditioner", &FCISolver::set_spin_adapt_full_preconditioner)
+        .def("set_ms0_symmetry", &FCISolver::set_ms0_symmetry)
+        .def("sigma_ms0_arrays", &FCISolver::sigma_ms0_arrays, py::arg("C"), py::arg("S"), py::arg("eta"))
         .def("csf_size", &FCISolver::csf_size)
         .def("ncsf", &FCISolver::ncsf)
         .def("det_to_csf", &FCISolver::det_to_csf)

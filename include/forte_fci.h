@@ -72,6 +72,10 @@ typedef struct fci_options {
     double mixed_switch;     /* mixed precision: residual norm at which FP64 refinement starts */
     int spin_adapt;          /* CI_SPIN_ADAPT: Davidson-Liu in the CSF basis of the multiplicity */
     int spin_adapt_full_preconditioner; /* CI_SPIN_ADAPT_FULL_PRECONDITIONER: <CSF|H|CSF> diagonal */
+    int ms0_symmetry;        /* no counterpart in the reference (default 1): with na == nb, an odd
+                                multiplicity and equal alpha / beta integrals, keep the vectors in the
+                                spin-flip-symmetric subspace C^T = (-1)^S C and build sigma without its
+                                beta same-spin term (fci_sigma_ms0_device); 0 = always the full sigma */
 } fci_options;
 
 void fci_default_options(fci_options* opts);
@@ -129,6 +133,9 @@ int fci_energies(const fci_solver* s, double* energies, double* spin2);
  * converged flag, learned-preconditioner fallback flag. */
 int fci_solve_stats(const fci_solver* s, int* iterations, int* nsigma, int* nsigma_fp64,
                     double* seconds, int* converged, int* fell_back);
+/* No counterpart in the reference: the spin-flip sign eta (+1 / -1) the last solve kept its vectors
+ * symmetric under (C^T = eta C), or 0 when it used the full sigma. */
+int fci_solve_ms0(const fci_solver* s, int* eta);
 /* FCISolver::evecs row `root` (fci_solver.cc:79-96), ndet doubles, host buffer */
 int fci_evec(const fci_solver* s, int root, double* out);
 int64_t fci_ndet(const fci_solver* s);
@@ -190,6 +197,13 @@ int fci_sigma_device(fci_solver* s, const void* C_dev, void* sigma_dev, int dtyp
  * overlap on one GPU (other streams or threads) race. */
 #define FCI_NOALLOC_SCRATCH_BYTES (64u << 20)
 int fci_sigma_device_noalloc(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, void* stream);
+/* No counterpart in the reference: sigma = H C for an Ms = 0 vector with C^T = eta C (eta = +1 or
+ * -1; states of spin S have eta = (-1)^S in this determinant order), on device buffers as
+ * fci_sigma_device. The beta same-spin term is eta times the transposed alpha one and is not built;
+ * the alpha-beta term enters through its eta-symmetric part, so sigma is exactly eta-symmetric.
+ * Needs na == nb and equal alpha and beta integrals (FCI_ERR_ARG otherwise); C must be
+ * eta-symmetric (not checked: for other C the result is not H C). */
+int fci_sigma_ms0_device(fci_solver* s, const void* C_dev, void* sigma_dev, int dtype, int eta, void* stream);
 /* Per-kernel device time (ms) of the last fci_sigma_device call measured with HIP events:
  * [0] same-spin beta (incl. transposes), [1] same-spin alpha, [2] alpha-beta. */
 int fci_sigma_timings(const fci_solver* s, double* ms3);

@@ -44,7 +44,7 @@ def test_default_options_without_gpu():
                     ("r_convergence", ctypes.c_double), ("die_if_not_converged", ctypes.c_int),
                     ("print", ctypes.c_int), ("precision", ctypes.c_int), ("preconditioner", ctypes.c_int),
                     ("mixed_switch", ctypes.c_double), ("spin_adapt", ctypes.c_int),
-                    ("spin_adapt_full_preconditioner", ctypes.c_int)]
+                    ("spin_adapt_full_preconditioner", ctypes.c_int), ("ms0_symmetry", ctypes.c_int)]
     o = Opts()
     lib.fci_default_options(ctypes.byref(o))
     # forte/options.yaml DL_* defaults and E/R convergence of ActiveSpaceSolver
@@ -52,6 +52,8 @@ def test_default_options_without_gpu():
             o.maxiter) == (1, 1, 50, 2, 10, 100)
     assert o.e_convergence == 1e-10 and o.r_convergence == 1e-6
     assert (o.spin_adapt, o.spin_adapt_full_preconditioner) == (0, 0)  # CI_SPIN_ADAPT defaults
+    assert o.ms0_symmetry == 1
+    assert ctypes.sizeof(Opts) == 88
 
 
 def test_seeded_integrals_match_oracle(oracle):
