@@ -49,14 +49,6 @@ class DavidsonLiu {
     void set_olsen(bool v) { olsen_ = v; }
     // determinants allowed to be nonzero (point-group sector); random vectors are zeroed elsewhere
     void set_mask(const uint8_t* d_mask) { mask_ = d_mask; }
-    // projection onto a symmetry subspace that the sigma builder, the diagonal and the guesses
-    // respect exactly (not in the reference): applied to the vectors made outside the
-    // Davidson-Liu algebra, i.e. corrections from a custom preconditioner and random vectors
-    using SymFn = std::function<void(double* v)>;
-    void set_symmetrizer(SymFn f) { symm_ = std::move(f); }
-    // a kept basis (warm restart) projected with the symmetrizer and orthonormalized again;
-    // false if fewer than nroot vectors survive
-    bool symmetrize_basis();
     // called after the residual norms of an iteration are known; returning true ends the solve
     // as converged (used to end a reduced-precision phase once it stalls)
     using IterFn = std::function<bool(int iter, const std::vector<double>& resid, const std::vector<double>& lambda)>;
@@ -98,7 +90,6 @@ class DavidsonLiu {
     DBuf<double> olsen_x_;
     void olsen_correct(int k);
     const uint8_t* mask_ = nullptr;
-    SymFn symm_;
     bool have_hdiag_ = false;
     DBuf<double> hdiag_, b_, sigma_v_, r_, temp_;
     DBuf<double> part_;  // reduction partials

@@ -808,23 +808,8 @@ void DavidsonLiu::fill_random(double* v, int nvec) {
         rng_ += 0x9E3779B97F4A7C15ULL;
         k_random<<<nblk(size_, 256), 256>>>(v + size_t(k) * size_, size_, rng_);
         if (mask_) k_apply_mask<<<nblk(size_, 256), 256>>>(v + size_t(k) * size_, mask_, size_);
-        if (symm_) symm_(v + size_t(k) * size_);
     }
     HIPCHK(hipGetLastError());
-}
-
-bool DavidsonLiu::symmetrize_basis() {
-    if (!symm_ || basis_size_ == 0) return basis_size_ >= nroot_;
-    const int bs = basis_size_;
-    HIPCHK(hipMemcpy(temp_.p, b_.p, size_t(bs) * size_ * sizeof(double), hipMemcpyDeviceToDevice));
-    for (int k = 0; k < bs; ++k) symm_(temp_.p + size_t(k) * size_);
-    std::fill(S_.begin(), S_.end(), 0.0);
-    basis_size_ = 0;
-    const int added = add_rows_and_orthonormalize(b_.p, 0, temp_.p, bs);
-    basis_size_ = added;
-    sigma_size_ = 0;
-    gvalid_ = 0;
-    return added >= nroot_;
 }
 
 void DavidsonLiu::form_G() {
@@ -976,8 +961,6 @@ bool DavidsonLiu::solve() {
                 for (int k = 0; k < nroot_; ++k)
                     k_diag_precond<<<nblk(size_, 256), 256>>>(r_.p + size_t(k) * size_, hdiag_.p, lambda_[k], size_);
                 HIPCHK(hipGetLastError());
-            } else if (symm_) {
-                for (int k = 0; k < nroot_; ++k) symm_(r_.p + size_t(k) * size_);
             }
         }
         // project_out_roots(r_), normalize_vectors(r_, nroot_), :457-465

@@ -192,6 +192,9 @@ class SigmaEngine {
     // exactly (bitwise) eta-symmetric.
     void apply_sym(const double* C, double* S, int eta, hipStream_t st);
     void apply_sym(const float* C, float* S, int eta, hipStream_t st);
+    // the first half: S (holding 1/2 e_core C on entry) += H_aa C + 1/2 H_ab C; sigma = S + eta S^T
+    void apply_sym_half(const double* C, double* S, int eta, hipStream_t st);
+    void apply_sym_half(const float* C, float* S, int eta, hipStream_t st);
     // smallest external scratch one sigma needs with w-byte elements
     size_t scratch_min_bytes(size_t w) const;
     void timings(double* ms3) const;
@@ -199,7 +202,7 @@ class SigmaEngine {
 
   private:
     template <class T> void apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, size_t ext_bytes);
-    template <class T> void apply_sym_t(const T* C, T* S, int eta, hipStream_t st);
+    template <class T> void apply_sym_t(const T* C, T* S, int eta, hipStream_t st, bool init, bool finish);
     const SpinTables* a_;
     const SpinTables* b_;
     const DevInts* ints_;
@@ -247,6 +250,25 @@ void transpose_add_device(const T* in, T* out, int64_t rows, int64_t cols, const
 // so the result is exactly eta-symmetric); with avg, S <- (S + eta S^T) / 2
 template <class T>
 void sym_add_device(T* S, int64_t N, int eta, bool avg, hipStream_t st);
+// Ms = 0 vectors with C^T = eta C in packed coordinates: the upper triangle of C[Ia][Ib] (N x N),
+// entry (i, j), i < j, = (C_ij + eta C_ji) / sqrt2, entry (i, i) = C_ii (eta = +1 only; eta = -1
+// vectors have a zero diagonal and no diagonal entries). These are the coordinates in an
+// orthonormal basis of the eta-symmetric subspace, so dot products and norms are preserved.
+int64_t ms0_packed_size(int64_t N, int eta);
+int64_t ms0_packed_index(int64_t i, int64_t j, int64_t N, int eta);
+// packed(i, j) = s_off (F_ij + mirror F_ji) for i < j, s_diag F_ii on the diagonal (vectors:
+// mirror = eta, s_off = 1/sqrt2, s_diag = 1; a symmetric array such as the H diagonal: mirror = +1,
+// s_off = 1/2, s_diag = 1; the packed sigma of the first half S of the symmetric form, sigma =
+// S + eta S^T: mirror = eta, s_off = sqrt2, s_diag = 2)
+template <class TI, class TO>
+void ms0_pack_device(const TI* full, TO* packed, int64_t N, int eta, int mirror, double s_off, double s_diag,
+                     hipStream_t st);
+// the eta-symmetric full vector of packed coordinates (exactly symmetric in the output precision);
+// with s_init, also s_init = s_scale * full
+template <class TI, class TO>
+void ms0_unpack_device(const TI* packed, TO* full, int64_t N, int eta, hipStream_t st, TO* s_init = nullptr,
+                       double s_scale = 0.0);
+void ms0_pack_mask_device(const uint8_t* full, uint8_t* packed, int64_t N, int eta, hipStream_t st);
 
 // ---- CSF spin adaptation (fci_csf.hip) -----------------------------------------------------------
 // a determinant as its alpha / beta occupation bits

@@ -186,6 +186,118 @@ template <class T> void sym_add_device(T* S, int64_t N, int eta, bool avg, hipSt
 template void sym_add_device<double>(double*, int64_t, int, bool, hipStream_t);
 template void sym_add_device<float>(float*, int64_t, int, bool, hipStream_t);
 
+// ---- spin-flip-symmetric vectors packed to the upper triangle ------------------------------------
+// Row i of the packed layout holds columns j = i + d .. N - 1 (d = 0: diagonal included, d = 1:
+// excluded), at offset i (N - d) - i (i - 1) / 2.
+__device__ __host__ inline int64_t ms0_row_off(int64_t i, int64_t N, int d) { return i * (N - d) - i * (i - 1) / 2; }
+int64_t ms0_packed_size(int64_t N, int eta) { return eta == 1 ? N * (N + 1) / 2 : N * (N - 1) / 2; }
+int64_t ms0_packed_index(int64_t i, int64_t j, int64_t N, int eta) {
+    const int d = eta == 1 ? 0 : 1;
+    return ms0_row_off(i, N, d) + (j - i - d);
+}
+
+// packed(i, j) = s_off (F_ij + mirror F_ji) for i < j, s_diag F_ii on the diagonal (d = 0); tile
+// pair (I, J), I <= J, per block, both tiles of F read into LDS
+template <class TI, class TO>
+__global__ __launch_bounds__(256) void k_ms0_pack(const TI* __restrict__ F, TO* __restrict__ P, int64_t N, int d,
+                                                  int mirror, double s_off, double s_diag) {
+    const int I = blockIdx.y, J = blockIdx.x;
+    if (J < I) return;
+    __shared__ TI ta[64][65], tb[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t r0 = int64_t(I) * 64, c0 = int64_t(J) * 64;
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t r = r0 + j, c = c0 + tx;
+        if (r < N && c < N) ta[j][tx] = F[r * N + c];
+        const int64_t r2 = c0 + j, c2 = r0 + tx;
+        if (I != J && r2 < N && c2 < N) tb[j][tx] = F[r2 * N + c2];
+    }
+    __syncthreads();
+    const TI m = TI(mirror), so = TI(s_off), sd = TI(s_diag);
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t i = r0 + j, c = c0 + tx;
+        if (i >= N || c >= N || c < i + d) continue;
+        const TI a = ta[j][tx];
+        TI v;
+        if (c == i) {
+            v = sd * a;
+        } else {
+            const TI b = I != J ? tb[tx][j] : ta[tx][j];
+            v = so * (a + m * b);
+        }
+        P[ms0_row_off(i, N, d) + (c - i - d)] = TO(v);
+    }
+}
+
+// F_ij = P(i, j) / sqrt2 and F_ji = eta F_ij for i < j, F_ii = P(i, i) (d = 0) or 0 (d = 1); a
+// mirrored pair is converted once, so F is exactly eta-symmetric in its own precision
+template <class TI, class TO>
+__global__ __launch_bounds__(256) void k_ms0_unpack(const TI* __restrict__ P, TO* __restrict__ F, int64_t N, int d,
+                                                    int eta, TO* __restrict__ S, TO s_scale) {
+    const int I = blockIdx.y, J = blockIdx.x;
+    if (J < I) return;
+    __shared__ TO tp[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t r0 = int64_t(I) * 64, c0 = int64_t(J) * 64;
+    const TI h = TI(0.70710678118654752440);
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t i = r0 + j, c = c0 + tx;
+        TO v = TO(0);
+        if (i < N && c < N && c >= i + d) {
+            const TI x = P[ms0_row_off(i, N, d) + (c - i - d)];
+            v = c == i ? TO(x) : TO(x * h);
+        }
+        tp[j][tx] = v;
+    }
+    __syncthreads();
+    const TO e = TO(eta);
+    for (int j = ty; j < 64; j += 4) {
+        const int64_t i = r0 + j, c = c0 + tx;
+        if (i < N && c < N) {
+            // upper part of tile (I, J); in the diagonal tile the lower part mirrors the upper one
+            TO v;
+            if (c >= i) v = tp[j][tx];
+            else v = e * tp[tx][j];
+            F[i * N + c] = v;
+            if (S) S[i * N + c] = s_scale * v;
+        }
+        const int64_t i2 = c0 + j, c2 = r0 + tx;  // tile (J, I): the mirror
+        if (I != J && i2 < N && c2 < N) {
+            const TO v = e * tp[tx][j];
+            F[i2 * N + c2] = v;
+            if (S) S[i2 * N + c2] = s_scale * v;
+        }
+    }
+}
+
+__global__ void k_ms0_pack_mask(const uint8_t* __restrict__ F, uint8_t* __restrict__ P, int64_t N, int d) {
+    const int64_t i = blockIdx.y, c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (c >= N || c < i + d) return;
+    P[ms0_row_off(i, N, d) + (c - i - d)] = F[i * N + c];
+}
+
+template <class TI, class TO>
+void ms0_pack_device(const TI* full, TO* packed, int64_t N, int eta, int mirror, double s_off, double s_diag,
+                     hipStream_t st) {
+    const unsigned t = unsigned((N + 63) / 64);
+    k_ms0_pack<TI, TO><<<dim3(t, t), 256, 0, st>>>(full, packed, N, eta == 1 ? 0 : 1, mirror, s_off, s_diag);
+    HIPCHK(hipGetLastError());
+}
+template <class TI, class TO>
+void ms0_unpack_device(const TI* packed, TO* full, int64_t N, int eta, hipStream_t st, TO* s_init, double s_scale) {
+    const unsigned t = unsigned((N + 63) / 64);
+    k_ms0_unpack<TI, TO><<<dim3(t, t), 256, 0, st>>>(packed, full, N, eta == 1 ? 0 : 1, eta, s_init, TO(s_scale));
+    HIPCHK(hipGetLastError());
+}
+void ms0_pack_mask_device(const uint8_t* full, uint8_t* packed, int64_t N, int eta, hipStream_t st) {
+    k_ms0_pack_mask<<<dim3(unsigned((N + 255) / 256), unsigned(N)), 256, 0, st>>>(full, packed, N, eta == 1 ? 0 : 1);
+    HIPCHK(hipGetLastError());
+}
+template void ms0_pack_device<double, double>(const double*, double*, int64_t, int, int, double, double, hipStream_t);
+template void ms0_pack_device<float, float>(const float*, float*, int64_t, int, int, double, double, hipStream_t);
+template void ms0_unpack_device<double, double>(const double*, double*, int64_t, int, hipStream_t, double*, double);
+template void ms0_unpack_device<double, float>(const double*, float*, int64_t, int, hipStream_t, float*, double);
+
 // ---- same-spin pair-clique GEMM ----------------------------------------------------------------
 constexpr int CQ_TW = 64;   // columns per block (4 waves x 16)
 constexpr int CQ_MT = 48;   // output members per block (3 MFMA row tiles)
@@ -2311,20 +2423,22 @@ void SigmaEngine::apply_t(const T* C, T* S, hipStream_t st, unsigned char* ext, 
 // (packed integrals scaled by 1/2, exact), then S <- S + eta S^T; the eta-symmetric part of H_ab C
 // replaces H_ab C itself, which differs from it only by rounding.
 template <class T>
-void SigmaEngine::apply_sym_t(const T* C, T* S, int eta, hipStream_t st) {
+void SigmaEngine::apply_sym_t(const T* C, T* S, int eta, hipStream_t st, bool init, bool finish) {
     const SpinTables& a = *a_;
     const SpinTables& b = *b_;
     if (a.nstr != b.nstr || a.k != b.k) throw Error(-1, "sigma: the spin-flip form needs equal alpha and beta strings");
     if (eta != 1 && eta != -1) throw Error(-1, "sigma: eta must be +1 or -1");
     const int64_t N = a.nstr, nd = N * N;
     if (timing_) HIPCHK(hipEventRecord(w_->ev[0], st));
-    k_scale_copy<T><<<nblk(nd, 256), 256, 0, st>>>(C, S, nd, T(0.5 * ints_->e_core));
-    HIPCHK(hipGetLastError());
+    if (init) {
+        k_scale_copy<T><<<nblk(nd, 256), 256, 0, st>>>(C, S, nd, T(0.5 * ints_->e_core));
+        HIPCHK(hipGetLastError());
+    }
     if (timing_) HIPCHK(hipEventRecord(w_->ev[1], st));
     if (a.d > 0) launch_clique<T>(C, S, N, a, *ints_, 0, true, *w_, st, nullptr, 0);
     if (timing_) HIPCHK(hipEventRecord(w_->ev[2], st));
     if (a.k > 0) launch_ab(C, S, a, b, *ints_, st, true);
-    sym_add_device<T>(S, N, eta, false, st);
+    if (finish) sym_add_device<T>(S, N, eta, false, st);
     if (timing_) {
         HIPCHK(hipEventRecord(w_->ev[3], st));
         HIPCHK(hipEventSynchronize(w_->ev[3]));
@@ -2335,8 +2449,18 @@ void SigmaEngine::apply_sym_t(const T* C, T* S, int eta, hipStream_t st) {
         }
     }
 }
-void SigmaEngine::apply_sym(const double* C, double* S, int eta, hipStream_t st) { apply_sym_t<double>(C, S, eta, st); }
-void SigmaEngine::apply_sym(const float* C, float* S, int eta, hipStream_t st) { apply_sym_t<float>(C, S, eta, st); }
+void SigmaEngine::apply_sym(const double* C, double* S, int eta, hipStream_t st) {
+    apply_sym_t<double>(C, S, eta, st, true, true);
+}
+void SigmaEngine::apply_sym(const float* C, float* S, int eta, hipStream_t st) {
+    apply_sym_t<float>(C, S, eta, st, true, true);
+}
+void SigmaEngine::apply_sym_half(const double* C, double* S, int eta, hipStream_t st) {
+    apply_sym_t<double>(C, S, eta, st, false, false);
+}
+void SigmaEngine::apply_sym_half(const float* C, float* S, int eta, hipStream_t st) {
+    apply_sym_t<float>(C, S, eta, st, false, false);
+}
 
 void SigmaEngine::apply(const double* C, double* S, hipStream_t st) { apply_t<double>(C, S, st, nullptr, 0); }
 void SigmaEngine::apply(const float* C, float* S, hipStream_t st) { apply_t<float>(C, S, st, nullptr, 0); }

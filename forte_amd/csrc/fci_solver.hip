@@ -556,8 +556,10 @@ class Solver {
     std::vector<double> energies_, spin2_;
     bool solved_ = false;
     // Ms = 0 spin-flip symmetry of the current solve: C^T = eta C, eta = (-1)^S (0: not used)
-    int eta_ = 0;
-    DBuf<double> hsym_;  // the H diagonal made exactly symmetric (Davidson-Liu preconditioner)
+    int eta_ = 0, dl_eta_ = 0;  // of the current solve / of the Davidson-Liu basis held in dl_
+    bool ms0_solved_ = false;
+    DBuf<double> ms0_hd_, ms0_wb_, ms0_ws_, ms0_ev_;  // packed diagonal, full work vectors, eigenvectors
+    DBuf<uint8_t> ms0_mask_;
     std::vector<SparseVec> bad_roots_;  // lower roots of other multiplicities from the last guess
     int stat_iter_ = 0, stat_nsigma_ = 0, stat_nsigma64_ = 0, stat_conv_ = 0, stat_fellback_ = 0;
     double stat_sec_ = 0.0;
@@ -581,7 +583,8 @@ class Solver {
     // eigenvector `root` in the determinant basis (device); spin-adapted solves convert theirs
     const double* evec_device(int root) const {
         if (csf_solved_) return csf_ev_.p + size_t(root) * ndet_;
-        if (!dl_ || dl_csf_) throw Error(-2, "no determinant-basis eigenvector is available");
+        if (ms0_solved_) return ms0_ev_.p + size_t(root) * ndet_;
+        if (!dl_ || dl_csf_ || dl_eta_) throw Error(-2, "no determinant-basis eigenvector is available");
         return dl_->eigenvector_device(root);
     }
     void sigma64(const double* C, double* S) { eng_->apply(C, S, 0); }
@@ -594,21 +597,22 @@ class Solver {
         if (sa_.nstr != sb_.nstr || ndet_ < 2) return 0;
         return ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
     }
-    // a sparse vector made exactly eta-symmetric: entries of (Ia, Ib) and (Ib, Ia) become
-    // s = (c + eta c') / 2 and eta s
-    SparseVec symmetrize_sparse(const SparseVec& v, int eta) const {
+    // a sparse determinant vector in the packed coordinates of the eta-symmetric subspace: pair
+    // (Ia, Ib), Ia < Ib -> (c + eta c') / sqrt2, diagonal (eta = +1) -> c
+    SparseVec pack_sparse(const SparseVec& v, int eta) const {
         const int64_t N = sa_.nstr;
         std::map<int64_t, double> m;
         for (const auto& [i, c] : v) m[i] = c;  // later duplicates overwrite, as set_vector
         std::map<int64_t, double> out;
-        for (const auto& [i, c] : m) {
-            const int64_t j = (i % N) * N + i / N;
-            if (j < i && m.count(j)) continue;  // the pair was handled at j
-            auto it = m.find(j);
-            const double cj = it == m.end() ? 0.0 : it->second;
-            const double sv = 0.5 * (c + eta * cj);
-            out[i] = sv;
-            out[j] = eta * sv;
+        for (const auto& [k, c] : m) {
+            const int64_t i = k / N, j = k % N;
+            if (i == j) {
+                if (eta == 1) out[ms0_packed_index(i, i, N, eta)] = c;
+                continue;
+            }
+            const int64_t lo = std::min(i, j), hi = std::max(i, j);
+            // c enters as C_lo,hi (i < j) or as eta C_hi,lo's mirror (i > j)
+            out[ms0_packed_index(lo, hi, N, eta)] += (i < j ? c : eta * c) * 0.70710678118654752440;
         }
         return SparseVec(out.begin(), out.end());
     }
@@ -812,34 +816,29 @@ double Solver::compute_energy() {
     auto t0 = std::chrono::steady_clock::now();
     solved_ = false;
     const int nroot = opt_.nroot;
-    // the Davidson-Liu basis: CSFs of the multiplicity (fci_solver.cc:186-201) or determinants
-    const int64_t basis = csf ? csf_space().ncsf() : ndet_;
-    if (basis == 0) throw Error(-1, "no CSF has the requested multiplicity");
+    // Ms = 0 spin-flip symmetry: the Davidson-Liu runs on packed coordinates of the eta-symmetric
+    // subspace (upper triangle of C[Ia][Ib]), a basis in which its algebra is that of the
+    // determinant solve restricted to the subspace, on half the data
+    eta_ = ms0_eta(csf);
+    const int64_t N = sa_.nstr;
+    // the Davidson-Liu basis: CSFs of the multiplicity (fci_solver.cc:186-201), packed symmetric
+    // coordinates or determinants
+    const int64_t basis =
+        csf ? csf_space().ncsf() : (eta_ ? (ms0_packed_size(N, eta_) + 1) / 2 * 2 : ndet_);
+    if (basis == 0) throw Error(-1, csf ? "no CSF has the requested multiplicity" : "empty determinant space");
     bool first_run = false;
-    if (!dl_ || dl_->nroot() != nroot || dl_->size() != basis || dl_csf_ != csf) {
+    if (!dl_ || dl_->nroot() != nroot || dl_->size() != basis || dl_csf_ != csf || dl_eta_ != eta_) {
         dl_.reset(new DavidsonLiu(basis, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
         dl_csf_ = csf;
+        dl_eta_ = eta_;
         first_run = true;
     }
     csf_solved_ = false;
+    ms0_solved_ = false;
     dl_->set_print_level(opt_.print);
     dl_->set_olsen(opt_.preconditioner == FCI_PRECOND_OLSEN);
     const int num_guess_states = int(std::min<int64_t>(int64_t(opt_.guess_per_root) * nroot, basis));
     compute_hdiag();
-    eta_ = ms0_eta(csf);
-    if (eta_) {
-        // every vector of the solve is kept exactly eta-symmetric: the diagonal (equal for a
-        // determinant and its mirror, up to rounding) made symmetric, the guesses and the roots to
-        // project out symmetrized, custom corrections and random vectors projected
-        if (hsym_.n < size_t(ndet_)) hsym_.alloc(ndet_);
-        HIPCHK(hipMemcpy(hsym_.p, hdiag_.p, size_t(ndet_) * sizeof(double), hipMemcpyDeviceToDevice));
-        sym_add_device<double>(hsym_.p, sa_.nstr, 1, true, 0);
-        const int eta = eta_;
-        const int64_t N = sa_.nstr;
-        dl_->set_symmetrizer([eta, N](double* v) { sym_add_device<double>(v, N, eta, true, 0); });
-    } else {
-        dl_->set_symmetrizer(nullptr);
-    }
     if (csf) {
         compute_hdiag_csf();
         dl_->add_h_diag_device(csf_hd_.p, basis);
@@ -858,25 +857,45 @@ double Solver::compute_energy() {
             dl_->add_project_out_vectors({});
         }
     } else {
-        dl_->add_h_diag_device(eta_ ? hsym_.p : hdiag_.p, ndet_);
         bool use_initial_guess = int64_t(num_guess_states) * opt_.ndets_per_guess >= nspace_;
-        dl_->set_mask(masked() ? spacedet_.p : nullptr);
-        if (!(first_run || use_initial_guess) && eta_ && !dl_->symmetrize_basis()) use_initial_guess = true;
+        if (eta_) {
+            // the diagonal of H in the packed basis (both mirrors averaged) and the packed space mask
+            if (ms0_hd_.n < size_t(basis)) ms0_hd_.alloc(basis);
+            // (a padding coordinate past the packed ones, zero in every vector, keeps the basis
+            // length even for the paired loads of the Davidson-Liu kernels)
+            HIPCHK(hipMemset(ms0_hd_.p, 0, basis * sizeof(double)));
+            ms0_pack_device<double, double>(hdiag_.p, ms0_hd_.p, N, eta_, 1, 0.5, 1.0, 0);
+            dl_->add_h_diag_device(ms0_hd_.p, basis);
+            if (ms0_mask_.n < size_t(basis)) ms0_mask_.alloc(basis);
+            HIPCHK(hipMemset(ms0_mask_.p, 0, basis));
+            if (masked()) {
+                ms0_pack_mask_device(spacedet_.p, ms0_mask_.p, N, eta_, 0);
+            } else {
+                HIPCHK(hipMemset(ms0_mask_.p, 1, ms0_packed_size(N, eta_)));
+            }
+            dl_->set_mask(ms0_mask_.p);
+            if (ms0_wb_.n < size_t(ndet_)) ms0_wb_.alloc(ndet_);
+            if (ms0_ws_.n < size_t(ndet_)) ms0_ws_.alloc(ndet_);
+            HIPCHK(hipStreamSynchronize(0));
+        } else {
+            dl_->add_h_diag_device(hdiag_.p, ndet_);
+            dl_->set_mask(masked() ? spacedet_.p : nullptr);
+        }
         if (first_run || use_initial_guess) {
             dl_->reset();
             auto [g, bad] = initial_guess(num_guess_states);
             if (eta_)
-                for (auto& v : g) v = symmetrize_sparse(v, eta_);
+                for (auto& v : g) v = pack_sparse(v, eta_);
             dl_->add_guesses(g);
             bad_roots_ = std::move(bad);
         }
-        // the roots to project out, kept from the last guess and set for every solve, so that a warm
-        // restart uses them in the form of its own subspace
+        // the roots to project out, kept from the last guess and set for every solve in the
+        // coordinates of its basis
         if (eta_) {
-            // roots of the other spin parity vanish under the projection; the rest are renormalized
+            // roots of the other spin parity vanish in the packed coordinates; the rest are renormalized
             std::vector<SparseVec> keep;
             for (const auto& v : bad_roots_) {
-                SparseVec w = symmetrize_sparse(v, eta_);
+                SparseVec w = pack_sparse(v, eta_);
                 double n2 = 0.0;
                 for (const auto& e : w) n2 += e.second * e.second;
                 if (n2 < 1e-12) continue;
@@ -893,10 +912,19 @@ double Solver::compute_energy() {
     bool converged = false;
     auto lpr = std::make_shared<LPRun>();
     if (learned) {
-        dl_->set_preconditioner([this, lpr](double* r, const double* lam, int nr, int) {
+        dl_->set_preconditioner([this, lpr, N](double* r, const double* lam, int nr, int) {
             if (lpr->fell_back) return false;
             const auto& rn = dl_->residual_norms();
-            for (int k = 0; k < nr; ++k) lp_.apply(r + size_t(k) * ndet_, hdiag_.p, lam[k], rn[k], sa_, sb_, 0);
+            for (int k = 0; k < nr; ++k) {
+                if (eta_) {  // the network reads determinants: unpack, apply, project back
+                    double* rk = r + size_t(k) * dl_->size();
+                    ms0_unpack_device<double, double>(rk, ms0_wb_.p, N, eta_, 0);
+                    lp_.apply(ms0_wb_.p, hdiag_.p, lam[k], rn[k], sa_, sb_, 0);
+                    ms0_pack_device<double, double>(ms0_wb_.p, rk, N, eta_, eta_, 0.70710678118654752440, 1.0, 0);
+                } else {
+                    lp_.apply(r + size_t(k) * ndet_, hdiag_.p, lam[k], rn[k], sa_, sb_, 0);
+                }
+            }
             return true;
         });
     } else {
@@ -921,12 +949,15 @@ double Solver::compute_energy() {
                 csf_->to_csf<float, float>(s32_.p, s, 0);
             });
         } else if (eta_) {
-            dl_->add_sigma_builder32([this](const double* b, float* s) {
-                if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
-                k_f64_to_f32<<<unsigned((ndet_ + 255) / 256), 256>>>(b, c32_.p, ndet_);
-                HIPCHK(hipGetLastError());
-                eng_->apply_sym(c32_.p, s, eta_, 0);
-                mask_sigma(s);
+            if (c32_.n < size_t(ndet_)) c32_.alloc(ndet_);
+            if (s32_.n < size_t(ndet_)) s32_.alloc(ndet_);
+            dl_->add_sigma_builder32([this, N](const double* b, float* s) {
+                // C and S = e_core C / 2 in one pass; S += H_aa C + H_ab C / 2; the packed
+                // coordinates of sigma = S + eta S^T straight from S
+                ms0_unpack_device<double, float>(b, c32_.p, N, eta_, 0, s32_.p, 0.5 * di_.e_core);
+                eng_->apply_sym_half(c32_.p, s32_.p, eta_, 0);
+                mask_sigma(s32_.p);
+                ms0_pack_device<float, float>(s32_.p, s, N, eta_, eta_, 1.41421356237309504880, 2.0, 0);
             });
         } else {
             dl_->add_sigma_builder32([this](const double* b, float* s) {
@@ -967,9 +998,11 @@ double Solver::compute_energy() {
             csf_->to_csf<double, double>(csf_ws_.p, s, 0);
         });
     } else if (eta_) {
-        dl_->add_sigma_builder([this](const double* b, double* s) {
-            eng_->apply_sym(b, s, eta_, 0);
-            mask_sigma(s);
+        dl_->add_sigma_builder([this, N](const double* b, double* s) {
+            ms0_unpack_device<double, double>(b, ms0_wb_.p, N, eta_, 0, ms0_ws_.p, 0.5 * di_.e_core);
+            eng_->apply_sym_half(ms0_wb_.p, ms0_ws_.p, eta_, 0);
+            mask_sigma(ms0_ws_.p);
+            ms0_pack_device<double, double>(ms0_ws_.p, s, N, eta_, eta_, 1.41421356237309504880, 2.0, 0);
         });
     } else {
         dl_->add_sigma_builder([this](const double* b, double* s) {
@@ -1006,6 +1039,14 @@ double Solver::compute_energy() {
         HIPCHK(hipStreamSynchronize(0));
         csf_solved_ = true;
     }
+    if (eta_) {
+        // the eigenvectors as determinant vectors
+        if (ms0_ev_.n < size_t(nroot) * ndet_) ms0_ev_.alloc(size_t(nroot) * ndet_);
+        for (int r = 0; r < nroot; ++r)
+            ms0_unpack_device<double, double>(dl_->eigenvector_device(r), ms0_ev_.p + size_t(r) * ndet_, N, eta_, 0);
+        HIPCHK(hipStreamSynchronize(0));
+        ms0_solved_ = true;
+    }
     energies_.assign(nroot, 0.0);
     spin2_.assign(nroot, 0.0);
     for (int r = 0; r < nroot; ++r) {
@@ -1035,7 +1076,9 @@ int64_t Solver::lp_harvest(int64_t per_iter, int64_t max_groups, uint64_t seed, 
     const int nroot = opt_.nroot;
     dl_.reset(new DavidsonLiu(ndet_, nroot, opt_.collapse_per_root, opt_.subspace_per_root));
     dl_csf_ = false;
+    dl_eta_ = 0;
     csf_solved_ = false;
+    ms0_solved_ = false;
     dl_->set_print_level(opt_.print);
     compute_hdiag();
     dl_->add_h_diag_device(hdiag_.p, ndet_);
