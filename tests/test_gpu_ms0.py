@@ -226,8 +226,9 @@ def test_ms0_not_used_where_it_does_not_apply():
 
 
 def test_ms0_warm_restarts():
-    """Warm restarts into the symmetric subspace: a basis kept from a solve without the symmetry is
-    projected and orthonormalized again, and new (rotated) integrals restart from the kept basis."""
+    """Switching the symmetry on between two solves of one solver (a fresh guess: the kept basis
+    lives in determinant coordinates), then new (rotated) integrals, which restart from the kept
+    packed basis with the roots to project out re-packed."""
     import forte_amd
     n, k = 8, 4
     h, eri = forte_amd.seeded_integrals(n, 12)
