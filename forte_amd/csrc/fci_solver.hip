@@ -557,6 +557,7 @@ class Solver {
     bool solved_ = false;
     // Ms = 0 spin-flip symmetry of the current solve: C^T = eta C, eta = (-1)^S (0: not used)
     int eta_ = 0, dl_eta_ = 0;  // of the current solve / of the Davidson-Liu basis held in dl_
+    int csf_eta_ = 0;           // spin-adapted solve with the symmetric sigma
     bool ms0_solved_ = false;
     DBuf<double> ms0_hd_, ms0_wb_, ms0_ws_, ms0_ev_;  // packed diagonal, full work vectors, eigenvectors
     DBuf<uint8_t> ms0_mask_;
@@ -592,7 +593,13 @@ class Solver {
     // determinant basis (CSF vectors are spin-adapted by other means); eta = (-1)^S, the sign of
     // C^T = eta C for a state of spin S in this string order (C[Ia][Ib] alpha-major)
     int ms0_eta(bool csf) const {
-        if (!opt_.ms0_symmetry || csf || na_ != nb_ || opt_.multiplicity % 2 == 0) return 0;
+        if (csf) return 0;
+        return ms0_sign();
+    }
+    // the sign alone (also for spin-adapted solves, whose determinant vectors have it exactly in
+    // exact arithmetic: their sigma may take the symmetric form)
+    int ms0_sign() const {
+        if (!opt_.ms0_symmetry || na_ != nb_ || opt_.multiplicity % 2 == 0) return 0;
         if (hi_.oei_a != hi_.oei_b || hi_.tei_aa != hi_.tei_bb) return 0;
         if (sa_.nstr != sb_.nstr || ndet_ < 2) return 0;
         return ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
@@ -820,6 +827,10 @@ double Solver::compute_energy() {
     // subspace (upper triangle of C[Ia][Ib]), a basis in which its algebra is that of the
     // determinant solve restricted to the subspace, on half the data
     eta_ = ms0_eta(csf);
+    // a spin-adapted solve's determinant vectors (CSF expansions) carry the spin-flip sign up to
+    // rounding: its sigma takes the symmetric form, whose error on a rounding-level antisymmetric
+    // part is of the same level and is projected away by the transform back to CSFs
+    csf_eta_ = csf ? ms0_sign() : 0;
     const int64_t N = sa_.nstr;
     // the Davidson-Liu basis: CSFs of the multiplicity (fci_solver.cc:186-201), packed symmetric
     // coordinates or determinants
@@ -945,7 +956,10 @@ double Solver::compute_energy() {
             if (s32_.n < size_t(ndet_)) s32_.alloc(ndet_);
             dl_->add_sigma_builder32([this](const double* b, float* s) {
                 csf_->to_det<float>(b, c32_.p, ndet_, 0);
-                eng_->apply(c32_.p, s32_.p, 0);
+                if (csf_eta_)
+                    eng_->apply_sym(c32_.p, s32_.p, csf_eta_, 0);
+                else
+                    eng_->apply(c32_.p, s32_.p, 0);
                 csf_->to_csf<float, float>(s32_.p, s, 0);
             });
         } else if (eta_) {
@@ -994,7 +1008,10 @@ double Solver::compute_energy() {
         // sigma in the CSF basis through the determinant sigma (fci_solver.cc:246-254)
         dl_->add_sigma_builder([this](const double* b, double* s) {
             csf_->to_det<double>(b, csf_wb_.p, ndet_, 0);
-            eng_->apply(csf_wb_.p, csf_ws_.p, 0);
+            if (csf_eta_)
+                eng_->apply_sym(csf_wb_.p, csf_ws_.p, csf_eta_, 0);
+            else
+                eng_->apply(csf_wb_.p, csf_ws_.p, 0);
             csf_->to_csf<double, double>(csf_ws_.p, s, 0);
         });
     } else if (eta_) {
@@ -1297,7 +1314,7 @@ int fci_solve_stats(const fci_solver* s, int* it, int* ns, int* ns64, double* se
 
 int fci_solve_ms0(const fci_solver* s, int* eta) {
     if (!s || !eta) return FCI_ERR_ARG;
-    *eta = s->s->eta_;
+    *eta = s->s->eta_ ? s->s->eta_ : s->s->csf_eta_;
     return FCI_OK;
 }
 

@@ -133,8 +133,9 @@ int fci_energies(const fci_solver* s, double* energies, double* spin2);
  * converged flag, learned-preconditioner fallback flag. */
 int fci_solve_stats(const fci_solver* s, int* iterations, int* nsigma, int* nsigma_fp64,
                     double* seconds, int* converged, int* fell_back);
-/* No counterpart in the reference: the spin-flip sign eta (+1 / -1) the last solve kept its vectors
- * symmetric under (C^T = eta C), or 0 when it used the full sigma. */
+/* No counterpart in the reference: the spin-flip sign eta (+1 / -1) of the last solve (C^T = eta C:
+ * its determinant vectors were kept in that subspace, or, spin adapted, its sigma took the
+ * symmetric form), or 0 when it used the full sigma. */
 int fci_solve_ms0(const fci_solver* s, int* eta);
 /* FCISolver::evecs row `root` (fci_solver.cc:79-96), ndet doubles, host buffer */
 int fci_evec(const fci_solver* s, int root, double* out);

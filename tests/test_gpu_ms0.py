@@ -134,7 +134,7 @@ def test_solve_ms0_vs_oracle_and_full(oracle, n, k, nroot, mult, prec):
         say(f"  symmetry {on}: {g.energies()} {st}")
         assert st["converged"]
         eta = (-1) ** ((mult - 1) // 2)
-        assert st["ms0_symmetry"] == (eta if on else 0)  # spin-adapted solves do not use it
+        assert st["ms0_symmetry"] == (eta if on else 0)  # the spin-adapted reference: option off
         res[on] = (np.array(g.energies()), np.array(g.spin2()), [g.evec(r) for r in range(nroot)])
         if on:
             N = int(round(np.sqrt(g.ndet())))
@@ -215,11 +215,16 @@ def test_ms0_not_used_where_it_does_not_apply():
     g = gpu_solver(q, 1, 3)
     g.compute_energy()
     assert g.stats()["ms0_symmetry"] == 0
-    # spin adaptation works on CSF vectors
-    g = gpu_solver(p)
-    g.set_spin_adapt(True)
-    g.compute_energy()
-    assert g.stats()["ms0_symmetry"] == 0
+    # spin adaptation keeps CSF vectors; its sigma takes the symmetric form unless switched off
+    e = {}
+    for on in (True, False):
+        g = gpu_solver(p)
+        g.set_spin_adapt(True)
+        g.set_ms0_symmetry(on)
+        g.set_e_convergence(1e-11)
+        e[on] = g.compute_energy()
+        assert g.stats()["ms0_symmetry"] == (1 if on else 0)
+    assert abs(e[True] - e[False]) < 1e-9
     g = gpu_solver(p)
     with pytest.raises(RuntimeError, match="eta"):
         g.sigma_ms0_arrays(c, s, 2)
