@@ -2,6 +2,7 @@
 """CAS(16,16) FCI on one MI355X: sigma throughput and time to solution.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--solve 0|1] [--cpu 0|1] [--n 16]
+                    [--legs all|name,name] [--rconvs 1e-4,1e-6]
 
 A step is one sigma build H C over all 165,636,900 determinants of CAS(16,16) (Ms = 0, C1,
 seeded integrals, seed 3) with the mixed-precision kernels (FP32 C and sigma, BF16x3 MFMA
@@ -237,6 +238,10 @@ def main():
     ap.add_argument("--batch-threads", type=int, default=16)
     ap.add_argument("--configs", type=int, default=1,
                     help="also solve CAS(8,8) (1 root) and CAS(14,14) (3 roots, RDMs) in FP64")
+    ap.add_argument("--legs", default="all",
+                    help="time-to-solution legs to run, comma-separated names as in the output "
+                         "(e.g. mixed+diagonal,fp64+diagonal/full_sigma), or all")
+    ap.add_argument("--rconvs", default="1e-4,1e-6", help="residual thresholds of the time-to-solution legs")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
@@ -429,6 +434,10 @@ def main():
         def leg_name(prec, pc, sym):
             return f"{prec}+{pc}" + ("" if sym else "/full_sigma")
 
+        if args.legs != "all":
+            wanted = set(args.legs.split(","))
+            legs = [leg for leg in legs if leg_name(*leg) in wanted]
+
         def solve_leg(prec, pc, sym, r_conv):
             s = forte_amd.FCISolver(na, nb, 1, 1, ints)
             s.set_e_convergence(1e-9)
@@ -459,23 +468,25 @@ def main():
                         **{k: st[k] for k in ("iterations", "sigma_builds", "sigma_builds_fp64",
                                               "converged", "fell_back", "ms0_symmetry")})
 
+        def speedup(num, den):
+            return round(num["seconds"] / den["seconds"], 3) if num and den else None
+
         result["time_to_solution"] = {}
-        for r_conv in (1e-4, 1e-6):
+        for r_conv in (float(x) for x in args.rconvs.split(",")):
             tts = {leg_name(*leg): solve_leg(*leg, r_conv) for leg in legs}
-            ref = tts["fp64+diagonal"]
-            plain = tts["fp64+diagonal/full_sigma"]
-            best = tts.get("mixed+learned", tts["mixed+diagonal"])
+            ref = tts.get("fp64+diagonal")
+            plain = tts.get("fp64+diagonal/full_sigma")
+            mixed = tts.get("mixed+diagonal")
+            best = tts.get("mixed+learned", mixed)
             result["time_to_solution"][f"r_conv_{r_conv:g}"] = {
                 "runs": tts,
-                "speedup_mixed_learned_vs_fp64": round(ref["seconds"] / best["seconds"], 3),
-                "speedup_mixed_diagonal_vs_fp64": round(ref["seconds"] / tts["mixed+diagonal"]["seconds"], 3),
-                "speedup_mixed_olsen_vs_fp64": round(ref["seconds"] / tts["mixed+olsen"]["seconds"], 3),
-                "speedup_mixed_diagonal_vs_fp64_full_sigma": round(plain["seconds"] / tts["mixed+diagonal"]["seconds"],
-                                                                   3),
-                "speedup_ms0_symmetry_fp64": round(plain["seconds"] / ref["seconds"], 3),
-                "speedup_ms0_symmetry_mixed": round(tts["mixed+diagonal/full_sigma"]["seconds"]
-                                                    / tts["mixed+diagonal"]["seconds"], 3),
-                "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()},
+                "speedup_mixed_learned_vs_fp64": speedup(ref, best),
+                "speedup_mixed_diagonal_vs_fp64": speedup(ref, mixed),
+                "speedup_mixed_olsen_vs_fp64": speedup(ref, tts.get("mixed+olsen")),
+                "speedup_mixed_diagonal_vs_fp64_full_sigma": speedup(plain, mixed),
+                "speedup_ms0_symmetry_fp64": speedup(plain, ref),
+                "speedup_ms0_symmetry_mixed": speedup(tts.get("mixed+diagonal/full_sigma"), mixed),
+                "abs_dE_vs_fp64": {k: abs(v["energy"] - ref["energy"]) for k, v in tts.items()} if ref else None,
                 "thresholds": {"e_conv": 1e-9, "r_conv": r_conv, "maxiter": args.maxiter},
                 "learned_weights": os.path.relpath(forte_amd.learned.WEIGHTS, ROOT) if lp_w is not None else None}
 
@@ -529,7 +540,7 @@ def main():
             sps = result["cpu_baseline"]["seconds_per_sigma_extrapolated"]
             result["cpu_baseline"]["time_to_solution_extrapolated_seconds"] = {
                 k: round(sps * v["runs"]["fp64+diagonal/full_sigma"]["sigma_builds"], 1)
-                for k, v in result["time_to_solution"].items()}
+                for k, v in result["time_to_solution"].items() if "fp64+diagonal/full_sigma" in v["runs"]}
         print(f"# cpu baseline {result['cpu_baseline']}", file=sys.stderr, flush=True)
     else:
         result["cpu_baseline"] = None
