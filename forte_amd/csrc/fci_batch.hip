@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
+#include <cstring>
 #include <map>
 
 namespace fci {
@@ -757,13 +758,17 @@ struct BatchClass {
     SpinTables sa, sb;
     DBuf<uint32_t> binom;
     int gd[2] = {0, 0};
-    DBuf<double> g2a, g2b, vpk, hd, b, sg, r, tmp, tms, part, scratch;
+    DBuf<double> g2a, g2b, vpk, vpkh, hd, b, sg, r, tmp, tms, part, scratch;
+    // Ms = 0 spin-flip symmetry (C^T = eta C): the Davidson-Liu vectors are packed coordinates
+    // (nd of them) of full vectors of ndf = N^2 determinants, and sigma has no beta same-spin term
+    int eta = 0;
+    int64_t N = 0, ndf = 0;
     DBuf<BState> st;
     DBuf<int> pvo, done_dev;
     DBuf<int64_t> pvoff, pidx;
     DBuf<double> pval;
     // sigma tables
-    DBuf<const double*> c_tab, g2a_tab, g2b_tab, vpk_tab;
+    DBuf<const double*> c_tab, g2a_tab, g2b_tab, vpk_tab, vpkh_tab;
     DBuf<double*> s_tab, ct_tab, yt_tab, e_tab;
     hipStream_t stream = nullptr;
     int* done_host = nullptr;
@@ -840,8 +845,13 @@ struct BatchClass {
             t.e = e_tab.p;
             t.g2a = g2a_tab.p + i0;
             t.g2b = g2b_tab.p + i0;
-            t.vpk = vpk_tab.p + i0;
-            batch_sigma_f64(sa, sb, gd[0], gd[1], t, stream);
+            if (eta) {
+                t.vpk = vpkh_tab.p + i0;
+                batch_sigma_f64_ms0(sa, sb, gd[0], eta, t, stream);
+            } else {
+                t.vpk = vpk_tab.p + i0;
+                batch_sigma_f64(sa, sb, gd[0], gd[1], t, stream);
+            }
         }
     }
 
@@ -940,7 +950,19 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             HIPCHK(hipStreamSynchronize(C.stream));
             std::string why;
             if (!batch_sigma_supported(C.sa, C.sb, &why)) throw Error(FCI_ERR_ARG, why);
-            C.nd = C.sa.nstr * C.sb.nstr;
+            C.ndf = C.sa.nstr * C.sb.nstr;
+            C.N = C.sa.nstr;
+            {
+                const size_t m2 = size_t(C.n) * C.n, m4 = m2 * m2;
+                bool same = opts->ms0_symmetry && C.na == C.nb && opts->multiplicity % 2 == 1 && C.ndf >= 2;
+                for (int i : ids) {
+                    if (!same) break;
+                    same = std::memcmp(oei_a[i], oei_b[i], m2 * sizeof(double)) == 0 &&
+                           std::memcmp(tei_aa[i], tei_bb[i], m4 * sizeof(double)) == 0;
+                }
+                C.eta = same ? (((opts->multiplicity - 1) / 2) % 2 == 0 ? 1 : -1) : 0;
+            }
+            C.nd = C.eta ? ms0_packed_size(C.N, C.eta) : C.ndf;
             C.MS = int(std::min<int64_t>(MS0, C.nd));
             C.CS = int(std::min<int64_t>(std::max(opts->collapse_per_root * NR, NR), C.MS));
             C.ng = int(std::min<int64_t>(int64_t(opts->guess_per_root) * NR, C.nd));
@@ -957,6 +979,8 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             const size_t n2 = size_t(n) * n, n4 = n2 * n2;
             std::vector<double> g2a_h, g2b_h, vpk_h;
             C.hd.alloc(size_t(P) * nd);
+            DBuf<double> hfull;  // the full H diagonal of one problem (symmetric classes)
+            if (C.eta) hfull.alloc(C.ndf);
             DBuf<double> oa, ob, taa, tab, tbb;
             oa.alloc(n2); ob.alloc(n2); taa.alloc(n4); tab.alloc(n4); tbb.alloc(n4);
             std::vector<std::vector<SparseVec>> guesses(P), bad(P);
@@ -982,17 +1006,41 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 vpk_h.insert(vpk_h.end(), pk.vpk.begin(), pk.vpk.end());
                 oa.upload(oei_a[i], n2); ob.upload(oei_b[i], n2);
                 taa.upload(tei_aa[i], n4); tab.upload(tei_ab[i], n4); tbb.upload(tei_bb[i], n4);
-                double* hdq = C.hd.p + size_t(q) * nd;
+                double* hdq = C.eta ? hfull.p : C.hd.p + size_t(q) * nd;
                 hdiag_device(C.sa, C.sb, oa.p, ob.p, taa.p, tab.p, tbb.p, 0.0, hdq, C.stream);
                 HIPCHK(hipStreamSynchronize(C.stream));
-                auto gb = initial_guess_dets(hdq, nd, nd, C.sb.nstr, n, C.na, C.nb, hi, opts->multiplicity,
+                auto gb = initial_guess_dets(hdq, C.ndf, C.ndf, C.sb.nstr, n, C.na, C.nb, hi, opts->multiplicity,
                                              opts->ndets_per_guess, C.ng, C.stream);
                 guesses[q] = std::move(gb.first);
                 bad[q] = std::move(gb.second);
+                if (C.eta) {
+                    // the diagonal in the packed basis (a determinant and its mirror averaged), the
+                    // guesses and the roots to project out in packed coordinates (roots of the other
+                    // spin parity vanish there)
+                    ms0_pack_device<double, double>(hfull.p, C.hd.p + size_t(q) * nd, C.N, C.eta, 1, 0.5, 1.0,
+                                                    C.stream);
+                    HIPCHK(hipStreamSynchronize(C.stream));
+                    for (auto& g : guesses[q]) g = ms0_pack_sparse(g, C.N, C.eta);
+                    std::vector<SparseVec> keep;
+                    for (const auto& v : bad[q]) {
+                        SparseVec w = ms0_pack_sparse(v, C.N, C.eta);
+                        double n2 = 0.0;
+                        for (const auto& e : w) n2 += e.second * e.second;
+                        if (n2 < 1e-12) continue;
+                        for (auto& e : w) e.second /= std::sqrt(n2);
+                        keep.push_back(std::move(w));
+                    }
+                    bad[q] = std::move(keep);
+                }
             }
             C.g2a.alloc(g2a_h.size()); C.g2a.upload(g2a_h.data(), g2a_h.size());
             C.g2b.alloc(g2b_h.size()); C.g2b.upload(g2b_h.data(), g2b_h.size());
             C.vpk.alloc(vpk_h.size()); C.vpk.upload(vpk_h.data(), vpk_h.size());
+            if (C.eta) {  // halved packed integrals (exact) for the symmetric sigma
+                for (double& x : vpk_h) x *= 0.5;
+                C.vpkh.alloc(vpk_h.size());
+                C.vpkh.upload(vpk_h.data(), vpk_h.size());
+            }
             // vectors and state
             C.b.alloc(size_t(P) * C.MS * nd);
             C.sg.alloc(size_t(P) * C.MS * nd);
@@ -1041,12 +1089,13 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             C.pidx.alloc(pidx.size()); C.pidx.upload(pidx.data(), pidx.size());
             C.pval.alloc(pval.size()); C.pval.upload(pval.data(), pval.size());
             // sigma tables: per item integrals, per slot scratch
-            std::vector<const double*> ga(nitem), gb2(nitem), vp(nitem);
+            std::vector<const double*> ga(nitem), gb2(nitem), vp(nitem), vph(nitem);
             for (int it = 0; it < nitem; ++it) {
                 const int q = it / C.NSL;
                 ga[it] = C.g2a.p + size_t(q) * C.gd[0] * C.gd[0];
                 gb2[it] = C.g2b.p + size_t(q) * C.gd[1] * C.gd[1];
                 vp[it] = C.vpk.p + size_t(q) * size_t(n * (n + 1) / 2) * size_t(n * (n + 1) / 2);
+                vph[it] = C.eta ? C.vpkh.p + size_t(q) * size_t(n * (n + 1) / 2) * size_t(n * (n + 1) / 2) : nullptr;
             }
             std::vector<double*> ct(slots), yt(slots), et(slots);
             for (int s = 0; s < slots; ++s) {
@@ -1058,6 +1107,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             C.g2a_tab.alloc(nitem); C.g2a_tab.upload(ga.data(), nitem);
             C.g2b_tab.alloc(nitem); C.g2b_tab.upload(gb2.data(), nitem);
             C.vpk_tab.alloc(nitem); C.vpk_tab.upload(vp.data(), nitem);
+            C.vpkh_tab.alloc(nitem); C.vpkh_tab.upload(vph.data(), nitem);
             C.ct_tab.alloc(slots); C.ct_tab.upload(ct.data(), slots);
             C.yt_tab.alloc(slots); C.yt_tab.upload(yt.data(), slots);
             C.e_tab.alloc(slots); C.e_tab.upload(et.data(), slots);
@@ -1191,9 +1241,20 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 const BState& s = hs[q];
                 if (s.status < 0) throw Error(s.status, "Davidson-Liu numerical failure in a batched problem");
                 for (int k = 0; k < NR; ++k) energies[size_t(i) * NR + k] = s.lam[k] + C.ecore[q];
-                if (evecs && evecs[i])
-                    HIPCHK(hipMemcpy(evecs[i], C.b.p + size_t(q) * C.MS * C.nd, sizeof(double) * NR * C.nd,
-                                     hipMemcpyDeviceToHost));
+                if (evecs && evecs[i]) {
+                    const double* bq = C.b.p + size_t(q) * C.MS * C.nd;
+                    if (C.eta) {  // determinant vectors of the packed ones
+                        DBuf<double> full(C.ndf);
+                        for (int k = 0; k < NR; ++k) {
+                            ms0_unpack_device<double, double>(bq + size_t(k) * C.nd, full.p, C.N, C.eta, C.stream);
+                            HIPCHK(hipStreamSynchronize(C.stream));
+                            HIPCHK(hipMemcpy(evecs[i] + size_t(k) * C.ndf, full.p, sizeof(double) * C.ndf,
+                                             hipMemcpyDeviceToHost));
+                        }
+                    } else {
+                        HIPCHK(hipMemcpy(evecs[i], bq, sizeof(double) * NR * C.nd, hipMemcpyDeviceToHost));
+                    }
+                }
                 if (iterations) iterations[i] = s.iters;
                 if (converged) converged[i] = s.conv;
                 nconv += s.conv;

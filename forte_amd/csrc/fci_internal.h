@@ -235,6 +235,11 @@ size_t batch_sigma_scratch_elems(const SpinTables& a, const SpinTables& b, size_
 void batch_sigma_prepare(const SpinTables& a, const SpinTables& b);
 void batch_sigma_f64(const SpinTables& a, const SpinTables& b, int gd_a, int gd_b, const BatchSigmaTabs& t,
                      hipStream_t st);
+// the same for Ms = 0 items with C^T = eta C held in packed coordinates (ms0_pack_device): c and s
+// are packed vectors, ct and yt full-size scratch (C and S), the vpk tables hold halved integrals;
+// no beta same-spin term is built
+void batch_sigma_f64_ms0(const SpinTables& a, const SpinTables& b, int gd_a, int eta, const BatchSigmaTabs& t,
+                         hipStream_t st);
 
 // per-string diagonal of the same-spin clique operator: out[I] = sum over the d-subsets P of
 // occ(I) of g[P][P] (d = 2: pairs a > b, index a(a-1)/2 + b; d = 1: orbitals)
@@ -256,6 +261,10 @@ void sym_add_device(T* S, int64_t N, int eta, bool avg, hipStream_t st);
 // orthonormal basis of the eta-symmetric subspace, so dot products and norms are preserved.
 int64_t ms0_packed_size(int64_t N, int eta);
 int64_t ms0_packed_index(int64_t i, int64_t j, int64_t N, int eta);
+// a sparse determinant vector (index Ia * N + Ib) in packed coordinates: pair (Ia, Ib), Ia < Ib ->
+// (c + eta c') / sqrt2, diagonal (eta = +1) -> c; later duplicates of an index overwrite
+std::vector<std::pair<int64_t, double>> ms0_pack_sparse(const std::vector<std::pair<int64_t, double>>& v,
+                                                        int64_t N, int eta);
 // packed(i, j) = s_off (F_ij + mirror F_ji) for i < j, s_diag F_ii on the diagonal (vectors:
 // mirror = eta, s_off = 1/sqrt2, s_diag = 1; a symmetric array such as the H diagonal: mirror = +1,
 // s_off = 1/2, s_diag = 1; the packed sigma of the first half S of the symmetric form, sigma =

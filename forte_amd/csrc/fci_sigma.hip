@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 
 namespace fci {
 
@@ -195,14 +196,41 @@ int64_t ms0_packed_index(int64_t i, int64_t j, int64_t N, int eta) {
     const int d = eta == 1 ? 0 : 1;
     return ms0_row_off(i, N, d) + (j - i - d);
 }
+std::vector<std::pair<int64_t, double>> ms0_pack_sparse(const std::vector<std::pair<int64_t, double>>& v,
+                                                        int64_t N, int eta) {
+    std::map<int64_t, double> m;
+    for (const auto& [i, c] : v) m[i] = c;  // later duplicates overwrite, as set_vector
+    std::map<int64_t, double> out;
+    for (const auto& [k, c] : m) {
+        const int64_t i = k / N, j = k % N;
+        if (i == j) {
+            if (eta == 1) out[ms0_packed_index(i, i, N, eta)] = c;
+            continue;
+        }
+        // C_ij enters the pair (min, max) directly (i < j) or through its mirror's sign (i > j)
+        const int64_t lo = std::min(i, j), hi = std::max(i, j);
+        out[ms0_packed_index(lo, hi, N, eta)] += (i < j ? c : eta * c) * 0.70710678118654752440;
+    }
+    return std::vector<std::pair<int64_t, double>>(out.begin(), out.end());
+}
 
 // packed(i, j) = s_off (F_ij + mirror F_ji) for i < j, s_diag F_ii on the diagonal (d = 0); tile
 // pair (I, J), I <= J, per block, both tiles of F read into LDS
+// Batched launches (many problems of one size) pass per-item pointer tables indexed by blockIdx.z;
+// an item whose `act` entry is null is idle. Null tables: the plain pointers are used.
 template <class TI, class TO>
 __global__ __launch_bounds__(256) void k_ms0_pack(const TI* __restrict__ F, TO* __restrict__ P, int64_t N, int d,
-                                                  int mirror, double s_off, double s_diag) {
+                                                  int mirror, double s_off, double s_diag,
+                                                  const void* const* __restrict__ act = nullptr,
+                                                  const TI* const* __restrict__ F_t = nullptr,
+                                                  TO* const* __restrict__ P_t = nullptr) {
     const int I = blockIdx.y, J = blockIdx.x;
     if (J < I) return;
+    if (act) {
+        if (!act[blockIdx.z]) return;
+        F = F_t[blockIdx.z];
+        P = P_t[blockIdx.z];
+    }
     __shared__ TI ta[64][65], tb[64][65];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int64_t r0 = int64_t(I) * 64, c0 = int64_t(J) * 64;
@@ -233,9 +261,17 @@ __global__ __launch_bounds__(256) void k_ms0_pack(const TI* __restrict__ F, TO* 
 // mirrored pair is converted once, so F is exactly eta-symmetric in its own precision
 template <class TI, class TO>
 __global__ __launch_bounds__(256) void k_ms0_unpack(const TI* __restrict__ P, TO* __restrict__ F, int64_t N, int d,
-                                                    int eta, TO* __restrict__ S, TO s_scale) {
+                                                    int eta, TO* __restrict__ S, TO s_scale,
+                                                    const void* const* __restrict__ act = nullptr,
+                                                    const TI* const* __restrict__ P_t = nullptr,
+                                                    TO* const* __restrict__ F_t = nullptr) {
     const int I = blockIdx.y, J = blockIdx.x;
     if (J < I) return;
+    if (act) {
+        if (!act[blockIdx.z]) return;
+        P = P_t[blockIdx.z];
+        F = F_t[blockIdx.z];
+    }
     __shared__ TO tp[64][65];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int64_t r0 = int64_t(I) * 64, c0 = int64_t(J) * 64;
@@ -271,9 +307,10 @@ __global__ __launch_bounds__(256) void k_ms0_unpack(const TI* __restrict__ P, TO
 }
 
 __global__ void k_ms0_pack_mask(const uint8_t* __restrict__ F, uint8_t* __restrict__ P, int64_t N, int d) {
-    const int64_t i = blockIdx.y, c = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (c >= N || c < i + d) return;
-    P[ms0_row_off(i, N, d) + (c - i - d)] = F[i * N + c];
+    const int64_t c = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (c >= N) return;
+    for (int64_t i = blockIdx.y; i < N; i += gridDim.y)  // rows strided over the grid's y extent
+        if (c >= i + d) P[ms0_row_off(i, N, d) + (c - i - d)] = F[i * N + c];
 }
 
 template <class TI, class TO>
@@ -290,7 +327,8 @@ void ms0_unpack_device(const TI* packed, TO* full, int64_t N, int eta, hipStream
     HIPCHK(hipGetLastError());
 }
 void ms0_pack_mask_device(const uint8_t* full, uint8_t* packed, int64_t N, int eta, hipStream_t st) {
-    k_ms0_pack_mask<<<dim3(unsigned((N + 255) / 256), unsigned(N)), 256, 0, st>>>(full, packed, N, eta == 1 ? 0 : 1);
+    const unsigned gy = unsigned(std::min<int64_t>(N, 65535));
+    k_ms0_pack_mask<<<dim3(unsigned((N + 255) / 256), gy), 256, 0, st>>>(full, packed, N, eta == 1 ? 0 : 1);
     HIPCHK(hipGetLastError());
 }
 template void ms0_pack_device<double, double>(const double*, double*, int64_t, int, int, double, double, hipStream_t);
@@ -2317,6 +2355,8 @@ void batch_sigma_prepare(const SpinTables& a, const SpinTables& b) {
     throw Error(-1, "batched alpha-beta kernel: no instance for this shape");
 }
 
+static void batch_ab_f64(const SpinTables& a, const SpinTables& b, const BatchSigmaTabs& t, hipStream_t st);
+
 void batch_sigma_f64(const SpinTables& a, const SpinTables& b, int gd_a, int gd_b, const BatchSigmaTabs& t,
                      hipStream_t st) {
     const int64_t Na = a.nstr, Nb = b.nstr;
@@ -2331,7 +2371,30 @@ void batch_sigma_f64(const SpinTables& a, const SpinTables& b, int gd_a, int gd_
     HIPCHK(hipGetLastError());
     // alpha same-spin on C, accumulated into sigma
     batch_clique_f64(a, gd_a, Nb, t.act, t.c, t.e, t.g2a, t.s, true, t.nitem, st);
-    // alpha-beta
+    batch_ab_f64(a, b, t, st);
+}
+
+void batch_sigma_f64_ms0(const SpinTables& a, const SpinTables& b, int gd_a, int eta, const BatchSigmaTabs& t,
+                         hipStream_t st) {
+    const int64_t N = a.nstr;
+    const unsigned ni = unsigned(t.nitem), tg = unsigned((N + 63) / 64);
+    // full C of every item from its packed coordinates
+    k_ms0_unpack<double, double><<<dim3(tg, tg, ni), 256, 0, st>>>(nullptr, nullptr, N, eta == 1 ? 0 : 1, eta,
+                                                                    nullptr, 0.0, t.act, t.c, t.ct);
+    HIPCHK(hipGetLastError());
+    // S = H_aa C (written) + 1/2 H_ab C (the item tables hold halved integrals)
+    BatchSigmaTabs u = t;
+    u.c = t.ct;
+    u.s = t.yt;
+    batch_clique_f64(a, gd_a, N, u.act, u.c, u.e, u.g2a, u.s, false, u.nitem, st);
+    batch_ab_f64(a, b, u, st);
+    // packed coordinates of sigma = S + eta S^T
+    k_ms0_pack<double, double><<<dim3(tg, tg, ni), 256, 0, st>>>(nullptr, nullptr, N, eta == 1 ? 0 : 1, eta,
+                                                                  1.41421356237309504880, 2.0, t.act, u.s, t.s);
+    HIPCHK(hipGetLastError());
+}
+
+static void batch_ab_f64(const SpinTables& a, const SpinTables& b, const BatchSigmaTabs& t, hipStream_t st) {
     const int ks = std::max(1, (a.ns + 15) / 16) * 4;
     const int tpw = ((b.np + 15) / 16 + 3) / 4;
     const bool split = tpw > 1 && (b.np + 15) / 16 == 4 * (tpw - 1) + 1;

@@ -604,25 +604,8 @@ class Solver {
         if (sa_.nstr != sb_.nstr || ndet_ < 2) return 0;
         return ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
     }
-    // a sparse determinant vector in the packed coordinates of the eta-symmetric subspace: pair
-    // (Ia, Ib), Ia < Ib -> (c + eta c') / sqrt2, diagonal (eta = +1) -> c
-    SparseVec pack_sparse(const SparseVec& v, int eta) const {
-        const int64_t N = sa_.nstr;
-        std::map<int64_t, double> m;
-        for (const auto& [i, c] : v) m[i] = c;  // later duplicates overwrite, as set_vector
-        std::map<int64_t, double> out;
-        for (const auto& [k, c] : m) {
-            const int64_t i = k / N, j = k % N;
-            if (i == j) {
-                if (eta == 1) out[ms0_packed_index(i, i, N, eta)] = c;
-                continue;
-            }
-            const int64_t lo = std::min(i, j), hi = std::max(i, j);
-            // c enters as C_lo,hi (i < j) or as eta C_hi,lo's mirror (i > j)
-            out[ms0_packed_index(lo, hi, N, eta)] += (i < j ? c : eta * c) * 0.70710678118654752440;
-        }
-        return SparseVec(out.begin(), out.end());
-    }
+    // a sparse determinant vector in the packed coordinates of the eta-symmetric subspace
+    SparseVec pack_sparse(const SparseVec& v, int eta) const { return ms0_pack_sparse(v, sa_.nstr, eta); }
     void sigma_mixed(const double* C, double* S);
     void sigma_mixed32(const double* C, float* S);
     void compute_hdiag() {

@@ -104,3 +104,46 @@ def test_batch_rejects_unsupported_options():
         forte_amd._forte_amd.batch_solve([ints], [3], [3], 2, 1, {"FCI_PRECISION": "mixed"}, False)
     with pytest.raises(RuntimeError):
         forte_amd._forte_amd.batch_solve([ints], [3], [3], 5, 1, {}, False)
+
+
+@pytest.mark.parametrize("mult", [1, 3])
+def test_batch_spin_flip_symmetry(mult):
+    """Symmetric classes (packed Davidson-Liu vectors, sigma without the beta same-spin term) against
+    spin-adapted single solves, singlets and triplets, and against the same batch with the symmetry
+    off where that converges (its n = 10 triplet does not in 400 iterations: the determinant solve
+    without the symmetry lets singlet components back in); the eigenvectors come back as exactly
+    (-1)^S-symmetric determinant vectors."""
+    import forte_amd
+    from forte_amd.batch import solve_many
+    sel = [(6, 3), (6, 5), (8, 2), (8, 7), (10, 1)]
+    ints = []
+    for n, seed in sel:
+        h, eri = forte_amd.seeded_integrals(n, seed)
+        ints.append(forte_amd.make_active_space_integrals(h, eri))
+    na = [n // 2 for n, _ in sel]
+    out = {}
+    for on in (True, False):
+        opts = {"E_CONVERGENCE": 1e-11, "R_CONVERGENCE": 1e-7, "DL_MAXITER": 400, "FCI_MS0_SYMMETRY": on}
+        out[on] = solve_many(ints, na, na, nroot=2, multiplicity=mult, options=opts, want_evecs=True)
+    eta = 1 if mult == 1 else -1
+    a, b = out[True], out[False]
+    for i, (n, seed) in enumerate(sel):
+        ref = forte_amd.FCISolver(n // 2, n // 2, mult, 2, ints[i])
+        ref.set_spin_adapt(True)
+        ref.set_e_convergence(1e-11)
+        ref.set_r_convergence(1e-7)
+        ref.set_maxiter(400)
+        ref.compute_energy()
+        say(f"  n={n} seed={seed} mult={mult}: symmetric {a['energies'][i]} ({a['iterations'][i]} it), "
+            f"full {b['energies'][i]} ({b['iterations'][i]} it, converged {b['converged'][i]}), "
+            f"CSF solve {ref.energies()}")
+        assert a["converged"][i]
+        assert np.abs(np.array(a["energies"][i]) - np.array(ref.energies())).max() < 1e-9
+        if b["converged"][i]:
+            assert np.abs(np.array(a["energies"][i]) - np.array(b["energies"][i])).max() < 1e-9
+        V = np.asarray(a["evecs"][i])
+        N = int(round(np.sqrt(V.shape[1])))
+        for v in V:
+            M = v.reshape(N, N)
+            assert np.array_equal(M.T, eta * M)
+        assert np.abs(V @ V.T - np.eye(2)).max() < 1e-10
