@@ -1775,22 +1775,63 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
     }
     const double* Crow = C + Ja * Nb;
     const int ntile = npp >> 4;
-    for (int64_t c0 = int64_t(wc) * 32; c0 < Nb; c0 += 32 * WC) {
+    // the gathered rows, targets and C[Ja] columns of one 32-column tile; the next tile's are
+    // loaded after the current tile's products, so they are in flight during its scatter
+    auto load = [&](int64_t c0, double2 (&xv)[KS], uint4 (&tg)[TPW], double2& cd) {
         const int64_t Ib = c0 + 2 * cl;
-        const bool ok = Ib < Nb;
-        const int64_t Ibc = ok ? Ib : 0;  // columns past the end: computed, never scattered
-        uint4 tg[TPW];
+        const int64_t Ibc = Ib < Nb ? Ib : 0;  // columns past the end: computed, never scattered
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
             const int mtile = SPLIT && j == TPW - 1 ? mts : wm + WM * j;  // the table row c0 / 32 exists
             const int mtc = mtile < nmt ? mtile : 0;  // for all lanes; tiles past nmt are skipped below
             tg[j] = *reinterpret_cast<const uint4*>(outT + (((c0 >> 5) * ntile + mtc) * 64 + lane) * 8);
         }
-        double2 xv[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
             xv[ks] = *reinterpret_cast<const double2*>(C + gather_row(ent[ks * 4 + g]) * Nb + Ibc);
-        const double2 cd = *reinterpret_cast<const double2*>(Crow + Ibc);
+        cd = *reinterpret_cast<const double2*>(Crow + Ibc);
+    };
+    double2 xv[KS];
+    uint4 tg[TPW];
+    double2 cd;
+    int64_t c0 = int64_t(wc) * 32;
+    if (c0 < Nb) load(c0, xv, tg, cd);
+    for (; c0 < Nb; c0 += 32 * WC) {
+        const bool ok = c0 + 2 * cl < Nb;
+        d4 acc0[TPW], acc1[TPW];
+#pragma unroll
+        for (int j = 0; j < TF; ++j) {
+            acc0[j] = d4{0.0, 0.0, 0.0, 0.0};
+            acc1[j] = d4{0.0, 0.0, 0.0, 0.0};
+            if (wm + WM * j >= nmt) continue;  // wave-uniform
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                acc0[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].x, acc0[j], 0, 0, 0);
+                acc1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].y, acc1[j], 0, 0, 0);
+            }
+        }
+        if constexpr (SPLIT) {
+            acc0[TPW - 1] = d4{0.0, 0.0, 0.0, 0.0};
+            acc1[TPW - 1] = d4{0.0, 0.0, 0.0, 0.0};
+            if (mts < nmt) {  // this wave's quarter of the shared tile's k-steps (compile-time indices)
+#pragma unroll
+                for (int w = 0; w < WM; ++w)
+                    if (wm == w) {
+#pragma unroll
+                        for (int q = 0; q < KQ; ++q) {
+                            acc0[TPW - 1] =
+                                __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].x, acc0[TPW - 1], 0, 0, 0);
+                            acc1[TPW - 1] =
+                                __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].y, acc1[TPW - 1], 0, 0, 0);
+                        }
+                    }
+            }
+        }
+        uint4 tgc[TPW];
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) tgc[j] = tg[j];
+        const double2 cdc = cd;
+        if (c0 + 32 * WC < Nb) load(c0 + 32 * WC, xv, tg, cd);  // wave-uniform
         auto scatter = [&](const uint4& t4, const d4& a0, const d4& a1, const double (&vdj)[4]) {
             const uint32_t tw[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
@@ -1799,36 +1840,17 @@ __global__ __launch_bounds__(512, 2) void k_ab2_f64(const double* __restrict__ C
                 for (int v = 0; v < 4; ++v) {
                     const uint32_t tt = (tw[p * 2 + v / 2] >> (16 * (v & 1))) & 0xFFFFu;
                     if (!ok || (tt & 0x7FFFu) == kInv16) continue;
-                    const double a = (p ? a1[v] : a0[v]) + vdj[v] * (p ? cd.y : cd.x);
+                    const double a = (p ? a1[v] : a0[v]) + vdj[v] * (p ? cdc.y : cdc.x);
                     atomicAdd(&srow64[tt & 0x7FFFu], (tt >> 15) ? -a : a);
                 }
         };
 #pragma unroll
         for (int j = 0; j < TF; ++j) {
             if (wm + WM * j >= nmt) continue;  // wave-uniform
-            d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].x, a0, 0, 0, 0);
-                a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(va[j][ks], xv[ks].y, a1, 0, 0, 0);
-            }
-            scatter(tg[j], a0, a1, vd[j]);
+            scatter(tgc[j], acc0[j], acc1[j], vd[j]);
         }
-        if constexpr (SPLIT) {
-            if (mts < nmt) {  // this wave's quarter of the shared tile's k-steps (compile-time indices)
-                d4 a0 = d4{0.0, 0.0, 0.0, 0.0}, a1 = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int w = 0; w < WM; ++w)
-                    if (wm == w) {
-#pragma unroll
-                        for (int q = 0; q < KQ; ++q) {
-                            a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].x, a0, 0, 0, 0);
-                            a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(vs[q], xv[w * KQ + q].y, a1, 0, 0, 0);
-                        }
-                    }
-                scatter(tg[TPW - 1], a0, a1, vds);
-            }
-        }
+        if constexpr (SPLIT)
+            if (mts < nmt) scatter(tgc[TPW - 1], acc0[TPW - 1], acc1[TPW - 1], vds);
     }
     __syncthreads();
     double* Srow = S + Ja * Nb;
