@@ -961,6 +961,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                            std::memcmp(tei_aa[i], tei_bb[i], m4 * sizeof(double)) == 0;
                 }
                 C.eta = same ? (((opts->multiplicity - 1) / 2) % 2 == 0 ? 1 : -1) : 0;
+                if (C.eta && ms0_packed_size(C.N, C.eta) < 4 * int64_t(NR)) C.eta = 0;  // tiny spaces
             }
             C.nd = C.eta ? ms0_packed_size(C.N, C.eta) : C.ndf;
             C.MS = int(std::min<int64_t>(MS0, C.nd));

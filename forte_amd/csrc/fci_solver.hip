@@ -602,7 +602,10 @@ class Solver {
         if (!opt_.ms0_symmetry || na_ != nb_ || opt_.multiplicity % 2 == 0) return 0;
         if (hi_.oei_a != hi_.oei_b || hi_.tei_aa != hi_.tei_bb) return 0;
         if (sa_.nstr != sb_.nstr || ndet_ < 2) return 0;
-        return ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
+        const int eta = ((opt_.multiplicity - 1) / 2) % 2 == 0 ? 1 : -1;
+        // tiny spaces keep the full sigma: the subspace must hold the requested roots comfortably
+        if (ms0_packed_size(sa_.nstr, eta) < 4 * int64_t(opt_.nroot)) return 0;
+        return eta;
     }
     // a sparse determinant vector in the packed coordinates of the eta-symmetric subspace
     SparseVec pack_sparse(const SparseVec& v, int eta) const { return ms0_pack_sparse(v, sa_.nstr, eta); }
