@@ -36,6 +36,28 @@ __device__ __forceinline__ uint32_t csf_rank(uint32_t s, int n, const uint32_t* 
     return r;
 }
 
+// For n <= 16 the rank splits over the two bytes of the string: the high byte's bits have ordinals
+// counted from the top, the low byte's start after popc(high byte), so
+//   rank(s) = H[s >> 8] + L[popc(s >> 8)][s & 255]
+// with H and L (9 x 256) summing the binomials of the bits of one byte value. Both are 16-bit (every
+// rank at n <= 16 is below C(16, 8)) and staged in LDS by every kernel: two LDS reads per string
+// instead of a chain of one dependent table load per electron.
+constexpr int kRankTab = 256 * 10;
+__device__ __forceinline__ void csf_rank_stage(uint16_t* rl, const uint16_t* __restrict__ g) {
+    if (g)
+        for (int t = threadIdx.x; t < kRankTab; t += blockDim.x) rl[t] = g[t];
+}
+__device__ __forceinline__ uint32_t csf_rank2(uint32_t s, const uint16_t* rl, bool fast, int n,
+                                              const uint32_t* __restrict__ B) {
+    if (fast) {
+        const uint32_t h = s >> 8;
+        return uint32_t(rl[h]) + uint32_t(rl[256 + (__popc(h) << 8) + (s & 255u)]);
+    }
+    return csf_rank(s, n, B);
+}
+// address of the determinant (a, b) in C[Ia * nbs + Ib]
+#define CSF_ADDR(a, b) (int64_t(csf_rank2((a), rl, fast, n, B)) * nbs + csf_rank2((b), rl, fast, n, B))
+
 // alpha / beta strings of the configuration's determinant with open-shell spin pattern m
 // (bit k of m set: the k-th lowest open shell holds a beta electron)
 __device__ __forceinline__ void csf_strings(uint32_t docc, uint32_t socc, uint32_t m, uint32_t& a, uint32_t& b) {
@@ -53,13 +75,56 @@ __device__ __forceinline__ void csf_strings(uint32_t docc, uint32_t socc, uint32
 
 constexpr uint32_t kConfNeg = 0x80000000u;  // sign bit of the configuration factor, kept in docc
 
+// Stage the determinants of configurations [0, nc) (docc, socc point at the block's first one) into
+// xs[cc * ndp + j], signed by the configuration factor; rows cc in [nc, npad) are zeroed. Each
+// thread computes kStageU addresses before it loads, so that many random reads are in flight per
+// thread (blocks that hold large LDS tiles run few waves per CU).
+constexpr int kStageU = 8;
+template <class TI>
+__device__ __forceinline__ void csf_stage_dets(const TI* __restrict__ det, double* xs, int nc, int npad, int ndp,
+                                               const uint32_t* __restrict__ docc, const uint32_t* __restrict__ socc,
+                                               const uint32_t* __restrict__ occ, const uint16_t* rl, bool fast, int n,
+                                               int64_t nbs, const uint32_t* __restrict__ B) {
+    const int valid = nc * ndp, total = npad * ndp;
+    for (int t0 = threadIdx.x; t0 < total; t0 += kStageU * blockDim.x) {
+        int64_t addr[kStageU];
+        bool neg[kStageU];
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+            const int t = t0 + u * int(blockDim.x);
+            addr[u] = -1;
+            neg[u] = false;
+            if (t < valid) {
+                const int cc = t / ndp, j = t - cc * ndp;
+                const uint32_t dw = docc[cc];
+                uint32_t a, b;
+                csf_strings(dw & ~kConfNeg, socc[cc], occ[j], a, b);
+                addr[u] = CSF_ADDR(a, b);
+                neg[u] = (dw & kConfNeg) != 0;
+            }
+        }
+        double v[kStageU];
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) v[u] = addr[u] >= 0 ? double(det[addr[u]]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+            const int t = t0 + u * int(blockDim.x);
+            if (t < total) xs[t] = neg[u] ? -v[u] : v[u];
+        }
+    }
+}
+
 // det -> csf: thread per (configuration, CSF row); csf[off + i] = s_c sum_j T[i][j] det[addr(c, j)]
 template <class TI, class TO>
 __global__ void k_csf_gather(const TI* __restrict__ det, TO* __restrict__ csf, const uint32_t* __restrict__ docc,
                              const uint32_t* __restrict__ socc, const int64_t* __restrict__ off, int64_t nconf,
                              int ncsf, const int* __restrict__ rp, const uint32_t* __restrict__ rj,
                              const double* __restrict__ rv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                             const uint32_t* __restrict__ B) {
+                             const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= nconf * ncsf) return;
     const int64_t c = t / ncsf;
@@ -70,7 +135,7 @@ __global__ void k_csf_gather(const TI* __restrict__ det, TO* __restrict__ csf, c
     for (int e = rp[i]; e < rp[i + 1]; ++e) {
         uint32_t a, b;
         csf_strings(d, so, occ[rj[e]], a, b);
-        const int64_t addr = int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B);
+        const int64_t addr = CSF_ADDR(a, b);
         acc += rv[e] * double(det[addr]);
     }
     csf[off[c] + i] = TO((dw & kConfNeg) ? -acc : acc);
@@ -96,19 +161,17 @@ __global__ __launch_bounds__(256) void k_csf_gather_lds(const TI* __restrict__ d
                                                         int rpb, int mode, int ncsf, int ndp,
                                                         const int* __restrict__ rp, const uint32_t* __restrict__ rj,
                                                         const double* __restrict__ rv, const uint32_t* __restrict__ occ,
-                                                        int n, int64_t nbs, const uint32_t* __restrict__ B) {
+                                                        int n, int64_t nbs, const uint32_t* __restrict__ B,
+                                                        const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     extern __shared__ double xs[];  // [cpb][ndp]
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
     const int r0 = blockIdx.y * rpb, r1 = min(ncsf, r0 + rpb);
-    for (int t = threadIdx.x; t < nc * ndp; t += blockDim.x) {
-        const int cc = t / ndp, j = t - cc * ndp;
-        const uint32_t dw = docc[c0 + cc];
-        uint32_t a, b;
-        csf_strings(dw & ~kConfNeg, socc[c0 + cc], occ[j], a, b);
-        const double v = double(det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)]);
-        xs[t] = (dw & kConfNeg) ? -v : v;
-    }
+    csf_stage_dets(det, xs, nc, nc, ndp, docc + c0, socc + c0, occ, rl, fast, n, nbs, B);
     __syncthreads();
     const int ng = (nc + kCsfCT - 1) / kCsfCT;
     if (mode == 2) {
@@ -121,12 +184,21 @@ __global__ __launch_bounds__(256) void k_csf_gather_lds(const TI* __restrict__ d
                 double acc[kCsfCT];
 #pragma unroll
                 for (int k = 0; k < kCsfCT; ++k) acc[k] = 0.0;
-                for (int e = e0 + lane; e < e1; e += 64) {
-                    const double v = rv[e];
-                    const uint32_t jj = rj[e];
+                // four chunks of 64 nonzeros per pass: their loads are in flight together
+                for (int e = e0 + lane; e < e1; e += 4 * 64) {
+                    double v[4];
+                    uint32_t jj[4];
 #pragma unroll
-                    for (int k = 0; k < kCsfCT; ++k)
-                        if (k < kk) acc[k] += v * x0[k * ndp + jj];
+                    for (int u = 0; u < 4; ++u) {
+                        const int eu = e + 64 * u;
+                        v[u] = eu < e1 ? rv[eu] : 0.0;
+                        jj[u] = eu < e1 ? rj[eu] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int k = 0; k < kCsfCT; ++k)
+                            if (k < kk) acc[k] += v[u] * x0[k * ndp + jj[u]];
                 }
 #pragma unroll
                 for (int k = 0; k < kCsfCT; ++k) {
@@ -164,6 +236,29 @@ __global__ __launch_bounds__(256) void k_csf_gather_lds(const TI* __restrict__ d
     }
 }
 
+// LDS row stride of a staged dense T_N: odd, so rows read by consecutive lanes fall in distinct banks
+__host__ __device__ __forceinline__ int csf_tstride(int ndp) { return ndp | 1; }
+// dense T_N [ncsf][ndp] from global memory into LDS rows of stride ts, four loads in flight per thread
+__device__ __forceinline__ void csf_stage_table(double* T, const double* __restrict__ td, int ncsf, int ndp, int ts) {
+    const int total = ncsf * ndp;
+    for (int t0 = threadIdx.x; t0 < total; t0 += 4 * blockDim.x) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * int(blockDim.x);
+            v[u] = t < total ? td[t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * int(blockDim.x);
+            if (t < total) {
+                const int i = t / ndp;
+                T[i * ts + (t - i * ndp)] = v[u];
+            }
+        }
+    }
+}
+
 // Classes whose dense T_N fits in LDS beside the determinants of `cpb` configurations (up to 10
 // open shells: 42 x 252 doubles for singlets) use dense products: no index loads, every T_N
 // element and every staged determinant read from LDS, 2 x 2 outputs per thread.
@@ -176,35 +271,31 @@ __global__ __launch_bounds__(256) void k_csf_gather_dense(const TI* __restrict__
                                                           const int64_t* __restrict__ off, int64_t nconf, int cpb,
                                                           int ncsf, int ndp, const double* __restrict__ td,
                                                           const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                                                          const uint32_t* __restrict__ B) {
+                                                          const uint32_t* __restrict__ B,
+                                                          const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     extern __shared__ double sh[];
-    double* T = sh;                       // [ncsf][ndp]
-    double* X = sh + size_t(ncsf) * ndp;  // [cpb][ndp]
+    const int ts = csf_tstride(ndp);
+    double* T = sh;                      // [ncsf][ts]
+    double* X = sh + size_t(ncsf) * ts;  // [cpb][ndp]
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
-    for (int t = threadIdx.x; t < ncsf * ndp; t += blockDim.x) T[t] = td[t];
-    for (int t = threadIdx.x; t < cpb * ndp; t += blockDim.x) {
-        const int cc = t / ndp, j = t - cc * ndp;
-        double v = 0.0;
-        if (cc < nc) {
-            const uint32_t dw = docc[c0 + cc];
-            uint32_t a, b;
-            csf_strings(dw & ~kConfNeg, socc[c0 + cc], occ[j], a, b);
-            v = double(det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)]);
-            if (dw & kConfNeg) v = -v;
-        }
-        X[t] = v;
-    }
+    csf_stage_table(T, td, ncsf, ndp, ts);
+    csf_stage_dets(det, X, nc, cpb, ndp, docc + c0, socc + c0, occ, rl, fast, n, nbs, B);
     __syncthreads();
-    // 2 x 2 tiles: configurations (cc, cc + 1) x rows (i, i + 1)
-    const int tc = (nc + 1) / 2, ti = (ncsf + 1) / 2;
-    for (int t = threadIdx.x; t < tc * ti; t += blockDim.x) {
-        const int cc = 2 * (t / ti), i = 2 * (t % ti);
-        const bool c2 = cc + 1 < nc, i2 = i + 1 < ncsf;
+    // 2 x 2 tiles: configurations (cc, cc + 1) x rows (i, i + hi); consecutive lanes take
+    // consecutive rows, whose odd LDS stride puts them in distinct banks
+    const int tc = (nc + 1) / 2, hi = (ncsf + 1) / 2;
+    for (int t = threadIdx.x; t < tc * hi; t += blockDim.x) {
+        const int cc = 2 * (t / hi), i = t % hi;
+        const bool c2 = cc + 1 < nc, i2 = i + hi < ncsf;
         const double* x0 = X + cc * ndp;
         const double* x1 = X + (c2 ? cc + 1 : cc) * ndp;
-        const double* t0 = T + i * ndp;
-        const double* t1 = T + (i2 ? i + 1 : i) * ndp;
+        const double* t0 = T + i * ts;
+        const double* t1 = T + (i2 ? i + hi : i) * ts;
         double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
         for (int j = 0; j < ndp; ++j) {
             const double u0 = t0[j], u1 = t1[j], v0 = x0[j], v1 = x1[j];
@@ -215,11 +306,11 @@ __global__ __launch_bounds__(256) void k_csf_gather_dense(const TI* __restrict__
         }
         const int64_t o0 = off[c0 + cc];
         csf[o0 + i] = TO(a00);
-        if (i2) csf[o0 + i + 1] = TO(a01);
+        if (i2) csf[o0 + i + hi] = TO(a01);
         if (c2) {
             const int64_t o1 = off[c0 + cc + 1];
             csf[o1 + i] = TO(a10);
-            if (i2) csf[o1 + i + 1] = TO(a11);
+            if (i2) csf[o1 + i + hi] = TO(a11);
         }
     }
 }
@@ -232,13 +323,18 @@ __global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restr
                                                            const int64_t* __restrict__ off, int64_t nconf, int cpb,
                                                            int ncsf, int ndp, const double* __restrict__ td,
                                                            const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                                                           const uint32_t* __restrict__ B) {
+                                                           const uint32_t* __restrict__ B,
+                                                           const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     extern __shared__ double sh[];
     double* T = sh;                       // [ncsf][ndp]
     double* Y = sh + size_t(ncsf) * ndp;  // [cpb][ncsf], signed by the configuration factor
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
-    for (int t = threadIdx.x; t < ncsf * ndp; t += blockDim.x) T[t] = td[t];
+    csf_stage_table(T, td, ncsf, ndp, ndp);
     for (int t = threadIdx.x; t < cpb * ncsf; t += blockDim.x) {
         const int cc = t / ncsf, i = t - cc * ncsf;
         double v = 0.0;
@@ -249,13 +345,15 @@ __global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restr
         Y[t] = v;
     }
     __syncthreads();
-    const int tc = (nc + 1) / 2, tj = (ndp + 1) / 2;
-    for (int t = threadIdx.x; t < tc * tj; t += blockDim.x) {
-        const int cc = 2 * (t / tj), j = 2 * (t % tj);
-        const bool c2 = cc + 1 < nc, j2 = j + 1 < ndp;
+    // 2 x 2 tiles: configurations (cc, cc + 1) x patterns (j, j + hj); consecutive lanes read
+    // consecutive T_N elements
+    const int tc = (nc + 1) / 2, hj = (ndp + 1) / 2;
+    for (int t = threadIdx.x; t < tc * hj; t += blockDim.x) {
+        const int cc = 2 * (t / hj), j = t % hj;
+        const bool c2 = cc + 1 < nc, j2 = j + hj < ndp;
         const double* y0 = Y + cc * ncsf;
         const double* y1 = Y + (c2 ? cc + 1 : cc) * ncsf;
-        const int jb = j2 ? j + 1 : j;
+        const int jb = j2 ? j + hj : j;
         double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
         for (int i = 0; i < ncsf; ++i) {
             const double u0 = T[i * ndp + j], u1 = T[i * ndp + jb], v0 = y0[i], v1 = y1[i];
@@ -268,9 +366,9 @@ __global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restr
             const uint32_t dw = docc[c0 + cc + q], so = socc[c0 + cc + q];
             for (int r = 0; r < (j2 ? 2 : 1); ++r) {
                 uint32_t a, b;
-                csf_strings(dw & ~kConfNeg, so, occ[j + r], a, b);
+                csf_strings(dw & ~kConfNeg, so, occ[r == 0 ? j : jb], a, b);
                 const double v = q == 0 ? (r == 0 ? a00 : a01) : (r == 0 ? a10 : a11);
-                det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)] = TO(v);
+                det[CSF_ADDR(a, b)] = TO(v);
             }
         }
     }
@@ -283,10 +381,16 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
                                                          const uint32_t* __restrict__ docc,
                                                          const uint32_t* __restrict__ socc,
                                                          const int64_t* __restrict__ off, int64_t nconf, int cpb,
-                                                         int ppb, int ncsf, int ndp, const int* __restrict__ cp,
+                                                         int ppb, int mode, int ncsf, int ndp,
+                                                         const int* __restrict__ cp,
                                                          const uint32_t* __restrict__ ci, const double* __restrict__ cv,
                                                          const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                                                         const uint32_t* __restrict__ B) {
+                                                         const uint32_t* __restrict__ B,
+                                                         const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     extern __shared__ double ys[];  // [cpb][ncsf], signed by the configuration factor
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
@@ -298,6 +402,48 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
     __syncthreads();
     const int ng = (nc + kCsfCT - 1) / kCsfCT;
     const int j0 = blockIdx.y * ppb, j1 = min(ndp, j0 + ppb), np = j1 - j0;
+    if (mode == 2) {
+        // one pattern per wave, its column's nonzeros over the lanes (hundreds at 14+ open shells)
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        for (int j = j0 + wave; j < j1; j += nw) {
+            const int e0 = cp[j], e1 = cp[j + 1];
+            const uint32_t m = occ[j];
+            for (int g = 0; g < ng; ++g) {
+                const int cb = g * kCsfCT, kk = min(kCsfCT, nc - cb);
+                const double* y0 = ys + cb * ncsf;
+                double acc[kCsfCT];
+#pragma unroll
+                for (int k = 0; k < kCsfCT; ++k) acc[k] = 0.0;
+                for (int e = e0 + lane; e < e1; e += 4 * 64) {
+                    double v[4];
+                    uint32_t ii[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int eu = e + 64 * u;
+                        v[u] = eu < e1 ? cv[eu] : 0.0;
+                        ii[u] = eu < e1 ? ci[eu] : 0u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int k = 0; k < kCsfCT; ++k)
+                            if (k < kk) acc[k] += v[u] * y0[k * ncsf + ii[u]];
+                }
+                double mine = 0.0;
+#pragma unroll
+                for (int k = 0; k < kCsfCT; ++k) {
+                    const double t = csf_wave_sum(acc[k]);
+                    if (lane == k) mine = t;
+                }
+                if (lane < kk) {
+                    uint32_t a, b;
+                    csf_strings(docc[c0 + cb + lane] & ~kConfNeg, socc[c0 + cb + lane], m, a, b);
+                    det[CSF_ADDR(a, b)] = TO(mine);
+                }
+            }
+        }
+        return;
+    }
     for (int t = threadIdx.x; t < ng * np; t += blockDim.x) {
         const int g = t / np, j = j0 + (t - g * np);
         const int cb = g * kCsfCT, kk = min(kCsfCT, nc - cb);
@@ -318,8 +464,67 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
             if (k < kk) {
                 uint32_t a, b;
                 csf_strings(docc[c0 + cb + k] & ~kConfNeg, socc[c0 + cb + k], m, a, b);
-                det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)] = TO(acc[k]);
+                det[CSF_ADDR(a, b)] = TO(acc[k]);
             }
+    }
+}
+
+// csf -> det for classes with many configurations and long T_N columns: a wave covers cpw
+// configurations (lanes) x 64 / cpw patterns; the nonzeros of a pattern's column are read by all the
+// lanes of that pattern together and each lane reads its own configuration's CSF coefficients from
+// LDS (row stride ncsf | 1, so the lanes' rows fall in distinct banks)
+template <class TO>
+__global__ __launch_bounds__(256) void k_csf_scatter_cfg(const double* __restrict__ csf, TO* __restrict__ det,
+                                                         const uint32_t* __restrict__ docc,
+                                                         const uint32_t* __restrict__ socc,
+                                                         const int64_t* __restrict__ off, int64_t nconf, int cpw,
+                                                         int ppb, int ncsf, int ndp, const int* __restrict__ cp,
+                                                         const uint32_t* __restrict__ ci, const double* __restrict__ cv,
+                                                         const uint32_t* __restrict__ occ, int n, int64_t nbs,
+                                                         const uint32_t* __restrict__ B,
+                                                         const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    extern __shared__ double ys[];  // [cpw][ncsf | 1], signed by the configuration factor
+    const int ys_st = ncsf | 1;
+    const int64_t c0 = int64_t(blockIdx.x) * cpw;
+    const int nc = int(min(int64_t(cpw), nconf - c0));
+    for (int t = threadIdx.x; t < nc * ncsf; t += blockDim.x) {
+        const int cc = t / ncsf, i = t - cc * ncsf;
+        const double v = csf[off[c0 + cc] + i];
+        ys[cc * ys_st + i] = (docc[c0 + cc] & kConfNeg) ? -v : v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cc = lane % cpw, sub = lane / cpw, pw = 64 / cpw;  // patterns per wave
+    const bool live = cc < nc;
+    const uint32_t dw = live ? docc[c0 + cc] : 0u, so = live ? socc[c0 + cc] : 0u;
+    const double* y = ys + (live ? cc : 0) * ys_st;
+    const int j0 = blockIdx.y * ppb, j1 = min(ndp, j0 + ppb);
+    for (int jb = j0 + wave * pw; jb < j1; jb += 4 * pw) {
+        const int j = jb + sub;
+        const bool ok = j < j1;
+        const int e0 = ok ? cp[j] : 0, e1 = ok ? cp[j + 1] : 0;
+        double acc = 0.0;
+        int e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            double v[4];
+            uint32_t ii[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v[u] = cv[e + u];
+                ii[u] = ci[e + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u] * y[ii[u]];
+        }
+        for (; e < e1; ++e) acc += cv[e] * y[ci[e]];
+        if (live && ok) {
+            uint32_t a, b;
+            csf_strings(dw & ~kConfNeg, so, occ[j], a, b);
+            det[CSF_ADDR(a, b)] = TO(acc);
+        }
     }
 }
 
@@ -330,7 +535,11 @@ __global__ void k_csf_scatter(const double* __restrict__ csf, TO* __restrict__ d
                               const uint32_t* __restrict__ socc, const int64_t* __restrict__ off, int64_t nconf,
                               int ndp, const int* __restrict__ cp, const uint32_t* __restrict__ ci,
                               const double* __restrict__ cv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                              const uint32_t* __restrict__ B) {
+                              const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= nconf * ndp) return;
     const int64_t c = t / ndp;
@@ -341,7 +550,7 @@ __global__ void k_csf_scatter(const double* __restrict__ csf, TO* __restrict__ d
     for (int e = cp[j]; e < cp[j + 1]; ++e) acc += cv[e] * csf[o + ci[e]];
     uint32_t a, b;
     csf_strings(dw & ~kConfNeg, so, occ[j], a, b);
-    det[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)] = TO((dw & kConfNeg) ? -acc : acc);
+    det[CSF_ADDR(a, b)] = TO((dw & kConfNeg) ? -acc : acc);
 }
 
 // FCISolver::form_Hdiag_csf without the full preconditioner (fci_solver_initial_guess.cc:134-141):
@@ -350,7 +559,11 @@ __global__ void k_csf_hdiag(const double* __restrict__ hd, double* __restrict__ 
                             const uint32_t* __restrict__ socc, const int64_t* __restrict__ off, int64_t nconf,
                             int ncsf, const int* __restrict__ rp, const uint32_t* __restrict__ rj,
                             const double* __restrict__ rv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
-                            const uint32_t* __restrict__ B) {
+                            const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
+    __shared__ uint16_t rl[kRankTab];
+    const bool fast = RT != nullptr;
+    csf_rank_stage(rl, RT);
+    __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= nconf * ncsf) return;
     const int64_t c = t / ncsf;
@@ -360,7 +573,7 @@ __global__ void k_csf_hdiag(const double* __restrict__ hd, double* __restrict__ 
     for (int e = rp[i]; e < rp[i + 1]; ++e) {
         uint32_t a, b;
         csf_strings(d, so, occ[rj[e]], a, b);
-        acc += rv[e] * rv[e] * hd[int64_t(csf_rank(a, n, B)) * nbs + csf_rank(b, n, B)];
+        acc += rv[e] * rv[e] * hd[CSF_ADDR(a, b)];
     }
     out[off[c] + i] = acc;
 }
@@ -543,8 +756,30 @@ CsfSpace::CsfSpace(int n, int na, int nb, int twoS, const std::vector<int>* orbs
     }
     ncsf_ = offc;
     nconf_ = int64_t(keys.size());
+    if (n <= 16) {
+        // H[v]: bits of v at orbitals 8..15, ordinals from the top; L[c][v]: bits of v at orbitals
+        // 0..7 below c electrons of the high byte (entries of byte values no string has are unused)
+        std::vector<uint16_t> tab(kRankTab, 0);
+        auto part = [&](uint32_t v, int base, int c) {
+            uint64_t r = 0;
+            for (int i = 7; i >= 0; --i)
+                if ((v >> i) & 1u) {
+                    const int orb = base + i;
+                    if (orb >= n) return uint16_t(0);
+                    r += binom_host(n - 1 - orb, 1 + c + __builtin_popcount(v >> (i + 1)));
+                }
+            return uint16_t(r);
+        };
+        for (uint32_t v = 0; v < 256; ++v) {
+            tab[v] = part(v, 8, 0);
+            for (int c = 0; c <= 8; ++c) tab[256 + c * 256 + v] = part(v, 0, c);
+        }
+        rtab_.alloc(tab.size());
+        rtab_.upload(tab.data(), tab.size());
+    }
     for (auto& C : cls_) {
         C.nconf = int64_t(C.h_docc.size());
+        ndet_cov_ += C.nconf * C.ndp;
         C.docc.alloc(C.nconf);
         C.docc.upload(C.h_docc.data(), C.nconf);
         C.socc.alloc(C.nconf);
@@ -582,34 +817,42 @@ template <class TI, class TO> void CsfSpace::to_csf(const TI* det, TO* csf, hipS
     for (const auto& C : cls_) {
         if (C.nconf == 0 || C.ncsf == 0) continue;
         const size_t row_bytes = size_t(C.ndp) * sizeof(double);
-        const size_t tbytes = size_t(C.ncsf) * row_bytes;
+        const size_t tbytes = size_t(C.ncsf) * csf_tstride(C.ndp) * sizeof(double);
         if (C.td.p && tbytes + 8 * row_bytes <= kCsfDenseTotal) {
             const int cpb = int(std::min<size_t>(64, (kCsfDenseTotal - tbytes) / row_bytes));
             k_csf_gather_dense<TI, TO><<<unsigned((C.nconf + cpb - 1) / cpb), 256, tbytes + size_t(cpb) * row_bytes, st>>>(
-                det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_);
+                det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_, rtab_.p);
         } else if (row_bytes <= kCsfLdsBytes) {
-            // as many configurations per block as fit in 96 KB of LDS (at least one); rows split
-            // over blocks so that classes with few configurations still fill the chip
-            const int cpb = int(std::max<size_t>(1, std::min<size_t>(64, (96u << 10) / row_bytes)));
+            // up to as many configurations per block as fit in 64 KB of LDS (at least one), fewer
+            // when that leaves under 512 blocks; rows split over blocks so that classes with few
+            // configurations still fill the chip (each split stages the determinants again)
+            const int64_t fit = int64_t(std::max<size_t>(1, std::min<size_t>(64, (64u << 10) / row_bytes)));
+            const int cpb = int(std::max<int64_t>(1, std::min<int64_t>(fit, C.nconf / 512)));
             const int64_t nxb = (C.nconf + cpb - 1) / cpb;
             const double per_row = double(C.h_rj.size()) / C.ncsf;
             const int mode = per_row >= 32.0 ? 2 : (C.ncsf < 64 ? 0 : 1);
-            const int rpb = mode == 0 ? C.ncsf : (mode == 2 ? 16 : 256);
+            int rpb = mode == 0 ? C.ncsf : 256;
+            if (mode == 2) {
+                const int64_t nyw = std::max<int64_t>(1, std::min<int64_t>((1024 + nxb - 1) / nxb, (C.ncsf + 7) / 8));
+                rpb = int((C.ncsf + nyw - 1) / nyw);
+            }
             const unsigned nyb = unsigned((C.ncsf + rpb - 1) / rpb);
             k_csf_gather_lds<TI, TO><<<dim3(unsigned(nxb), nyb), 256, size_t(cpb) * row_bytes, st>>>(
                 det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, rpb, mode, C.ncsf, C.ndp, C.rp.p, C.rj.p, C.rv.p,
-                C.occ.p, n_, nbs_, binom_);
+                C.occ.p, n_, nbs_, binom_, rtab_.p);
         } else {
             k_csf_gather<TI, TO><<<grid_of(C.nconf * C.ncsf), 256, 0, st>>>(
                 det, csf, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ncsf, C.rp.p, C.rj.p, C.rv.p, C.occ.p, n_, nbs_,
-                binom_);
+                binom_, rtab_.p);
         }
     }
     HIPCHK(hipGetLastError());
 }
 
 template <class TO> void CsfSpace::to_det(const double* csf, TO* det, int64_t ndet, hipStream_t st) const {
-    HIPCHK(hipMemsetAsync(det, 0, size_t(ndet) * sizeof(TO), st));
+    // every determinant of the configurations is written once; the others (other irreps, other
+    // GAS occupations, configurations with too few open shells) are zero
+    if (ndet_cov_ != ndet) HIPCHK(hipMemsetAsync(det, 0, size_t(ndet) * sizeof(TO), st));
     for (const auto& C : cls_) {
         if (C.nconf == 0) continue;
         const size_t tbytes = size_t(C.ncsf) * C.ndp * sizeof(double);
@@ -621,17 +864,43 @@ template <class TO> void CsfSpace::to_det(const double* csf, TO* det, int64_t nd
         if (C.td.p && tbytes + 8 * ybytes <= kCsfDenseTotal) {
             const int cpd = int(std::min<size_t>(64, (kCsfDenseTotal - tbytes) / ybytes));
             k_csf_scatter_dense<TO><<<unsigned((C.nconf + cpd - 1) / cpd), 256, tbytes + size_t(cpd) * ybytes, st>>>(
-                csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpd, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_);
+                csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpd, C.ncsf, C.ndp, C.td.p, C.occ.p, n_, nbs_, binom_, rtab_.p);
         } else if (C.ncsf > 0 && ybytes <= (32u << 10)) {
-            const int64_t nxb = (C.nconf + cpb - 1) / cpb;
-            const int ppb = nxb >= 1024 ? C.ndp : 256;
-            k_csf_scatter_lds<TO><<<dim3(unsigned(nxb), unsigned((C.ndp + ppb - 1) / ppb)), 256, size_t(cpb) * ybytes,
-                                    st>>>(csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpb, ppb, C.ncsf, C.ndp,
-                                          C.cp.p, C.ci.p, C.cv.p, C.occ.p, n_, nbs_, binom_);
+            const double per_col = double(C.h_rj.size()) / C.ndp;  // nonzeros per T_N column
+            // lanes over configurations: as many (a power of two up to 64) as fit in 72 KB
+            int cpw = 64;
+            while (cpw > 8 && size_t(cpw) * (C.ncsf | 1) * sizeof(double) > (72u << 10)) cpw /= 2;
+            const bool fits = size_t(cpw) * (C.ncsf | 1) * sizeof(double) <= (72u << 10);
+            if (per_col >= 16.0 && fits && C.nconf >= 8) {
+                const int64_t nxb = (C.nconf + cpw - 1) / cpw;
+                const int step = 4 * (64 / cpw);  // patterns per pass of a block
+                const int64_t nyw = std::max<int64_t>(1, std::min<int64_t>((1024 + nxb - 1) / nxb, C.ndp / step));
+                const int ppb = int(((C.ndp + nyw - 1) / nyw + step - 1) / step * step);
+                k_csf_scatter_cfg<TO><<<dim3(unsigned(nxb), unsigned((C.ndp + ppb - 1) / ppb)), 256,
+                                        size_t(cpw) * (C.ncsf | 1) * sizeof(double), st>>>(
+                    csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, cpw, ppb, C.ncsf, C.ndp, C.cp.p, C.ci.p, C.cv.p,
+                    C.occ.p, n_, nbs_, binom_, rtab_.p);
+            } else if (per_col >= 16.0) {
+                // few configurations: one pattern per wave, lanes over its column
+                const int cpw2 = int(std::max<int64_t>(1, std::min<int64_t>(C.nconf, std::min<int>(8, cpb))));
+                const int64_t nxb = (C.nconf + cpw2 - 1) / cpw2;
+                const int ppb = 16;
+                k_csf_scatter_lds<TO><<<dim3(unsigned(nxb), unsigned((C.ndp + ppb - 1) / ppb)), 256,
+                                        size_t(cpw2) * ybytes, st>>>(csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf,
+                                                                     cpw2, ppb, 2, C.ncsf, C.ndp, C.cp.p, C.ci.p,
+                                                                     C.cv.p, C.occ.p, n_, nbs_, binom_, rtab_.p);
+            } else {
+                const int64_t nxb = (C.nconf + cpb - 1) / cpb;
+                const int ppb = nxb >= 1024 ? C.ndp : 256;
+                k_csf_scatter_lds<TO><<<dim3(unsigned(nxb), unsigned((C.ndp + ppb - 1) / ppb)), 256,
+                                        size_t(cpb) * ybytes, st>>>(csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf,
+                                                                    cpb, ppb, 0, C.ncsf, C.ndp, C.cp.p, C.ci.p,
+                                                                    C.cv.p, C.occ.p, n_, nbs_, binom_, rtab_.p);
+            }
         } else {
             k_csf_scatter<TO><<<grid_of(C.nconf * C.ndp), 256, 0, st>>>(
                 csf, det, C.docc.p, C.socc.p, C.off.p, C.nconf, C.ndp, C.cp.p, C.ci.p, C.cv.p, C.occ.p, n_, nbs_,
-                binom_);
+                binom_, rtab_.p);
         }
     }
     HIPCHK(hipGetLastError());
@@ -642,7 +911,7 @@ void CsfSpace::hdiag(const double* hdet, double* hcsf, hipStream_t st) const {
         if (C.nconf == 0 || C.ncsf == 0) continue;
         k_csf_hdiag<<<grid_of(C.nconf * C.ncsf), 256, 0, st>>>(hdet, hcsf, C.docc.p, C.socc.p, C.off.p, C.nconf,
                                                               C.ncsf, C.rp.p, C.rj.p, C.rv.p, C.occ.p, n_, nbs_,
-                                                              binom_);
+                                                              binom_, rtab_.p);
     }
     HIPCHK(hipGetLastError());
 }

@@ -334,6 +334,8 @@ class CsfSpace {
     int64_t nbs_;
     const uint32_t* binom_;
     int64_t ncsf_ = 0, ncoupling_ = 0, nconf_ = 0;
+    int64_t ndet_cov_ = 0;    // determinants of the configurations (to_det writes each once)
+    DBuf<uint16_t> rtab_;     // byte tables of the string ranks (n <= 16), else empty
     std::vector<CsfClass> cls_;
 };
 

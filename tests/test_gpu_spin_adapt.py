@@ -72,6 +72,62 @@ def test_transforms_and_diagonal_vs_oracle(oracle, n, na, nb, twoS, orbsym, irre
     assert np.abs(s.hdiag_csf() - np.diag(T @ H @ T.T)).max() < 1e-11
 
 
+@pytest.mark.parametrize("n,na,nb,twoS", [(13, 7, 6, 1), (14, 7, 7, 0), (14, 7, 7, 2), (12, 8, 4, 4)])
+def test_transforms_of_large_open_shell_classes_sampled(n, na, nb, twoS):
+    """Every open-shell class, including those of 10 to 14 open shells that the dense kernels do not
+    take (row-per-wave gathers, pattern-per-wave scatters, scatters with lanes over configurations),
+    at sizes beyond a full oracle transform: sampled configurations of each class, their CSF offsets
+    from the oracle's configuration order, rows and columns of T built from the oracle's couplings,
+    overlaps and determinant signs, against det_to_csf of a random determinant vector and
+    csf_to_det of a random CSF vector; and the round trip det_to_csf(csf_to_det(c)) = c."""
+    from oracle.oracle import make_strings
+    from oracle.spin_adapt import (_popc, configurations, csf_determinant, det_occupations, overlap,
+                                   spin_couplings)
+    say(f"large classes n={n} na={na} nb={nb} 2S={twoS}")
+    p = seeded_problem(n, na, nb, 7)
+    s = solver(p, 1, twoS + 1)
+    confs = [c for c in configurations(n, na, nb) if _popc(c[1]) >= twoS]
+    cpl, occ = {}, {}
+    offs, by_class = [], {}
+    off = 0
+    for k, (docc, socc) in enumerate(confs):
+        N = _popc(socc)
+        if N not in cpl:
+            cpl[N], occ[N] = spin_couplings(N, twoS), det_occupations(N, na - nb)
+        offs.append(off)
+        off += len(cpl[N])
+        by_class.setdefault(N, []).append(k)
+    assert s.csf_size()[0] == off
+    ra = {int(x): i for i, x in enumerate(make_strings(n, na))}
+    sb = make_strings(n, nb)
+    rb = {int(x): i for i, x in enumerate(sb)}
+    rng = np.random.default_rng(n + twoS)
+    d = rng.standard_normal(s.ndet())
+    c = rng.standard_normal(off)
+    got_csf, got_det = s.det_to_csf(d), s.csf_to_det(c)
+    assert np.abs(s.det_to_csf(got_det) - c).max() < 1e-12
+    for N, members in sorted(by_class.items()):
+        pick = sorted({members[0], members[-1], *rng.choice(members, size=min(3, len(members)), replace=False)})
+        rows = sorted({0, len(cpl[N]) - 1, *rng.choice(len(cpl[N]), size=min(4, len(cpl[N])), replace=False)})
+        cols = sorted({0, len(occ[N]) - 1, *rng.choice(len(occ[N]), size=min(4, len(occ[N])), replace=False)})
+        err_c = err_d = 0.0
+        for k in pick:
+            docc, socc = confs[k]
+            dets = [csf_determinant(docc, socc, o) for o in occ[N]]
+            addr = [ra[a] * len(sb) + rb[b] for a, b, _ in dets]
+            for i in rows:
+                want = sum(sg * ov * d[ad] for (_, _, sg), ad, o in zip(dets, addr, occ[N])
+                           if (ov := overlap(N, cpl[N][i], o)) != 0.0)
+                err_c = max(err_c, abs(got_csf[offs[k] + i] - want))
+            for j in cols:
+                _, _, sg = dets[j]
+                want = sum(sg * overlap(N, cp, occ[N][j]) * c[offs[k] + i] for i, cp in enumerate(cpl[N]))
+                err_d = max(err_d, abs(got_det[addr[j]] - want))
+        say(f"  N={N}: {len(members)} configurations, {len(cpl[N])} CSFs x {len(occ[N])} determinants each; "
+            f"max errors det->csf {err_c:.1e} csf->det {err_d:.1e}")
+        assert err_c < 1e-12 and err_d < 1e-12
+
+
 SOLVES = [(6, 3, 3, 0, 3, None, 0), (6, 3, 3, 2, 2, None, 0), (7, 4, 3, 1, 2, None, 0), (7, 4, 3, 3, 1, None, 0),
           (8, 4, 4, 0, 2, [0, 1, 0, 1, 2, 3, 2, 3], 1), (8, 4, 4, 4, 1, None, 0)]
 
