@@ -31,7 +31,8 @@ written and re-read), inputs resident in HBM. The line also carries:
                     the symmetry off ("/full_sigma", every sigma built in full as in Forte);
   configs       CAS(8,8) FP64 and CAS(14,14) FP64 3-root solves (with root-0 RDM time) and their
                 CPU baselines;
-  spin_adaptation  CAS(14,14) solved in determinants and in CSFs (CI_SPIN_ADAPT);
+  spin_adaptation  CAS(14,14) (or CAS(n,n) with --spin-adapt-n) solved in determinants and in
+                   CSFs (CI_SPIN_ADAPT);
   batched       512 small problems in one batched call against 16 concurrent solvers;
   cpu_baseline  the CPU restatement of Forte's sigma (oracle/fci_oracle.c, one core) on a
                 strided sample of its loops, extrapolated to one full sigma.
@@ -194,14 +195,15 @@ def other_configs(forte_amd, with_cpu):
     return out
 
 
-def spin_adapt_leg(forte_amd):
-    """CAS(14,14) singlet, seed 3, FP64, e_conv 1e-9 / r_conv 1e-6: the determinant-basis solve and
-    the CSF-basis one (CI_SPIN_ADAPT), with the CSF and coupling counts."""
-    h, eri = forte_amd.seeded_integrals(14, 3)
+def spin_adapt_leg(forte_amd, n=14):
+    """CAS(n,n) singlet (n = 14 by default), seed 3, FP64, e_conv 1e-9 / r_conv 1e-6: the
+    determinant-basis solve and the CSF-basis one (CI_SPIN_ADAPT), with the CSF and coupling counts."""
+    h, eri = forte_amd.seeded_integrals(n, 3)
     ints = forte_amd.make_active_space_integrals(h, eri)
-    out = {"workload": "CAS(14,14) seed 3 singlet, FP64, e_conv 1e-9, r_conv 1e-6"}
+    k = n // 2
+    out = {"workload": f"CAS({n},{n}) seed 3 singlet, FP64, e_conv 1e-9, r_conv 1e-6"}
     for label, sa in (("determinants", False), ("csf", True)):
-        s = forte_amd.FCISolver(7, 7, 1, 1, ints)
+        s = forte_amd.FCISolver(k, k, 1, 1, ints)
         s.set_spin_adapt(sa)
         s.set_e_convergence(1e-9)
         s.set_r_convergence(1e-6)
@@ -217,7 +219,7 @@ def spin_adapt_leg(forte_amd):
                           converged=st["converged"])
         print(f"# spin adaptation {label}: {out[label]}", file=sys.stderr, flush=True)
         del s
-    out["ndet"] = int(forte_amd.FCISolver(7, 7, 1, 1, ints).ndet())
+    out["ndet"] = int(forte_amd.FCISolver(k, k, 1, 1, ints).ndet())
     return out
 
 
@@ -242,6 +244,8 @@ def main():
                     help="time-to-solution legs to run, comma-separated names as in the output "
                          "(e.g. mixed+diagonal,fp64+diagonal/full_sigma), or all")
     ap.add_argument("--rconvs", default="1e-4,1e-6", help="residual thresholds of the time-to-solution legs")
+    ap.add_argument("--spin-adapt-n", type=int, default=14, choices=(8, 10, 12, 14, 16),
+                    help="active orbitals of the spin adaptation leg (CAS(n,n) singlet)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy kernel whose "
                          "FETCH_SIZE/WRITE_SIZE counters calibrate the traffic figure)")
@@ -493,7 +497,7 @@ def main():
     # ---- the other configurations: CAS(8,8) and CAS(14,14) FP64 solves ---------------------------
     if args.configs:
         result["configs"] = other_configs(forte_amd, args.cpu)
-        result["spin_adaptation"] = spin_adapt_leg(forte_amd)
+        result["spin_adaptation"] = spin_adapt_leg(forte_amd, args.spin_adapt_n)
 
     # ---- batched small problems -----------------------------------------------------------------
     if args.batch > 0:
