@@ -22,8 +22,11 @@
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
+#include <atomic>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <thread>
 
 namespace fci {
 
@@ -986,8 +989,7 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
             oa.alloc(n2); ob.alloc(n2); taa.alloc(n4); tab.alloc(n4); tbb.alloc(n4);
             std::vector<std::vector<SparseVec>> guesses(P), bad(P);
             C.ecore.resize(P);
-            for (int q = 0; q < P; ++q) {
-                const int i = ids[q];
+            auto host_ints = [&](int i) {
                 HostInts hi;
                 hi.n = n;
                 hi.oei_a.assign(oei_a[i], oei_a[i] + n2);
@@ -996,8 +998,17 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 hi.tei_ab.assign(tei_ab[i], tei_ab[i] + n4);
                 hi.tei_bb.assign(tei_bb[i], tei_bb[i] + n4);
                 hi.e_core = 0.0;  // solved without the constant; added to the energies at the end
+                return hi;
+            };
+            // device part, problem by problem on the class stream: packed integrals, H diagonal,
+            // the determinants of the guess space (one set of selection buffers for all)
+            const int ngd = int(std::min<int64_t>(int64_t(C.ng) * opts->ndets_per_guess, C.ndf));
+            std::vector<std::vector<int64_t>> sel_idx(P);
+            SelectScratch sel;
+            for (int q = 0; q < P; ++q) {
+                const int i = ids[q];
                 C.ecore[q] = e_core ? e_core[i] : 0.0;
-                PackedInts pk = pack_integrals(hi, C.sa, C.sb);
+                PackedInts pk = pack_integrals(host_ints(i), C.sa, C.sb);
                 if (q == 0) {
                     C.gd[0] = pk.gd[0];
                     C.gd[1] = pk.gd[1];
@@ -1009,30 +1020,52 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 taa.upload(tei_aa[i], n4); tab.upload(tei_ab[i], n4); tbb.upload(tei_bb[i], n4);
                 double* hdq = C.eta ? hfull.p : C.hd.p + size_t(q) * nd;
                 hdiag_device(C.sa, C.sb, oa.p, ob.p, taa.p, tab.p, tbb.p, 0.0, hdq, C.stream);
-                HIPCHK(hipStreamSynchronize(C.stream));
-                auto gb = initial_guess_dets(hdq, C.ndf, C.ndf, C.sb.nstr, n, C.na, C.nb, hi, opts->multiplicity,
-                                             opts->ndets_per_guess, C.ng, C.stream);
-                guesses[q] = std::move(gb.first);
-                bad[q] = std::move(gb.second);
-                if (C.eta) {
-                    // the diagonal in the packed basis (a determinant and its mirror averaged), the
-                    // guesses and the roots to project out in packed coordinates (roots of the other
-                    // spin parity vanish there)
+                std::vector<double> vals;
+                smallest_k(hdq, C.ndf, ngd, vals, sel_idx[q], C.stream, &sel);  // ends synchronized
+                // the diagonal in the packed basis (a determinant and its mirror averaged); the
+                // next problem's diagonal is written after this on the same stream
+                if (C.eta)
                     ms0_pack_device<double, double>(hfull.p, C.hd.p + size_t(q) * nd, C.N, C.eta, 1, 0.5, 1.0,
                                                     C.stream);
-                    HIPCHK(hipStreamSynchronize(C.stream));
-                    for (auto& g : guesses[q]) g = ms0_pack_sparse(g, C.N, C.eta);
-                    std::vector<SparseVec> keep;
-                    for (const auto& v : bad[q]) {
-                        SparseVec w = ms0_pack_sparse(v, C.N, C.eta);
-                        double n2 = 0.0;
-                        for (const auto& e : w) n2 += e.second * e.second;
-                        if (n2 < 1e-12) continue;
-                        for (auto& e : w) e.second /= std::sqrt(n2);
-                        keep.push_back(std::move(w));
+            }
+            HIPCHK(hipStreamSynchronize(C.stream));
+            // host part, problems over threads: the guess vectors and the roots of other
+            // multiplicities to project out (in packed coordinates for symmetric classes, where
+            // roots of the other spin parity vanish)
+            {
+                std::atomic<int> next{0};
+                std::vector<std::exception_ptr> err(P);
+                auto work = [&] {
+                    for (int q; (q = next.fetch_add(1)) < P;) {
+                        try {
+                            auto gb = initial_guess_from_selection(sel_idx[q], C.sb.nstr, n, C.na, C.nb,
+                                                                   host_ints(ids[q]), opts->multiplicity, C.ng);
+                            guesses[q] = std::move(gb.first);
+                            bad[q] = std::move(gb.second);
+                            if (!C.eta) continue;
+                            for (auto& g : guesses[q]) g = ms0_pack_sparse(g, C.N, C.eta);
+                            std::vector<SparseVec> keep;
+                            for (const auto& v : bad[q]) {
+                                SparseVec w = ms0_pack_sparse(v, C.N, C.eta);
+                                double w2 = 0.0;
+                                for (const auto& e : w) w2 += e.second * e.second;
+                                if (w2 < 1e-12) continue;
+                                for (auto& e : w) e.second /= std::sqrt(w2);
+                                keep.push_back(std::move(w));
+                            }
+                            bad[q] = std::move(keep);
+                        } catch (...) {
+                            err[q] = std::current_exception();
+                        }
                     }
-                    bad[q] = std::move(keep);
-                }
+                };
+                const int nth = std::max(1, std::min<int>({P, 16, int(std::thread::hardware_concurrency())}));
+                std::vector<std::thread> pool;
+                for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+                work();
+                for (auto& t : pool) t.join();
+                for (auto& e : err)
+                    if (e) std::rethrow_exception(e);
             }
             C.g2a.alloc(g2a_h.size()); C.g2a.upload(g2a_h.data(), g2a_h.size());
             C.g2b.alloc(g2b_h.size()); C.g2b.upload(g2b_h.data(), g2b_h.size());

@@ -10,6 +10,12 @@ namespace fci {
 
 using SparseVec = std::vector<std::pair<int64_t, double>>;
 
+// The guess construction of initial_guess_dets from already selected determinant addresses
+// (host only: spin completion, S^2 and H blocks, per-multiplicity eigenvectors).
+std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_from_selection(
+    const std::vector<int64_t>& idx, int64_t nbs, int n, int na, int nb, const HostInts& hi, int multiplicity,
+    int num_guess_states);
+
 // Symmetric eigensolver (cyclic Jacobi), ascending; V column-major: eigenvector k at V[k*n..].
 void jacobi_eigh(int n, const std::vector<double>& A, std::vector<double>& evals, std::vector<double>& V);
 
@@ -21,7 +27,8 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_dets(con
                                                                              int64_t nsel, int64_t nbs, int n, int na,
                                                                              int nb, const HostInts& hi,
                                                                              int multiplicity, int ndets_per_guess,
-                                                                             int num_guess_states, hipStream_t st);
+                                                                             int num_guess_states, hipStream_t st,
+                                                                             SelectScratch* ws = nullptr);
 
 class DavidsonLiu {
   public:

@@ -125,8 +125,14 @@ void hdiag_device(const SpinTables& a, const SpinTables& b, const double* d_oei_
                   const double* d_tei_bb, double e_core, double* d_out, hipStream_t st);
 
 // k smallest (value, index) pairs of a device array, ties by index; result sorted ascending.
+// ws: reduction buffers kept across calls (grow only), so repeated selections (one per problem
+// of a batch) allocate nothing after the first; null for buffers of this call only
+struct SelectScratch {
+    DBuf<double> v[2];
+    DBuf<int64_t> i[2];
+};
 void smallest_k(const double* d_v, int64_t n, int k, std::vector<double>& vals,
-                std::vector<int64_t>& idx, hipStream_t st);
+                std::vector<int64_t>& idx, hipStream_t st, SelectScratch* ws = nullptr);
 
 // ---- integrals on the device ------------------------------------------------------------------
 struct DevInts {

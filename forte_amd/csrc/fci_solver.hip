@@ -671,11 +671,18 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_dets(con
                                                                              int64_t nsel, int64_t nbs, int n_, int na_,
                                                                              int nb_, const HostInts& hi_,
                                                                              int multiplicity, int ndets_per_guess,
-                                                                             int num_guess_states, hipStream_t st) {
+                                                                             int num_guess_states, hipStream_t st,
+                                                                             SelectScratch* ws) {
     int64_t ngd = std::min<int64_t>(int64_t(num_guess_states) * ndets_per_guess, nsel);
     std::vector<double> vals;
     std::vector<int64_t> idx;
-    smallest_k(d_hdiag, ndet_, int(ngd), vals, idx, st);
+    smallest_k(d_hdiag, ndet_, int(ngd), vals, idx, st, ws);
+    return initial_guess_from_selection(idx, nbs, n_, na_, nb_, hi_, multiplicity, num_guess_states);
+}
+
+std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_from_selection(
+    const std::vector<int64_t>& idx, int64_t nbs, int n_, int na_, int nb_, const HostInts& hi_, int multiplicity,
+    int num_guess_states) {
     std::vector<Det> dets;
     for (int64_t I : idx) dets.push_back({unrank_host(uint64_t(I / nbs), n_, na_), unrank_host(uint64_t(I % nbs), n_, nb_)});
     spin_complete(dets, n_);
@@ -690,17 +697,23 @@ std::pair<std::vector<SparseVec>, std::vector<SparseVec>> initial_guess_dets(con
         }
     std::vector<double> s2val, U;  // U column-major: eigenvector k at U[k*G..]
     jacobi_eigh(G, S2, s2val, U);
+    // U^T H U over the nonzeros of each eigenvector of S^2 (the rotations mix only determinants of
+    // one spin family, the other entries stay exactly zero): the same sums without the zero terms
+    std::vector<std::vector<int>> nzU(G);
+    for (int k = 0; k < G; ++k)
+        for (int j = 0; j < G; ++j)
+            if (U[size_t(k) * G + j] != 0.0) nzU[k].push_back(j);
     std::vector<double> HU(size_t(G) * G, 0.0), HS2(size_t(G) * G, 0.0);
     for (int i = 0; i < G; ++i)
         for (int k = 0; k < G; ++k) {
             double s = 0.0;
-            for (int j = 0; j < G; ++j) s += H[size_t(i) * G + j] * U[size_t(k) * G + j];
+            for (int j : nzU[k]) s += H[size_t(i) * G + j] * U[size_t(k) * G + j];
             HU[size_t(i) * G + k] = s;
         }
     for (int a = 0; a < G; ++a)
         for (int k = 0; k < G; ++k) {
             double s = 0.0;
-            for (int i = 0; i < G; ++i) s += U[size_t(a) * G + i] * HU[size_t(i) * G + k];
+            for (int i : nzU[a]) s += U[size_t(a) * G + i] * HU[size_t(i) * G + k];
             HS2[size_t(a) * G + k] = s;
         }
     std::vector<double> mult(G);

@@ -542,24 +542,26 @@ __global__ void __launch_bounds__(1024) k_select(const double* __restrict__ v, c
 }
 
 void smallest_k(const double* d_v, int64_t n, int k, std::vector<double>& vals,
-                std::vector<int64_t>& idx, hipStream_t st) {
+                std::vector<int64_t>& idx, hipStream_t st, SelectScratch* ws) {
     if (k > kSelChunk / 2) throw Error(-1, "smallest_k: k too large");
     if (k > n) k = int(n);
+    SelectScratch local;
+    SelectScratch& w = ws ? *ws : local;
     const double* cur_v = d_v;
     const int64_t* cur_i = nullptr;
     int64_t cur_n = n;
-    DBuf<double> bv[2];
-    DBuf<int64_t> bi[2];
     int which = 0;
     while (true) {
-        HIPCHK(hipStreamSynchronize(st));  // the buffer about to be reallocated may still be read
-        int64_t nb = (cur_n + kSelChunk - 1) / kSelChunk;
-        bv[which].alloc(nb * k);
-        bi[which].alloc(nb * k);
-        k_select<<<unsigned(nb), 1024, 0, st>>>(cur_v, cur_i, cur_n, k, bv[which].p, bi[which].p);
+        const int64_t nb = (cur_n + kSelChunk - 1) / kSelChunk;
+        if (w.v[which].n < size_t(nb * k) || w.i[which].n < size_t(nb * k)) {
+            HIPCHK(hipStreamSynchronize(st));  // the buffer about to be reallocated may still be read
+            w.v[which].alloc(nb * k);
+            w.i[which].alloc(nb * k);
+        }
+        k_select<<<unsigned(nb), 1024, 0, st>>>(cur_v, cur_i, cur_n, k, w.v[which].p, w.i[which].p);
         HIPCHK(hipGetLastError());
-        cur_v = bv[which].p;
-        cur_i = bi[which].p;
+        cur_v = w.v[which].p;
+        cur_i = w.i[which].p;
         cur_n = nb * k;
         if (nb == 1) break;
         which ^= 1;
