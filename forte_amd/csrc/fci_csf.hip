@@ -47,16 +47,25 @@ __device__ __forceinline__ void csf_rank_stage(uint16_t* rl, const uint16_t* __r
     if (g)
         for (int t = threadIdx.x; t < kRankTab; t += blockDim.x) rl[t] = g[t];
 }
-__device__ __forceinline__ uint32_t csf_rank2(uint32_t s, const uint16_t* rl, bool fast, int n,
-                                              const uint32_t* __restrict__ B) {
-    if (fast) {
-        const uint32_t h = s >> 8;
-        return uint32_t(rl[h]) + uint32_t(rl[256 + (__popc(h) << 8) + (s & 255u)]);
+// address of the determinant (a, b) in C[Ia * nbs + Ib]; rl: the staged byte tables (fast) or
+// unused (the per-electron rank with the binomial table B)
+struct CsfAddr {
+    const uint16_t* rl;
+    bool fast;
+    int n;
+    int64_t nbs;
+    const uint32_t* __restrict__ B;
+    __device__ __forceinline__ uint32_t rank(uint32_t s) const {
+        if (fast) {
+            const uint32_t h = s >> 8;
+            return uint32_t(rl[h]) + uint32_t(rl[256 + (__popc(h) << 8) + (s & 255u)]);
+        }
+        return csf_rank(s, n, B);
     }
-    return csf_rank(s, n, B);
-}
-// address of the determinant (a, b) in C[Ia * nbs + Ib]
-#define CSF_ADDR(a, b) (int64_t(csf_rank2((a), rl, fast, n, B)) * nbs + csf_rank2((b), rl, fast, n, B))
+    __device__ __forceinline__ int64_t operator()(uint32_t a, uint32_t b) const {
+        return int64_t(rank(a)) * nbs + rank(b);
+    }
+};
 
 // alpha / beta strings of the configuration's determinant with open-shell spin pattern m
 // (bit k of m set: the k-th lowest open shell holds a beta electron)
@@ -83,8 +92,7 @@ constexpr int kStageU = 8;
 template <class TI>
 __device__ __forceinline__ void csf_stage_dets(const TI* __restrict__ det, double* xs, int nc, int npad, int ndp,
                                                const uint32_t* __restrict__ docc, const uint32_t* __restrict__ socc,
-                                               const uint32_t* __restrict__ occ, const uint16_t* rl, bool fast, int n,
-                                               int64_t nbs, const uint32_t* __restrict__ B) {
+                                               const uint32_t* __restrict__ occ, const CsfAddr& addr_of) {
     const int valid = nc * ndp, total = npad * ndp;
     for (int t0 = threadIdx.x; t0 < total; t0 += kStageU * blockDim.x) {
         int64_t addr[kStageU];
@@ -99,7 +107,7 @@ __device__ __forceinline__ void csf_stage_dets(const TI* __restrict__ det, doubl
                 const uint32_t dw = docc[cc];
                 uint32_t a, b;
                 csf_strings(dw & ~kConfNeg, socc[cc], occ[j], a, b);
-                addr[u] = CSF_ADDR(a, b);
+                addr[u] = addr_of(a, b);
                 neg[u] = (dw & kConfNeg) != 0;
             }
         }
@@ -122,7 +130,7 @@ __global__ void k_csf_gather(const TI* __restrict__ det, TO* __restrict__ csf, c
                              const double* __restrict__ rv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
                              const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -135,8 +143,7 @@ __global__ void k_csf_gather(const TI* __restrict__ det, TO* __restrict__ csf, c
     for (int e = rp[i]; e < rp[i + 1]; ++e) {
         uint32_t a, b;
         csf_strings(d, so, occ[rj[e]], a, b);
-        const int64_t addr = CSF_ADDR(a, b);
-        acc += rv[e] * double(det[addr]);
+        acc += rv[e] * double(det[addr(a, b)]);
     }
     csf[off[c] + i] = TO((dw & kConfNeg) ? -acc : acc);
 }
@@ -164,14 +171,14 @@ __global__ __launch_bounds__(256) void k_csf_gather_lds(const TI* __restrict__ d
                                                         int n, int64_t nbs, const uint32_t* __restrict__ B,
                                                         const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     extern __shared__ double xs[];  // [cpb][ndp]
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
     const int r0 = blockIdx.y * rpb, r1 = min(ncsf, r0 + rpb);
-    csf_stage_dets(det, xs, nc, nc, ndp, docc + c0, socc + c0, occ, rl, fast, n, nbs, B);
+    csf_stage_dets(det, xs, nc, nc, ndp, docc + c0, socc + c0, occ, addr);
     __syncthreads();
     const int ng = (nc + kCsfCT - 1) / kCsfCT;
     if (mode == 2) {
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(256) void k_csf_gather_dense(const TI* __restrict__
                                                           const uint32_t* __restrict__ B,
                                                           const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     extern __shared__ double sh[];
@@ -284,7 +291,7 @@ __global__ __launch_bounds__(256) void k_csf_gather_dense(const TI* __restrict__
     const int64_t c0 = int64_t(blockIdx.x) * cpb;
     const int nc = int(min(int64_t(cpb), nconf - c0));
     csf_stage_table(T, td, ncsf, ndp, ts);
-    csf_stage_dets(det, X, nc, cpb, ndp, docc + c0, socc + c0, occ, rl, fast, n, nbs, B);
+    csf_stage_dets(det, X, nc, cpb, ndp, docc + c0, socc + c0, occ, addr);
     __syncthreads();
     // 2 x 2 tiles: configurations (cc, cc + 1) x rows (i, i + hi); consecutive lanes take
     // consecutive rows, whose odd LDS stride puts them in distinct banks
@@ -326,7 +333,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restr
                                                            const uint32_t* __restrict__ B,
                                                            const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     extern __shared__ double sh[];
@@ -368,7 +375,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_dense(const double* __restr
                 uint32_t a, b;
                 csf_strings(dw & ~kConfNeg, so, occ[r == 0 ? j : jb], a, b);
                 const double v = q == 0 ? (r == 0 ? a00 : a01) : (r == 0 ? a10 : a11);
-                det[CSF_ADDR(a, b)] = TO(v);
+                det[addr(a, b)] = TO(v);
             }
         }
     }
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
                                                          const uint32_t* __restrict__ B,
                                                          const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     extern __shared__ double ys[];  // [cpb][ncsf], signed by the configuration factor
@@ -438,7 +445,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
                 if (lane < kk) {
                     uint32_t a, b;
                     csf_strings(docc[c0 + cb + lane] & ~kConfNeg, socc[c0 + cb + lane], m, a, b);
-                    det[CSF_ADDR(a, b)] = TO(mine);
+                    det[addr(a, b)] = TO(mine);
                 }
             }
         }
@@ -464,7 +471,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_lds(const double* __restric
             if (k < kk) {
                 uint32_t a, b;
                 csf_strings(docc[c0 + cb + k] & ~kConfNeg, socc[c0 + cb + k], m, a, b);
-                det[CSF_ADDR(a, b)] = TO(acc[k]);
+                det[addr(a, b)] = TO(acc[k]);
             }
     }
 }
@@ -484,7 +491,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_cfg(const double* __restric
                                                          const uint32_t* __restrict__ B,
                                                          const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     extern __shared__ double ys[];  // [cpw][ncsf | 1], signed by the configuration factor
     const int ys_st = ncsf | 1;
@@ -523,7 +530,7 @@ __global__ __launch_bounds__(256) void k_csf_scatter_cfg(const double* __restric
         if (live && ok) {
             uint32_t a, b;
             csf_strings(dw & ~kConfNeg, so, occ[j], a, b);
-            det[CSF_ADDR(a, b)] = TO(acc);
+            det[addr(a, b)] = TO(acc);
         }
     }
 }
@@ -537,7 +544,7 @@ __global__ void k_csf_scatter(const double* __restrict__ csf, TO* __restrict__ d
                               const double* __restrict__ cv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
                               const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -550,7 +557,7 @@ __global__ void k_csf_scatter(const double* __restrict__ csf, TO* __restrict__ d
     for (int e = cp[j]; e < cp[j + 1]; ++e) acc += cv[e] * csf[o + ci[e]];
     uint32_t a, b;
     csf_strings(dw & ~kConfNeg, so, occ[j], a, b);
-    det[CSF_ADDR(a, b)] = TO((dw & kConfNeg) ? -acc : acc);
+    det[addr(a, b)] = TO((dw & kConfNeg) ? -acc : acc);
 }
 
 // FCISolver::form_Hdiag_csf without the full preconditioner (fci_solver_initial_guess.cc:134-141):
@@ -561,7 +568,7 @@ __global__ void k_csf_hdiag(const double* __restrict__ hd, double* __restrict__ 
                             const double* __restrict__ rv, const uint32_t* __restrict__ occ, int n, int64_t nbs,
                             const uint32_t* __restrict__ B, const uint16_t* __restrict__ RT) {
     __shared__ uint16_t rl[kRankTab];
-    const bool fast = RT != nullptr;
+    const CsfAddr addr{rl, RT != nullptr, n, nbs, B};
     csf_rank_stage(rl, RT);
     __syncthreads();
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -573,7 +580,7 @@ __global__ void k_csf_hdiag(const double* __restrict__ hd, double* __restrict__ 
     for (int e = rp[i]; e < rp[i + 1]; ++e) {
         uint32_t a, b;
         csf_strings(d, so, occ[rj[e]], a, b);
-        acc += rv[e] * rv[e] * hd[CSF_ADDR(a, b)];
+        acc += rv[e] * rv[e] * hd[addr(a, b)];
     }
     out[off[c] + i] = acc;
 }
