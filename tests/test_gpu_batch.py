@@ -106,6 +106,15 @@ def test_batch_rejects_unsupported_options():
         forte_amd._forte_amd.batch_solve([ints], [3], [3], 5, 1, {}, False)
 
 
+def test_batch_guess_errors_reach_the_caller():
+    """The guess construction runs over host threads for the problems of a class; an error there
+    (no determinant of 3 + 3 electrons in 6 orbitals has S = 4) comes back as the call's error."""
+    import forte_amd
+    ints = [forte_amd.make_active_space_integrals(*forte_amd.seeded_integrals(6, s)) for s in (1, 2, 3, 4)]
+    with pytest.raises(RuntimeError, match="multiplicity"):
+        forte_amd._forte_amd.batch_solve(ints, [3] * 4, [3] * 4, 1, 9, {}, False)
+
+
 @pytest.mark.parametrize("mult", [1, 3])
 def test_batch_spin_flip_symmetry(mult):
     """Symmetric classes (packed Davidson-Liu vectors, sigma without the beta same-spin term) against
