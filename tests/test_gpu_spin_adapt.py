@@ -185,13 +185,15 @@ def test_spin_adapted_matches_determinant_solve_and_rdms(oracle):
             assert np.abs(x - y).max() < 1e-7
 
 
-def test_spin_adapted_mixed_precision():
+@pytest.mark.parametrize("n,na,nb,mult", [(12, 6, 6, 1), (13, 7, 6, 2)])
+def test_spin_adapted_mixed_precision(n, na, nb, mult):
     """Mixed precision in the CSF basis (FP32 determinant sigma, FP64 refinement) reaches the FP64
-    CSF energy."""
-    p = seeded_problem(12, 6, 6, 5)
+    CSF energy; the 13-orbital doublet runs the FP32 transforms of classes with 11 open shells
+    (156 configurations: lanes over configurations) and 13 (one configuration)."""
+    p = seeded_problem(n, na, nb, 5)
     e = {}
     for prec in ("fp64", "mixed"):
-        s = solver(p, 1, 1)
+        s = solver(p, 1, mult)
         s.set_spin_adapt(True)
         s.set_precision(prec)
         s.set_e_convergence(1e-10)
