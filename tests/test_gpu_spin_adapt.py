@@ -72,6 +72,23 @@ def test_transforms_and_diagonal_vs_oracle(oracle, n, na, nb, twoS, orbsym, irre
     assert np.abs(s.hdiag_csf() - np.diag(T @ H @ T.T)).max() < 1e-11
 
 
+@pytest.mark.parametrize("n,na,nb,twoS", [(18, 2, 2, 0), (17, 3, 1, 2), (18, 3, 2, 1)])
+def test_transforms_beyond_16_orbitals_vs_oracle(n, na, nb, twoS):
+    """Above 16 orbitals the string ranks are summed electron by electron (the byte tables cover
+    16-bit strings): both transforms against the oracle's SpinAdapter."""
+    from oracle.spin_adapt import SpinAdapter
+    say(f"transforms n={n} na={na} nb={nb} 2S={twoS}")
+    p = seeded_problem(n, na, nb, 9)
+    s = solver(p, 1, twoS + 1)
+    ref = SpinAdapter(n, na, nb, twoS)
+    assert tuple(s.csf_size()) == (ref.ncsf, ref.ncoupling)
+    rng = np.random.default_rng(n + na + twoS)
+    d = rng.standard_normal(s.ndet())
+    c = rng.standard_normal(ref.ncsf)
+    assert np.abs(s.det_to_csf(d) - ref.det_to_csf(d)).max() < 1e-13
+    assert np.abs(s.csf_to_det(c) - ref.csf_to_det(c)).max() < 1e-13
+
+
 @pytest.mark.parametrize("n,na,nb,twoS", [(13, 7, 6, 1), (14, 7, 7, 0), (14, 7, 7, 2), (12, 8, 4, 4)])
 def test_transforms_of_large_open_shell_classes_sampled(n, na, nb, twoS):
     """Every open-shell class, including those of 10 to 14 open shells that the dense kernels do not
