@@ -256,6 +256,131 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab(const double* __restrict__ L,
     }
 }
 
+// alpha-beta 2-RDM on FP64 MFMA, block per alpha row Ja: out[e][z] = sum_Ib X[e][Ib] Z[z][Ib] with
+// X the rows of R reached from Ja by the alpha single excitations e (signed; one diagonal entry per
+// occupied orbital, as above) and Z[2P + dir] the row L[Ja] gathered through the beta excitation
+// of pair P in direction dir, for every pair at once: the alpha rows are gathered once per block
+// (not once per pair chunk) and each 16 x 16 output tile is a chain of 16x16x4 MFMAs over
+// 32-column chunks. LDS holds the chunks k-major with odd row strides, so the 16 lanes reading
+// one operand column take consecutive doubles.
+constexpr int RM_TW = 32;   // columns per chunk (the products' K)
+constexpr int RM_TPW = 24;  // output tiles per wave at most
+typedef double rd4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict__ L, const double* __restrict__ R,
+                                                        int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a,
+                                                        int ns, const uint32_t* __restrict__ str_a,
+                                                        const uint32_t* __restrict__ str_b,
+                                                        const uint32_t* __restrict__ out_b, int np, int n,
+                                                        double* __restrict__ g2ab) {
+    extern __shared__ double sm[];
+    __shared__ uint32_t ent[RD_MAXE];
+    __shared__ int ep[RD_MAXE], eq[RD_MAXE];
+    const int64_t Ja = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t occ = str_a[Ja];
+    const int ka = __popc(occ);
+    const int EF = ns + ka;
+    const int ET = (EF + 15) / 16, ZR = 2 * np, ZT = (ZR + 15) / 16;
+    const int XS = ET * 16 + 1, ZS = ZT * 16 + 1;  // k-major rows, odd strides
+    double* Xs = sm;                 // [RM_TW][XS]
+    double* Zs = sm + RM_TW * XS;    // [RM_TW][ZS]
+    for (int i = tid; i < EF; i += RD_BS) {
+        if (i < ns) {
+            const uint32_t t = into_a[Ja * ns + i];
+            ent[i] = t;
+            int hi, lo;
+            unpack_pair(int((t >> 21) & 0x3FFu), hi, lo);
+            const int p = ((occ >> hi) & 1u) ? hi : lo;
+            ep[i] = p;
+            eq[i] = p == hi ? lo : hi;
+        } else {
+            int j = i - ns, cnt = 0, q = 0;
+            for (q = 0; q < n; ++q)
+                if ((occ >> q) & 1u) {
+                    if (cnt == j) break;
+                    ++cnt;
+                }
+            ent[i] = uint32_t(Ja);
+            ep[i] = q;
+            eq[i] = q;
+        }
+    }
+    // padding rows of the operands stay zero for the whole block
+    for (int i = tid; i < RM_TW * XS; i += RD_BS) Xs[i] = 0.0;
+    for (int i = tid; i < RM_TW * ZS; i += RD_BS) Zs[i] = 0.0;
+    __syncthreads();
+    const int ntile = ET * ZT;
+    rd4 acc[RM_TPW];
+#pragma unroll
+    for (int t = 0; t < RM_TPW; ++t) acc[t] = rd4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t c0 = 0; c0 < Nb; c0 += RM_TW) {
+        for (int i = tid; i < EF * RM_TW; i += RD_BS) {
+            const int e = i / RM_TW, c = i % RM_TW;
+            double v = 0.0;
+            if (c0 + c < Nb) {
+                const uint32_t t = ent[e];
+                v = R[int64_t(t & kAddrMask) * Nb + c0 + c];
+                if (e < ns && (t >> 31)) v = -v;
+            }
+            Xs[c * XS + e] = v;
+        }
+        for (int i = tid; i < ZR * RM_TW; i += RD_BS) {
+            const int zr = i / RM_TW, c = i % RM_TW;
+            const int P = zr / 2, dir = zr % 2;
+            double v = 0.0;
+            const int64_t Ib = c0 + c;
+            if (Ib < Nb) {
+                const uint32_t t = out_b[int64_t(P) * Nb + Ib];
+                if (t != kInvalid) {
+                    int hi, lo;
+                    unpack_pair(P, hi, lo);
+                    const int d = (hi == lo) ? 0 : (((str_b[Ib] >> lo) & 1u) ? 0 : 1);
+                    if (d == dir) {
+                        v = L[Ja * Nb + (t & kAddrMask)];
+                        if (t >> 31) v = -v;
+                    }
+                }
+            }
+            Zs[c * ZS + zr] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < RM_TW / 4; ++ks) {
+            const int k = ks * 4 + (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < RM_TPW; ++t) {
+                const int tile = wave + 4 * t;
+                if (tile < ntile) {
+                    const int et = tile / ZT, zt = tile - et * ZT;
+                    const double a = Xs[k * XS + et * 16 + (lane & 15)];
+                    const double b = Zs[k * ZS + zt * 16 + (lane & 15)];
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D[row 4 v + lane / 16][column lane % 16] of each tile: row = excitation e, column = z
+#pragma unroll
+    for (int t = 0; t < RM_TPW; ++t) {
+        const int tile = wave + 4 * t;
+        if (tile >= ntile) continue;
+        const int et = tile / ZT, zt = tile - et * ZT;
+        const int zr = zt * 16 + (lane & 15);
+        if (zr >= ZR) continue;
+        const int P = zr / 2, dir = zr % 2;
+        int hi, lo;
+        unpack_pair(P, hi, lo);
+        if (hi == lo && dir == 1) continue;
+        const int r = dir == 0 ? hi : lo, s = dir == 0 ? lo : hi;  // create r, annihilate s
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int e = et * 16 + 4 * v + (lane >> 4);
+            if (e < EF) atomicAdd(&g2ab[((size_t(ep[e]) * n + r) * n + eq[e]) * n + s], acc[t][v]);
+        }
+    }
+}
+
 // <S^2> partial sums, block per alpha row Ja
 __global__ __launch_bounds__(RD_BS) void k_spin2(const double* __restrict__ L, const double* __restrict__ R,
                                                  int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a, int ns,
@@ -523,6 +648,16 @@ void rdm2_ab_device(const SpinTables& a, const SpinTables& b, const double* L, c
     HIPCHK(hipMemsetAsync(g2ab, 0, size_t(n) * n * n * n * sizeof(double), st));
     if (a.k == 0 || b.k == 0) return;
     if (a.ns + a.k > RD_MAXE) throw Error(-1, "rdm2ab: too many excitations per string");
+    // MFMA kernel when every output tile of a row has a wave slot and the chunks fit in LDS
+    const int ET = (a.ns + a.k + 15) / 16, ZT = (2 * b.np + 15) / 16;
+    const size_t shm = size_t(RM_TW) * (ET * 16 + 1 + ZT * 16 + 1) * sizeof(double);
+    if (ET * ZT <= 4 * RM_TPW && shm <= 150 * 1024) {
+        HIPCHK(hipFuncSetAttribute((const void*)k_rdm2_ab_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
+        k_rdm2_ab_mfma<<<unsigned(a.nstr), RD_BS, shm, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p,
+                                                             b.out.p, b.np, n, g2ab);
+        HIPCHK(hipGetLastError());
+        return;
+    }
     const int npc = (b.np + RA_PC - 1) / RA_PC;
     size_t sh = size_t(a.ns + a.k + 2 * RA_PC) * RA_TW * sizeof(double);
     k_rdm2_ab<<<unsigned(a.nstr * npc), RD_BS, sh, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p,
