@@ -263,15 +263,19 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab(const double* __restrict__ L,
 // (not once per pair chunk) and each 16 x 16 output tile is a chain of 16x16x4 MFMAs over
 // 32-column chunks. LDS holds the chunks k-major with odd row strides, so the 16 lanes reading
 // one operand column take consecutive doubles.
+// The staging loops issue RM_SU independent loads per thread before they store, and the row L[Ja]
+// is staged in LDS when it fits beside the chunks (lrow), so a chunk's gathers do not wait on one
+// dependent global load after another.
 constexpr int RM_TW = 32;   // columns per chunk (the products' K)
-constexpr int RM_TPW = 24;  // output tiles per wave at most
+constexpr int RM_SU = 8;    // loads in flight per thread while staging
 typedef double rd4 __attribute__((ext_vector_type(4)));
+template <int TPW>  // output tiles per wave at most
 __global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict__ L, const double* __restrict__ R,
                                                         int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a,
                                                         int ns, const uint32_t* __restrict__ str_a,
                                                         const uint32_t* __restrict__ str_b,
                                                         const uint32_t* __restrict__ out_b, int np, int n,
-                                                        double* __restrict__ g2ab) {
+                                                        double* __restrict__ g2ab, int lrow) {
     extern __shared__ double sm[];
     __shared__ uint32_t ent[RD_MAXE];
     __shared__ int ep[RD_MAXE], eq[RD_MAXE];
@@ -284,6 +288,12 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict
     const int XS = ET * 16 + 1, ZS = ZT * 16 + 1;  // k-major rows, odd strides
     double* Xs = sm;                 // [RM_TW][XS]
     double* Zs = sm + RM_TW * XS;    // [RM_TW][ZS]
+    double* Ls = Zs + RM_TW * ZS;    // [Nb] when lrow
+    const double* Lrow = L + Ja * Nb;
+    if (lrow) {
+        for (int64_t i = tid; i < Nb; i += RD_BS) Ls[i] = Lrow[i];
+        Lrow = Ls;
+    }
     for (int i = tid; i < EF; i += RD_BS) {
         if (i < ns) {
             const uint32_t t = into_a[Ja * ns + i];
@@ -310,45 +320,55 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict
     for (int i = tid; i < RM_TW * ZS; i += RD_BS) Zs[i] = 0.0;
     __syncthreads();
     const int ntile = ET * ZT;
-    rd4 acc[RM_TPW];
+    rd4 acc[TPW];
 #pragma unroll
-    for (int t = 0; t < RM_TPW; ++t) acc[t] = rd4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < TPW; ++t) acc[t] = rd4{0.0, 0.0, 0.0, 0.0};
     for (int64_t c0 = 0; c0 < Nb; c0 += RM_TW) {
-        for (int i = tid; i < EF * RM_TW; i += RD_BS) {
-            const int e = i / RM_TW, c = i % RM_TW;
-            double v = 0.0;
-            if (c0 + c < Nb) {
-                const uint32_t t = ent[e];
-                v = R[int64_t(t & kAddrMask) * Nb + c0 + c];
-                if (e < ns && (t >> 31)) v = -v;
+        for (int i0 = tid; i0 < EF * RM_TW; i0 += RM_SU * RD_BS) {
+            double v[RM_SU];
+#pragma unroll
+            for (int u = 0; u < RM_SU; ++u) {
+                const int i = i0 + u * RD_BS, e = i / RM_TW, c = i % RM_TW;
+                v[u] = 0.0;
+                if (i < EF * RM_TW && c0 + c < Nb) v[u] = R[int64_t(ent[e] & kAddrMask) * Nb + c0 + c];
             }
-            Xs[c * XS + e] = v;
+#pragma unroll
+            for (int u = 0; u < RM_SU; ++u) {
+                const int i = i0 + u * RD_BS, e = i / RM_TW, c = i % RM_TW;
+                if (i < EF * RM_TW) Xs[c * XS + e] = (e < ns && (ent[e] >> 31)) ? -v[u] : v[u];
+            }
         }
-        for (int i = tid; i < ZR * RM_TW; i += RD_BS) {
-            const int zr = i / RM_TW, c = i % RM_TW;
-            const int P = zr / 2, dir = zr % 2;
-            double v = 0.0;
-            const int64_t Ib = c0 + c;
-            if (Ib < Nb) {
-                const uint32_t t = out_b[int64_t(P) * Nb + Ib];
-                if (t != kInvalid) {
+        for (int i0 = tid; i0 < ZR * RM_TW; i0 += RM_SU * RD_BS) {
+            uint32_t t[RM_SU];
+#pragma unroll
+            for (int u = 0; u < RM_SU; ++u) {
+                const int i = i0 + u * RD_BS, zr = i / RM_TW, c = i % RM_TW;
+                t[u] = kInvalid;
+                if (i < ZR * RM_TW && c0 + c < Nb) {
+                    const int P = zr / 2, dir = zr % 2;
+                    const int64_t Ib = c0 + c;
                     int hi, lo;
                     unpack_pair(P, hi, lo);
+                    // valid direction annihilates the occupied orbital of the pair
                     const int d = (hi == lo) ? 0 : (((str_b[Ib] >> lo) & 1u) ? 0 : 1);
-                    if (d == dir) {
-                        v = L[Ja * Nb + (t & kAddrMask)];
-                        if (t >> 31) v = -v;
-                    }
+                    if (d == dir) t[u] = out_b[int64_t(P) * Nb + Ib];
                 }
             }
-            Zs[c * ZS + zr] = v;
+            double v[RM_SU];
+#pragma unroll
+            for (int u = 0; u < RM_SU; ++u) v[u] = t[u] != kInvalid ? Lrow[t[u] & kAddrMask] : 0.0;
+#pragma unroll
+            for (int u = 0; u < RM_SU; ++u) {
+                const int i = i0 + u * RD_BS, zr = i / RM_TW, c = i % RM_TW;
+                if (i < ZR * RM_TW) Zs[c * ZS + zr] = (t[u] != kInvalid && (t[u] >> 31)) ? -v[u] : v[u];
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < RM_TW / 4; ++ks) {
             const int k = ks * 4 + (lane >> 4);
 #pragma unroll
-            for (int t = 0; t < RM_TPW; ++t) {
+            for (int t = 0; t < TPW; ++t) {
                 const int tile = wave + 4 * t;
                 if (tile < ntile) {
                     const int et = tile / ZT, zt = tile - et * ZT;
@@ -362,7 +382,7 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict
     }
     // D[row 4 v + lane / 16][column lane % 16] of each tile: row = excitation e, column = z
 #pragma unroll
-    for (int t = 0; t < RM_TPW; ++t) {
+    for (int t = 0; t < TPW; ++t) {
         const int tile = wave + 4 * t;
         if (tile >= ntile) continue;
         const int et = tile / ZT, zt = tile - et * ZT;
@@ -651,10 +671,17 @@ void rdm2_ab_device(const SpinTables& a, const SpinTables& b, const double* L, c
     // MFMA kernel when every output tile of a row has a wave slot and the chunks fit in LDS
     const int ET = (a.ns + a.k + 15) / 16, ZT = (2 * b.np + 15) / 16;
     const size_t shm = size_t(RM_TW) * (ET * 16 + 1 + ZT * 16 + 1) * sizeof(double);
-    if (ET * ZT <= 4 * RM_TPW && shm <= 150 * 1024) {
-        HIPCHK(hipFuncSetAttribute((const void*)k_rdm2_ab_mfma, hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
-        k_rdm2_ab_mfma<<<unsigned(a.nstr), RD_BS, shm, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p,
-                                                             b.out.p, b.np, n, g2ab);
+    const size_t shl = shm + size_t(b.nstr) * sizeof(double);  // with the row L[Ja] staged
+    if (ET * ZT <= 4 * 24 && shm <= 150 * 1024) {  // 24: tiles per wave of the largest instance
+        const int lrow = shl <= 150 * 1024;
+        const size_t sh2 = lrow ? shl : shm;
+        auto run = [&](auto kern) {
+            HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(sh2)));
+            kern<<<unsigned(a.nstr), RD_BS, sh2, st>>>(L, R, a.nstr, b.nstr, a.into.p, a.ns, a.str.p, b.str.p, b.out.p,
+                                                       b.np, n, g2ab, lrow);
+        };
+        if (ET * ZT <= 4 * 16) run(k_rdm2_ab_mfma<16>);
+        else run(k_rdm2_ab_mfma<24>);
         HIPCHK(hipGetLastError());
         return;
     }
