@@ -26,6 +26,7 @@
 #include <cstring>
 #include <exception>
 #include <map>
+#include <system_error>
 #include <thread>
 
 namespace fci {
@@ -1061,7 +1062,11 @@ int fci_batch_solve(int nprob, const int* norb, const int* nalpha, const int* nb
                 };
                 const int nth = std::max(1, std::min<int>({P, 16, int(std::thread::hardware_concurrency())}));
                 std::vector<std::thread> pool;
-                for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+                try {
+                    for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+                } catch (const std::system_error&) {
+                    // fewer threads than asked for: the ones running and this one share the rest
+                }
                 work();
                 for (auto& t : pool) t.join();
                 for (auto& e : err)
