@@ -261,16 +261,18 @@ __global__ __launch_bounds__(RD_BS) void k_rdm2_ab(const double* __restrict__ L,
 // occupied orbital, as above) and Z[2P + dir] the row L[Ja] gathered through the beta excitation
 // of pair P in direction dir, for every pair at once: the alpha rows are gathered once per block
 // (not once per pair chunk) and each 16 x 16 output tile is a chain of 16x16x4 MFMAs over
-// 32-column chunks. LDS holds the chunks k-major with odd row strides, so the 16 lanes reading
+// 16-column chunks. LDS holds the chunks k-major with odd row strides, so the 16 lanes reading
 // one operand column take consecutive doubles.
 // The staging loops issue RM_SU independent loads per thread before they store, and the row L[Ja]
 // is staged in LDS when it fits beside the chunks (lrow), so a chunk's gathers do not wait on one
 // dependent global load after another.
-constexpr int RM_TW = 32;   // columns per chunk (the products' K)
+constexpr int RM_TW = 16;   // columns per chunk (the products' K)
 constexpr int RM_SU = 8;    // loads in flight per thread while staging
 typedef double rd4 __attribute__((ext_vector_type(4)));
-template <int TPW>  // output tiles per wave at most
-__global__ __launch_bounds__(RD_BS) void k_rdm2_ab_mfma(const double* __restrict__ L, const double* __restrict__ R,
+// TPW: output tiles per wave at most; the 16-tile instance fits 256 VGPRs, so two blocks share a CU
+// (their chunks and rows of L within half the LDS) and one stages while the other multiplies
+template <int TPW>
+__global__ __launch_bounds__(RD_BS, TPW <= 16 ? 2 : 1) void k_rdm2_ab_mfma(const double* __restrict__ L, const double* __restrict__ R,
                                                         int64_t Na, int64_t Nb, const uint32_t* __restrict__ into_a,
                                                         int ns, const uint32_t* __restrict__ str_a,
                                                         const uint32_t* __restrict__ str_b,
